@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""NCC all-pairs benchmark on one MI355X.
+
+Workload: 1024 synthetic patterns of 2048x1536 (16 cameras), all 523,776
+pairs as a triangular Gram on MFMA.  One step = per-pattern FP64 statistics +
+the split-BF16 Gram + the FP64 NCC epilogue over every pair, with the patterns
+already resident in HBM.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--precision split_bf16|fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HEIGHT, WIDTH, COUNT, CAMERAS = 1536, 2048, 1024, 16
+PEAK_TFLOPS = {"split_bf16": 2500.0, "fp32": 157.3, "fp8": 5000.0}
+PRODUCTS = {"split_bf16": 3, "fp32": 1, "fp8": 1}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="split_bf16", choices=["split_bf16", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison leg")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_patterns(torch):
+    """Zero-mean FP32 sensor-noise patterns: 0.02 * K_c + N(0, 1), seeded."""
+    g = torch.Generator(device="cuda").manual_seed(20261202)
+    n = HEIGHT * WIDTH
+    k = torch.randn((CAMERAS, n), generator=g, device="cuda")
+    p = torch.empty((COUNT, n), dtype=torch.float32, device="cuda")
+    chunk = 64
+    for b in range(0, COUNT, chunk):
+        e = min(COUNT, b + chunk)
+        blk = torch.randn((e - b, n), generator=g, device="cuda")
+        blk += 0.02 * k[torch.arange(b, e, device="cuda") % CAMERAS]
+        blk -= blk.mean(dim=1, keepdim=True)
+        p[b:e] = blk
+    del k
+    return p
+
+
+def fp64_check(torch, p, ncc, sample=48):
+    """max |dNCC| of `ncc` on a random sample of patterns against FP64 dot products on the host."""
+    idx = np.sort(np.random.default_rng(7).choice(COUNT, sample, replace=False))
+    host = p[torch.as_tensor(idx, device="cuda")].cpu().numpy().astype(np.float64)
+    g = host @ host.T
+    s = host.sum(axis=1)
+    d = np.sqrt(np.einsum("ij,ij->i", host, host) - s * s)
+    want = g / np.outer(d, d)
+    return float(np.abs(ncc[np.ix_(idx, idx)] - want).max())
+
+
+def load_traffic(kernel: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel)
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(torch, p):
+    """The C restatement of NormalizedCrossCorrelation.applyCPU, all host cores,
+    on all pairs of a 48-pattern sample of the same set."""
+    lib_path = os.path.join(ROOT, "oracle", "_build", "libprnu_cpu_port.so")
+    if not os.path.exists(lib_path):
+        return None
+    port = ctypes.CDLL(lib_path)
+    port.port_ncc_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]
+    port.port_sumsq_many.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]
+    port.port_max_threads.restype = ctypes.c_int
+    sample = 48
+    host = np.ascontiguousarray(p[:sample].cpu().numpy())
+    n = host.shape[1]
+    ptrs = (ctypes.c_void_p * sample)(*[host[i].ctypes.data for i in range(sample)])
+    sumsq = np.zeros(sample)
+    threads = port.port_max_threads()
+    port.port_sumsq_many(ptrs, sample, n, sumsq.ctypes.data, threads)
+    uniq = np.array([(i, j) for i in range(sample) for j in range(i + 1, sample)], dtype=np.int32)
+    out = np.zeros(len(uniq))
+    t0 = time.perf_counter()
+    port.port_ncc_pairs(ptrs, sumsq.ctypes.data, uniq[:256].ctypes.data, 256, n, out.ctypes.data, threads)
+    rate0 = 256 / (time.perf_counter() - t0)
+    # about 12 s of CPU work: cycle through the sample's pairs
+    count = int(max(len(uniq), min(rate0 * 12.0, 50 * len(uniq))))
+    pairs = np.ascontiguousarray(np.resize(uniq, (count, 2)))
+    out = np.zeros(count)
+    t0 = time.perf_counter()
+    port.port_ncc_pairs(ptrs, sumsq.ctypes.data, pairs.ctypes.data, count, n, out.ctypes.data, threads)
+    dt = time.perf_counter() - t0
+    return {"value": count / dt, "unit": "pairs/s", "cores": int(threads), "kind": "port",
+            "sample": f"{count} pair evaluations over the {len(uniq)} pairs of {sample} of the 1024 patterns "
+                      f"(2048x1536), C port of NormalizedCrossCorrelation.applyCPU, OpenMP, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    from rocket_amd import _lib, csi
+
+    lib = _lib.load()
+    prec = args.precision
+    t0 = time.time()
+    p = make_patterns(torch)
+    torch.cuda.synchronize()
+    log(f"patterns ready ({COUNT}x{HEIGHT}x{WIDTH} FP32, {p.numel() * 4 / 1e9:.1f} GB) in {time.time() - t0:.1f} s")
+
+    gram = csi.GramNCC(HEIGHT, WIDTH)
+    ptrs = csi.pattern_pointer_table(p)
+    out = torch.empty((COUNT, COUNT), dtype=torch.float64, device="cuda")
+    ctx = gram._ctx.handle
+    _lib.check(lib.prnu_ctx_set_timing(ctx, 1))
+
+    def step(precision):
+        stats = gram.stats(ptrs, COUNT, sync=False)
+        gram.ncc(precision=precision, stats=stats, out=out, sync=False, ptrs=ptrs)
+        return stats
+
+    for _ in range(args.warmup):
+        step(prec)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    ms = ctypes.c_float()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step(prec)
+        _lib.check(lib.prnu_ctx_last_kernel_ms(ctx, ctypes.byref(ms)))
+        kernel_ms.append(ms.value)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * elapsed / args.steps
+    pairs = COUNT * (COUNT - 1) // 2
+    value = pairs / (ms_per_step / 1000.0)
+    ncc_main = out.cpu().numpy()
+    log(f"{prec}: {ms_per_step:.3f} ms/step, {value:.4g} pairs/s, tile kernel {np.mean(kernel_ms):.3f} ms")
+
+    dncc = fp64_check(torch, p, ncc_main)
+    log(f"max |dNCC| vs FP64 on a 48-pattern sample: {dncc:.3e}")
+
+    extra = {}
+    if not args.no_fp32 and prec != "fp32":
+        for _ in range(2):
+            step("fp32")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        n32 = max(3, args.steps // 4)
+        k32 = []
+        for _ in range(n32):
+            step("fp32")
+            _lib.check(lib.prnu_ctx_last_kernel_ms(ctx, ctypes.byref(ms)))
+            k32.append(ms.value)
+        torch.cuda.synchronize()
+        ms32 = 1000.0 * (time.perf_counter() - t1) / n32
+        d32 = fp64_check(torch, p, out.cpu().numpy())
+        extra = {"fp32_pairs_per_s": pairs / (ms32 / 1000.0), "fp32_ms_per_step": ms32,
+                 "fp32_tile_kernel_ms": float(np.mean(k32)), "fp32_max_abs_dncc": d32,
+                 "speedup_vs_fp32": ms32 / ms_per_step}
+        log(f"fp32: {ms32:.3f} ms/step; split-BF16 speedup {ms32 / ms_per_step:.2f}x")
+
+    # roofline of the dominant kernel (the Gram tile kernel)
+    one_product = 2.0 * HEIGHT * WIDTH * COUNT * (COUNT + 1) / 2.0
+    flops = one_product * PRODUCTS[prec]
+    kms = float(np.mean(kernel_ms))
+    achieved = flops / (kms / 1000.0) / 1e12
+    kernel_name = f"gram_tile_kernel<{1 if prec == 'split_bf16' else 0}>"
+    traffic = load_traffic(kernel_name)
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic, "kernel": kernel_name,
+                "kernel_ms": kms, "algorithmic_flop": flops}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(torch, p)
+        if cpu:
+            log(f"CPU: {cpu['value']:.4g} pairs/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "NCC pairs/s, split-BF16 MFMA vs FP32 at |dNCC|<=1e-5; query p50/p99 latency",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if prec == "split_bf16" else "fp32",
+        "data": "synthetic: zero-mean FP32 patterns 0.02*K_c + N(0,1), 16 cameras, seeded",
+        "config": {"workload": "ncc1024_2048x1536", "patterns": COUNT, "height": HEIGHT, "width": WIDTH,
+                   "pairs": pairs, "precision": prec, "parallelism": "single GPU"},
+        "max_abs_dncc_vs_fp64": dncc,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    gram.close()
+
+
+if __name__ == "__main__":
+    main()

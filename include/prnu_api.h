@@ -1,0 +1,191 @@
+/*
+ * prnu_api.h -- C ABI of libprnu_hip.so, the MI355X (gfx950) implementation of
+ * Rocket's PRNU common-source-identification hot path.
+ *
+ * Reference paths are relative to src/main/java/nl/esciencecenter/ of
+ * JungleComputing/rocket:  CSI/ = common_source_identification/,
+ * R/ = rocket/.
+ *
+ * Conventions
+ *  - Every call returns an int status (prnu_status).  On failure a message is
+ *    kept per thread and returned by prnu_last_error().  No C++ exception
+ *    crosses this boundary.  PRNU_EINVAL is what the reference raises as
+ *    IllegalArgumentException (undersized buffer, bad size); the other codes
+ *    map to its CudaException.
+ *  - Device pointers are plain addresses in the current HIP device's memory.
+ *  - `stream` is a hipStream_t.  NULL selects the context's private stream and
+ *    makes the call synchronous, as every reference GPU method is
+ *    (CSI/kernels/compare/NormalizedCrossCorrelation.java:111-113,
+ *    CSI/kernels/filter/PRNUFilter.java:135-137).  A non-NULL stream makes the
+ *    call asynchronous on that stream.
+ *  - A context is not re-entrant: calls on one context must be serialised by
+ *    the caller (the reference calls them from one device thread,
+ *    R/scheduler/DeviceWorker.java:94-95).  Separate contexts may be used from
+ *    separate threads.
+ *  - The context owns its workspaces and twiddle tables; buffers passed in are
+ *    borrowed for the duration of the call and never freed by the library.
+ */
+#ifndef PRNU_API_H
+#define PRNU_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRNU_API_VERSION 1
+
+#if defined(__GNUC__) || defined(__clang__)
+#define PRNU_EXPORT __attribute__((visibility("default")))
+#else
+#define PRNU_EXPORT
+#endif
+
+typedef struct prnu_ctx prnu_ctx;
+typedef void *prnu_stream; /* hipStream_t */
+
+typedef enum {
+    PRNU_OK = 0,
+    PRNU_EINVAL = 1,   /* bad argument / undersized buffer */
+    PRNU_ENOMEM = 2,   /* device or host allocation failed */
+    PRNU_EDEVICE = 3,  /* HIP runtime error */
+    PRNU_ENOTSUP = 4   /* valid request this build does not implement */
+} prnu_status;
+
+typedef enum { PRNU_MODE_NCC = 0, PRNU_MODE_PCE = 1 } prnu_mode;
+
+typedef enum {
+    PRNU_INPUT_RGB8 = 0,   /* packed 8-bit, 3 bytes per pixel, libjpeg order */
+    PRNU_INPUT_GRAY8 = 1,  /* 8-bit gray, treated as R = G = B */
+    PRNU_INPUT_FLOAT32 = 2 /* already-gray FP32 image */
+} prnu_input_format;
+
+typedef enum {
+    PRNU_PREC_FP32 = 0,       /* FP32 MFMA (v_mfma_f32_16x16x4_f32) */
+    PRNU_PREC_SPLIT_BF16 = 1, /* hi+lo BF16, three MFMA products, FP32 accumulate */
+    PRNU_PREC_FP8 = 2         /* e4m3 with per-pattern scale, FP32 accumulate */
+} prnu_precision;
+
+typedef enum {
+    PRNU_PCE_REFERENCE = 0, /* the reference's semantics, quirks included */
+    PRNU_PCE_TEXTBOOK = 1   /* true cross spectrum, |c| peak, 11x11 cyclic window */
+} prnu_pce_variant;
+
+/* ------------------------------------------------------------------ errors */
+/* Message of the last failed call on this thread ("" if none). */
+PRNU_EXPORT const char *prnu_last_error(void);
+PRNU_EXPORT int prnu_api_version(void);
+
+/* --------------------------------------------- memory and streams
+ * These replace the JCuda wrapper the reference's host reaches HIP through:
+ * R/cubaapi/CudaContext.java (allocBytes/allocFloats, getMemInfo),
+ * R/cubaapi/CudaMem.java (copyFromHostAsync/copyToHostAsync/copyToDeviceAsync),
+ * R/cubaapi/CudaPinned.java, R/cubaapi/CudaStream.java:28-31. */
+PRNU_EXPORT int prnu_device_count(int *count);
+PRNU_EXPORT int prnu_set_device(int device);
+PRNU_EXPORT int prnu_mem_info(size_t *free_bytes, size_t *total_bytes);
+PRNU_EXPORT int prnu_malloc_device(void **ptr, size_t bytes);
+PRNU_EXPORT int prnu_free_device(void *ptr);
+PRNU_EXPORT int prnu_malloc_pinned(void **ptr, size_t bytes);
+PRNU_EXPORT int prnu_free_pinned(void *ptr);
+PRNU_EXPORT int prnu_memcpy_h2d(void *dst, const void *src, size_t bytes, prnu_stream stream);
+PRNU_EXPORT int prnu_memcpy_d2h(void *dst, const void *src, size_t bytes, prnu_stream stream);
+PRNU_EXPORT int prnu_memcpy_d2d(void *dst, const void *src, size_t bytes, prnu_stream stream);
+PRNU_EXPORT int prnu_stream_create(prnu_stream *stream);
+PRNU_EXPORT int prnu_stream_destroy(prnu_stream stream);
+PRNU_EXPORT int prnu_stream_synchronize(prnu_stream stream);
+
+/* ------------------------------------------------------------------ context
+ * CommonSourceIdentificationContext(CudaContext, Dimension, ComparisonStrategy)
+ * (CSI/CommonSourceIdentificationContext.java:32-61): a PRNU filter for one
+ * image size plus the comparator of `mode` (prnu_mode).  h, w >= 2. */
+PRNU_EXPORT int prnu_ctx_create(int height, int width, int mode, prnu_ctx **out);
+PRNU_EXPORT int prnu_ctx_destroy(prnu_ctx *ctx);
+/* getMaxFileSize = 3*h*w (CommonSourceIdentificationContext.java:64-66) */
+PRNU_EXPORT int64_t prnu_max_file_size(const prnu_ctx *ctx);
+/* getMaxInputSize = pattern bytes: 4*h*w (NCC) or 8*h*(w/2+1) (PCE)
+ * (CommonSourceIdentificationContext.java:69-71, PRNUFilter.java:103-112) */
+PRNU_EXPORT int64_t prnu_max_input_size(const prnu_ctx *ctx);
+/* getMaxOutputSize = 8 (CommonSourceIdentificationContext.java:74-76) */
+PRNU_EXPORT int64_t prnu_max_output_size(const prnu_ctx *ctx);
+
+/* ------------------------------------------------- drop-in per item / pair
+ * preprocessInputGPU(K, CudaMem input, CudaMem output)
+ * (R/scheduler/ApplicationContext.java:66, CommonSourceIdentificationContext.java:105-108,
+ * PRNUFilter.applyGPU at CSI/kernels/filter/PRNUFilter.java:124-138):
+ * image -> PRNU pattern (NCC mode: FP32 h*w) or its R2C spectrum (PCE mode:
+ * complex FP32 h*(w/2+1)).  `input_format` is a prnu_input_format; in_bytes
+ * must be at least h*w*{3,1,4}.  *out_bytes receives the pattern size. */
+PRNU_EXPORT int prnu_preprocess(prnu_ctx *ctx, int input_format, const void *d_in, size_t in_bytes,
+                    void *d_out, size_t out_capacity, size_t *out_bytes, prnu_stream stream);
+
+/* correlateGPU(K, CudaMem left, K, CudaMem right, CudaMem output)
+ * (R/scheduler/ApplicationContext.java:81, CommonSourceIdentificationContext.java:111-115,
+ * PatternComparator.applyGPU at CSI/kernels/compare/PatternComparator.java:45):
+ * one FP64 score into d_out.  NCC mode follows
+ * CSI/kernels/compare/normalizedcrosscorrelation.cu:85-178 (FP32 products,
+ * FP64 sums); PCE mode follows CSI/kernels/compare/peaktocorrelationenergy.cu
+ * (reference variant).  Inputs are read only. */
+PRNU_EXPORT int prnu_correlate(prnu_ctx *ctx, const void *d_left, size_t left_bytes,
+                   const void *d_right, size_t right_bytes,
+                   void *d_out, size_t out_capacity, size_t *out_bytes, prnu_stream stream);
+
+/* PCE of one pair with a chosen variant (prnu_pce_variant); PCE contexts only.
+ * PeakToCorrelationEnergy.compare (CSI/kernels/compare/PeakToCorrelationEnergy.java:202-232). */
+PRNU_EXPORT int prnu_correlate_pce(prnu_ctx *ctx, int variant, const void *d_left, const void *d_right,
+                       double *d_out, prnu_stream stream);
+
+/* ------------------------------------------------------- single filters
+ * The individual filter objects of PRNUFilter (PRNUFilter.java:157-175), for
+ * testing each stage.  All work in place on an FP32 h*w image except
+ * grayscale (in -> FP32) and spectral (FP32 -> complex h*(w/2+1)). */
+/* GrayscaleFilter.applyGPU, CSI/kernels/filter/GrayscaleFilter.java:72-85 */
+PRNU_EXPORT int prnu_filter_grayscale(prnu_ctx *ctx, int input_format, const void *d_in, size_t in_bytes,
+                          float *d_out, prnu_stream stream);
+/* FastNoiseFilter.applyGPU, CSI/kernels/filter/FastNoiseFilter.java:98-120 */
+PRNU_EXPORT int prnu_filter_fastnoise(prnu_ctx *ctx, float *d_image, prnu_stream stream);
+/* ZeroMeanTotalFilter.applyGPU, CSI/kernels/filter/ZeroMeanTotalFilter.java:77-99 */
+PRNU_EXPORT int prnu_filter_zeromean(prnu_ctx *ctx, float *d_image, prnu_stream stream);
+/* WienerFilter.applyGPU, CSI/kernels/filter/WienerFilter.java:187-240 */
+PRNU_EXPORT int prnu_filter_wiener(prnu_ctx *ctx, float *d_image, prnu_stream stream);
+/* SpectralFilter.applyGPU, CSI/kernels/filter/SpectralFilter.java:44-54 */
+PRNU_EXPORT int prnu_filter_spectral(prnu_ctx *ctx, const float *d_image, float *d_out, size_t out_capacity,
+                         prnu_stream stream);
+
+/* ------------------------------------------------------ batched extensions
+ * The reference has no batched entry points: it calls correlateGPU once per
+ * pair.  These replace M(M-1)/2 such calls.
+ *
+ * Per-pattern statistics: d_sum[i] = sum x, d_sumsq[i] = sum fl32(x*x) in
+ * FP64 (the per-pattern half of normalizedcrosscorrelation.cu:153-161), and
+ * optionally d_absmax[i] = max |x| (FP32, for FP8 scaling).
+ * d_patterns is a DEVICE array of `count` device pointers, each to n floats. */
+PRNU_EXPORT int prnu_pattern_stats(const float *const *d_patterns, int count, int64_t n,
+                       double *d_sum, double *d_sumsq, float *d_absmax, prnu_stream stream);
+
+/* All-pairs NCC of `count` patterns of the context's size as a triangular
+ * Gram on MFMA (`precision` is a prnu_precision), with the FP64 epilogue
+ * NCC = G_ij / (sqrt(Sxx_i - Sx_i^2) sqrt(Sxx_j - Sx_j^2)) of
+ * normalizedcrosscorrelation.cu:121-123.  d_sum/d_sumsq (and d_absmax for
+ * FP8) come from prnu_pattern_stats.  d_ncc is count*count doubles,
+ * row-major; both triangles and the diagonal are written.
+ * Replaces count*(count-1)/2 calls of correlateGPU in NCC mode. */
+PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int count, int precision,
+                  const double *d_sum, const double *d_sumsq, const float *d_absmax,
+                  double *d_ncc, prnu_stream stream);
+
+/* --------------------------------------------------------------- timing
+ * No reference counterpart (the reference times whole tasks on the host,
+ * R/RocketLauncher.java:115-203).  When enabled, the context records HIP
+ * events around the dominant kernel of each batched call, on the stream that
+ * kernel runs on; prnu_ctx_last_kernel_ms waits for the end event and returns
+ * the elapsed milliseconds of the last such call. */
+PRNU_EXPORT int prnu_ctx_set_timing(prnu_ctx *ctx, int enable);
+PRNU_EXPORT int prnu_ctx_last_kernel_ms(prnu_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRNU_API_H */

@@ -1,0 +1,371 @@
+"""Host-side mirror of the reference's CSI plugin for this path, over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference's Java
+classes (paths relative to
+``src/main/java/nl/esciencecenter/common_source_identification/``):
+
+* :class:`CommonSourceIdentificationContext` -- ``CommonSourceIdentificationContext.java:20-127``
+  (``ApplicationContext<ImageIdentifier, Double>``, ``R/scheduler/ApplicationContext.java:11-99``)
+* :class:`PRNUFilter` -- ``kernels/filter/PRNUFilter.java:41-194``
+* :class:`NormalizedCrossCorrelation`, :class:`PeakToCorrelationEnergy` --
+  ``kernels/compare/*.java`` behind :class:`PatternComparator`
+  (``kernels/compare/PatternComparator.java:34-54``)
+* :class:`GramNCC` -- the batched all-pairs extension (no reference counterpart:
+  it replaces M(M-1)/2 ``correlateGPU`` calls).
+
+Device buffers are torch CUDA tensors (PyTorch is only used for device memory
+and streams here); every call runs on torch's current stream and, like the
+reference's GPU methods, returns after that stream is synchronised unless
+``sync=False``.  Undersized buffers raise :class:`ValueError`
+(``IllegalArgumentException`` in the reference).
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import struct
+
+from . import _lib
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+class ComparisonStrategy(enum.Enum):
+    """``CommonSourceIdentificationContext.ComparisonStrategy`` (``:23-26``)."""
+    PCE = _lib.MODE_PCE
+    NCC = _lib.MODE_NCC
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _mem(t) -> tuple[int, int]:
+    """(device address, size in bytes) of a CUDA tensor (the CudaMem analogue)."""
+    if not (hasattr(t, "is_cuda") and t.is_cuda):
+        raise ValueError("device buffer must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise ValueError("device buffer must be contiguous")
+    return t.data_ptr(), t.numel() * t.element_size()
+
+
+def _sync(sync: bool) -> None:
+    if sync:
+        torch.cuda.current_stream().synchronize()
+
+
+class _Ctx:
+    """Owns one ``prnu_ctx``."""
+
+    def __init__(self, height: int, width: int, mode: int):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.prnu_ctx_create(int(height), int(width), int(mode), ctypes.byref(h)), "prnu_ctx_create")
+        self.handle = h
+        self.height = int(height)
+        self.width = int(width)
+        self.mode = int(mode)
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().prnu_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ filters
+class _FilterStage:
+    def __init__(self, ctx: _Ctx):
+        self._ctx = ctx
+
+    def cleanup(self) -> None:
+        pass
+
+
+class GrayscaleFilter(_FilterStage):
+    """``kernels/filter/GrayscaleFilter.java:72-85``."""
+
+    def apply_gpu(self, image, output, input_format: int = _lib.INPUT_RGB8, sync: bool = True) -> None:
+        ip, ib = _mem(image)
+        op, ob = _mem(output)
+        if ob < 4 * self._ctx.height * self._ctx.width:
+            raise ValueError("output smaller than h*w floats")
+        _lib.call("prnu_filter_grayscale", self._ctx.handle, input_format, ip, ib, op, _stream())
+        _sync(sync)
+
+
+class FastNoiseFilter(_FilterStage):
+    """``kernels/filter/FastNoiseFilter.java:98-120`` (in place)."""
+
+    def apply_gpu(self, image, sync: bool = True) -> None:
+        p, b = _mem(image)
+        if b < 4 * self._ctx.height * self._ctx.width:
+            raise ValueError("image smaller than h*w floats")
+        _lib.call("prnu_filter_fastnoise", self._ctx.handle, p, _stream())
+        _sync(sync)
+
+
+class ZeroMeanTotalFilter(_FilterStage):
+    """``kernels/filter/ZeroMeanTotalFilter.java:77-99`` (in place)."""
+
+    def apply_gpu(self, image, sync: bool = True) -> None:
+        p, b = _mem(image)
+        if b < 4 * self._ctx.height * self._ctx.width:
+            raise ValueError("image smaller than h*w floats")
+        _lib.call("prnu_filter_zeromean", self._ctx.handle, p, _stream())
+        _sync(sync)
+
+
+class WienerFilter(_FilterStage):
+    """``kernels/filter/WienerFilter.java:187-240`` (in place)."""
+
+    def apply_gpu(self, image, sync: bool = True) -> None:
+        p, b = _mem(image)
+        if b < 4 * self._ctx.height * self._ctx.width:
+            raise ValueError("image smaller than h*w floats")
+        _lib.call("prnu_filter_wiener", self._ctx.handle, p, _stream())
+        _sync(sync)
+
+
+class SpectralFilter(_FilterStage):
+    """``kernels/filter/SpectralFilter.java:44-54``: R2C half spectrum."""
+
+    def apply_gpu(self, image, output, sync: bool = True) -> None:
+        ip, _ = _mem(image)
+        op, ob = _mem(output)
+        _lib.call("prnu_filter_spectral", self._ctx.handle, ip, op, ob, _stream())
+        _sync(sync)
+
+
+class PRNUFilter:
+    """``kernels/filter/PRNUFilter.java:41-194``: gray -> FastNoise -> ZeroMean
+    -> Wiener (-> R2C spectrum when ``apply_spectral_filter``)."""
+
+    def __init__(self, height: int, width: int, apply_spectral_filter: bool = False, _ctx: _Ctx | None = None):
+        self.h = int(height)
+        self.w = int(width)
+        self.apply_spectral_filter = bool(apply_spectral_filter)
+        mode = _lib.MODE_PCE if apply_spectral_filter else _lib.MODE_NCC
+        self._ctx = _ctx if _ctx is not None else _Ctx(height, width, mode)
+        self._own = _ctx is None
+        self.grayscale_filter = GrayscaleFilter(self._ctx)
+        self.fast_noise_filter = FastNoiseFilter(self._ctx)
+        self.zero_mean_total_filter = ZeroMeanTotalFilter(self._ctx)
+        self.wiener_filter = WienerFilter(self._ctx)
+        self.spectral_filter = SpectralFilter(self._ctx)
+
+    def get_input_size(self) -> int:
+        return self.h * self.w * 3
+
+    def get_output_size(self) -> int:
+        n = 2 * self.h * (self.w // 2 + 1) if self.apply_spectral_filter else self.h * self.w
+        return 4 * n
+
+    def apply_gpu(self, image, output, input_format: int = _lib.INPUT_RGB8, sync: bool = True) -> int:
+        ip, ib = _mem(image)
+        op, ob = _mem(output)
+        out_bytes = ctypes.c_size_t()
+        _lib.call("prnu_preprocess", self._ctx.handle, input_format, ip, ib, op, ob, ctypes.byref(out_bytes), _stream())
+        _sync(sync)
+        return out_bytes.value
+
+    # getters, PRNUFilter.java:157-175
+    def get_grayscale_filter(self):
+        return self.grayscale_filter
+
+    def get_fast_noise_filter(self):
+        return self.fast_noise_filter
+
+    def get_zero_mean_total_filter(self):
+        return self.zero_mean_total_filter
+
+    def get_wiener_filter(self):
+        return self.wiener_filter
+
+    def get_spectral_filter(self):
+        return self.spectral_filter
+
+    def cleanup(self) -> None:
+        if self._own:
+            self._ctx.close()
+
+
+# -------------------------------------------------------------- comparators
+class PatternComparator:
+    """``kernels/compare/PatternComparator.java:34-54``."""
+
+    def apply_gpu(self, left, right, result, sync: bool = True) -> None:
+        raise NotImplementedError
+
+    def cleanup(self) -> None:
+        raise NotImplementedError
+
+
+class NormalizedCrossCorrelation(PatternComparator):
+    """``kernels/compare/NormalizedCrossCorrelation.java:18-145``."""
+
+    def __init__(self, h: int, w: int, _ctx: _Ctx | None = None):
+        self.h, self.w = int(h), int(w)
+        self._ctx = _ctx if _ctx is not None else _Ctx(h, w, _lib.MODE_NCC)
+        self._own = _ctx is None
+
+    def get_input_size(self) -> int:
+        return 4 * self.h * self.w
+
+    def get_output_size(self) -> int:
+        return 8
+
+    def apply_gpu(self, left, right, result, sync: bool = True) -> None:
+        lp, lb = _mem(left)
+        rp, rb = _mem(right)
+        op, ob = _mem(result)
+        out = ctypes.c_size_t()
+        _lib.call("prnu_correlate", self._ctx.handle, lp, lb, rp, rb, op, ob, ctypes.byref(out), _stream())
+        _sync(sync)
+
+    def cleanup(self) -> None:
+        if self._own:
+            self._ctx.close()
+
+
+class PeakToCorrelationEnergy(PatternComparator):
+    """``kernels/compare/PeakToCorrelationEnergy.java:36-263``.  ``variant``
+    selects the reference semantics or the textbook PCE."""
+
+    def __init__(self, h: int, w: int, variant: int = _lib.PCE_REFERENCE, _ctx: _Ctx | None = None):
+        self.h, self.w = int(h), int(w)
+        self.variant = int(variant)
+        self._ctx = _ctx if _ctx is not None else _Ctx(h, w, _lib.MODE_PCE)
+        self._own = _ctx is None
+
+    def get_input_size(self) -> int:
+        return 4 * 2 * self.h * (self.w // 2 + 1)
+
+    def get_output_size(self) -> int:
+        return 8
+
+    def apply_gpu(self, left, right, result, sync: bool = True) -> None:
+        lp, lb = _mem(left)
+        rp, rb = _mem(right)
+        op, ob = _mem(result)
+        if lb < self.get_input_size() or rb < self.get_input_size() or ob < 8:
+            raise ValueError("buffer smaller than the spectrum or the result")
+        _lib.call("prnu_correlate_pce", self._ctx.handle, self.variant, lp, rp, op, _stream())
+        _sync(sync)
+
+    def compare(self, left, right, result, sync: bool = True) -> None:
+        self.apply_gpu(left, right, result, sync)
+
+    def cleanup(self) -> None:
+        if self._own:
+            self._ctx.close()
+
+
+# ------------------------------------------------------------------ context
+class CommonSourceIdentificationContext:
+    """``CommonSourceIdentificationContext.java:20-127`` over the C ABI."""
+
+    def __init__(self, height: int, width: int, comp: ComparisonStrategy = ComparisonStrategy.NCC):
+        if not isinstance(comp, ComparisonStrategy):
+            raise ValueError(f"unknown comparison strategy: {comp}")
+        self.height, self.width = int(height), int(width)
+        self.comp = comp
+        self._ctx = _Ctx(height, width, comp.value)
+        self.filter = PRNUFilter(height, width, comp is ComparisonStrategy.PCE, _ctx=self._ctx)
+        if comp is ComparisonStrategy.PCE:
+            self.comparator = PeakToCorrelationEnergy(height, width, _ctx=self._ctx)
+        else:
+            self.comparator = NormalizedCrossCorrelation(height, width, _ctx=self._ctx)
+
+    def get_max_file_size(self) -> int:
+        return int(_lib.load().prnu_max_file_size(self._ctx.handle))
+
+    def get_max_input_size(self) -> int:
+        return int(_lib.load().prnu_max_input_size(self._ctx.handle))
+
+    def get_max_output_size(self) -> int:
+        return int(_lib.load().prnu_max_output_size(self._ctx.handle))
+
+    def preprocess_input_gpu(self, key, input, output, input_format: int = _lib.INPUT_RGB8) -> int:
+        """``:105-108``; returns the bytes of the pattern written."""
+        self.filter.apply_gpu(input, output, input_format)
+        return self.filter.get_output_size()
+
+    def correlate_gpu(self, left_key, left, right_key, right, output) -> int:
+        """``:111-115``; returns the bytes to copy back (8)."""
+        lp, lb = _mem(left)
+        rp, rb = _mem(right)
+        op, ob = _mem(output)
+        out = ctypes.c_size_t()
+        _lib.call("prnu_correlate", self._ctx.handle, lp, lb, rp, rb, op, ob, ctypes.byref(out), _stream())
+        _sync(True)
+        return out.value
+
+    def postprocess_output(self, left_key, right_key, buffer: bytes) -> float:
+        """``:118-120``: the first FP64 of the output (native byte order)."""
+        return struct.unpack_from("=d", buffer, 0)[0]
+
+    def destroy(self) -> None:
+        self._ctx.close()
+
+
+# ------------------------------------------------------------------ batched
+def pattern_pointer_table(patterns) -> "torch.Tensor":
+    """Device int64 array of the row addresses of a (M, ...) CUDA tensor or a
+    list of CUDA tensors."""
+    if isinstance(patterns, (list, tuple)):
+        addrs = [_mem(p)[0] for p in patterns]
+    else:
+        base, _ = _mem(patterns)
+        row = patterns[0].numel() * patterns.element_size()
+        addrs = [base + i * row for i in range(patterns.shape[0])]
+    return torch.tensor(addrs, dtype=torch.int64, device="cuda")
+
+
+class GramNCC:
+    """All-pairs NCC of M patterns of one size as a triangular Gram on MFMA.
+
+    ``precision``: "fp32" (FP32 MFMA), "split_bf16" (hi+lo BF16, three
+    products) or "fp8" (e4m3, per-pattern scale).  The statistics (sum x,
+    sum x^2 in FP64) are computed once per pattern set."""
+
+    def __init__(self, height: int, width: int):
+        self.height, self.width = int(height), int(width)
+        self._ctx = _Ctx(height, width, _lib.MODE_NCC)
+
+    def stats(self, ptrs, count: int, sync: bool = True):
+        n = self.height * self.width
+        s = torch.empty(count, dtype=torch.float64, device="cuda")
+        ss = torch.empty(count, dtype=torch.float64, device="cuda")
+        am = torch.empty(count, dtype=torch.float32, device="cuda")
+        _lib.call("prnu_pattern_stats", ptrs.data_ptr(), count, n, s.data_ptr(), ss.data_ptr(), am.data_ptr(), _stream())
+        _sync(sync)
+        return s, ss, am
+
+    def ncc(self, patterns=None, precision: str = "split_bf16", stats=None, out=None, sync: bool = True,
+            ptrs=None):
+        """NCC matrix of ``patterns`` (a (M, h*w) CUDA tensor or a list of
+        CUDA tensors), or of a prebuilt device pointer table ``ptrs``."""
+        if ptrs is None:
+            ptrs = pattern_pointer_table(patterns)
+        count = int(ptrs.numel())
+        if stats is None:
+            stats = self.stats(ptrs, count, sync=False)
+        s, ss, am = stats
+        if out is None:
+            out = torch.empty((count, count), dtype=torch.float64, device="cuda")
+        _lib.call("prnu_gram_ncc", self._ctx.handle, ptrs.data_ptr(), count, _lib.PRECISIONS[precision],
+                  s.data_ptr(), ss.data_ptr(), am.data_ptr(), out.data_ptr(), _stream())
+        _sync(sync)
+        return out
+
+    def close(self) -> None:
+        self._ctx.close()

@@ -1,0 +1,590 @@
+// PRNU extraction on gfx950: grayscale + FastNoise (one LDS halo-tile
+// kernel), zero-mean total, frequency-domain Wiener filter and the R2C
+// spectrum of the PCE strategy.
+//
+// Reference semantics (paths under CSI/kernels/filter/):
+//   grayscalefilter.cu:47-66      out = 0.299*c.z + 0.587*c.y + 0.114*c.x
+//   fastnoisefilter.cu:355-409    out = hgrad(dx*s) + vgrad(dy*s), s = 1/(1+|grad|)
+//   zeromeantotalfilter.cu        per column then per row: subtract even/odd means
+//   wienerfilter.cu               F = FFT2(x); varest = min_f box_f(|F|^2);
+//                                 F *= var/max(var, varest); x = Re(IFFT2(F))/n
+//   SpectralFilter.java:44-54     R2C FFT2, h x (w/2+1)
+//
+// Layout in HBM: images and patterns are row-major h x w FP32.  The Wiener
+// filter runs its row FFTs in LDS, transposes (w x h) so the column FFTs are
+// also LDS row FFTs, does the variance stencil and the scaling in that
+// transposed frequency domain (the zero-padded box-min is transpose
+// invariant), and transposes back for the inverse row FFTs.
+#include "prnu_internal.hpp"
+#include "fft_device.hpp"
+#include "fft_host.hpp"
+
+namespace prnu {
+namespace {
+
+constexpr float kWR = 0.299f, kWG = 0.587f, kWB = 0.114f;
+
+// ---------------------------------------------------------------- reductions
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// Deterministic sum over a 256-thread block.
+__device__ inline double block_sum_256(double v, double *sh) {
+    v = wave_sum_d(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// ---------------------------------------------------------------- grayscale
+template <int FMT>
+__device__ __forceinline__ float gray_at(const void *in, int64_t idx) {
+    if (FMT == PRNU_INPUT_RGB8) {
+        const uint8_t *p = static_cast<const uint8_t *>(in) + 3 * idx;
+        // byte 2 (c.z) carries 0.299, byte 0 (c.x) 0.114
+        return kWR * (float)p[2] + kWG * (float)p[1] + kWB * (float)p[0];
+    } else if (FMT == PRNU_INPUT_GRAY8) {
+        const float g = (float)static_cast<const uint8_t *>(in)[idx];
+        return kWR * g + kWG * g + kWB * g;
+    } else {
+        return static_cast<const float *>(in)[idx];
+    }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void gray_kernel(const void *in, float *out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        out[i] = gray_at<FMT>(in, i);
+}
+
+// ---------------------------------------------------------------- FastNoise
+constexpr int kFnTX = 64, kFnTY = 16;  // output tile
+
+// One-sided at the borders, central inside (fastnoisefilter.cu:355-381).
+__device__ __forceinline__ float grad1(float left, float center, float right, int pos, int len) {
+    if (pos == 0) return right - center;
+    if (pos == len - 1) return center - left;
+    return 0.5f * (right - left);
+}
+
+// Gray conversion + FastNoise for a 64 x 16 tile with a 2-pixel halo, and
+// optionally the zero-mean partial sums of the output tile: per column the
+// sums of even and odd rows (colpart), per row the sums of even and odd
+// columns (rowpart), FP64.
+template <int FMT>
+__global__ __launch_bounds__(256) void fastnoise_kernel(const void *in, float *out, int h, int w,
+                                                        double *colpart, double *rowpart) {
+    __shared__ float G[kFnTY + 4][kFnTX + 4];
+    __shared__ float U[kFnTY + 2][kFnTX + 2];
+    __shared__ float V[kFnTY + 2][kFnTX + 2];
+    __shared__ float O[kFnTY][kFnTX + 1];
+    const int x0 = blockIdx.x * kFnTX, y0 = blockIdx.y * kFnTY;
+    const int tid = threadIdx.x;
+
+    for (int idx = tid; idx < (kFnTY + 4) * (kFnTX + 4); idx += 256) {
+        const int ly = idx / (kFnTX + 4), lx = idx % (kFnTX + 4);
+        const int gy = y0 + ly - 2, gx = x0 + lx - 2;
+        float v = 0.f;
+        if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = gray_at<FMT>(in, (int64_t)gy * w + gx);
+        G[ly][lx] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < (kFnTY + 2) * (kFnTX + 2); idx += 256) {
+        const int ly = idx / (kFnTX + 2), lx = idx % (kFnTX + 2);
+        const int gy = y0 + ly - 1, gx = x0 + lx - 1;
+        float u = 0.f, v = 0.f;
+        if (gy >= 0 && gy < h && gx >= 0 && gx < w) {
+            const int cy = ly + 1, cx = lx + 1;  // position in G
+            const float dx = grad1(G[cy][cx - 1], G[cy][cx], G[cy][cx + 1], gx, w);
+            const float dy = grad1(G[cy - 1][cx], G[cy][cx], G[cy + 1][cx], gy, h);
+            const float norm = sqrtf(dx * dx + dy * dy);
+            const float scale = 1.0f / (1.0f + norm);
+            u = scale * dx;
+            v = scale * dy;
+        }
+        U[ly][lx] = u;
+        V[ly][lx] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kFnTY * kFnTX; idx += 256) {
+        const int ly = idx / kFnTX, lx = idx % kFnTX;
+        const int gy = y0 + ly, gx = x0 + lx;
+        float o = 0.f;
+        if (gy < h && gx < w) {
+            const int cy = ly + 1, cx = lx + 1;  // position in U/V
+            const float a = grad1(U[cy][cx - 1], U[cy][cx], U[cy][cx + 1], gx, w);
+            const float b = grad1(V[cy - 1][cx], V[cy][cx], V[cy + 1][cx], gy, h);
+            o = a + b;
+            out[(int64_t)gy * w + gx] = o;
+        }
+        O[ly][lx] = o;
+    }
+    if (!colpart) return;
+    __syncthreads();
+    if (tid < kFnTX) {
+        const int gx = x0 + tid;
+        double se = 0.0, so = 0.0;
+        for (int ly = 0; ly < kFnTY; ++ly) {
+            if (((y0 + ly) & 1) == 0) se += O[ly][tid];
+            else so += O[ly][tid];
+        }
+        if (gx < w) {
+            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 0] = se;
+            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 1] = so;
+        }
+    } else if (tid >= 64 && tid < 64 + kFnTY) {
+        const int ly = tid - 64, gy = y0 + ly;
+        double se = 0.0, so = 0.0;
+        for (int lx = 0; lx < kFnTX; ++lx) {
+            if (((x0 + lx) & 1) == 0) se += O[ly][lx];
+            else so += O[ly][lx];
+        }
+        if (gy < h) {
+            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 0] = se;
+            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 1] = so;
+        }
+    }
+}
+
+// -------------------------------------------------------------- zero-mean
+// Partial sums of an arbitrary image in the fastnoise tiling.
+__global__ __launch_bounds__(256) void zm_partials_kernel(const float *img, int h, int w, double *colpart,
+                                                          double *rowpart) {
+    __shared__ float O[kFnTY][kFnTX + 1];
+    const int x0 = blockIdx.x * kFnTX, y0 = blockIdx.y * kFnTY;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kFnTY * kFnTX; idx += 256) {
+        const int ly = idx / kFnTX, lx = idx % kFnTX;
+        const int gy = y0 + ly, gx = x0 + lx;
+        O[ly][lx] = (gy < h && gx < w) ? img[(int64_t)gy * w + gx] : 0.f;
+    }
+    __syncthreads();
+    if (tid < kFnTX) {
+        const int gx = x0 + tid;
+        double se = 0.0, so = 0.0;
+        for (int ly = 0; ly < kFnTY; ++ly) {
+            if (((y0 + ly) & 1) == 0) se += O[ly][tid];
+            else so += O[ly][tid];
+        }
+        if (gx < w) {
+            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 0] = se;
+            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 1] = so;
+        }
+    } else if (tid >= 64 && tid < 64 + kFnTY) {
+        const int ly = tid - 64, gy = y0 + ly;
+        double se = 0.0, so = 0.0;
+        for (int lx = 0; lx < kFnTX; ++lx) {
+            if (((x0 + lx) & 1) == 0) se += O[ly][lx];
+            else so += O[ly][lx];
+        }
+        if (gy < h) {
+            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 0] = se;
+            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 1] = so;
+        }
+    }
+}
+
+// colsum[p][x] = sum over tile rows; rowmean[p][y] = row sums / column counts.
+__global__ __launch_bounds__(256) void zm_reduce_kernel(const double *colpart, const double *rowpart, int h, int w,
+                                                        int tiles_y, int tiles_x, double *colsum, double *rowmean) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < w) {
+        double se = 0.0, so = 0.0;
+        for (int t = 0; t < tiles_y; ++t) {
+            se += colpart[((int64_t)t * w + i) * 2 + 0];
+            so += colpart[((int64_t)t * w + i) * 2 + 1];
+        }
+        colsum[i] = se;
+        colsum[w + i] = so;
+    }
+    if (i < h) {
+        double se = 0.0, so = 0.0;
+        for (int t = 0; t < tiles_x; ++t) {
+            se += rowpart[((int64_t)t * h + i) * 2 + 0];
+            so += rowpart[((int64_t)t * h + i) * 2 + 1];
+        }
+        rowmean[i] = se / (double)((w + 1) >> 1);
+        rowmean[h + i] = so / (double)(w >> 1);
+    }
+}
+
+// g[pi*2+pj] = mean of the (row parity pi, column parity pj) sub-lattice.
+// One block of 256 threads; also turns colsum into column means in place.
+__global__ __launch_bounds__(256) void zm_lattice_kernel(double *colsum, int h, int w, double *g) {
+    __shared__ double sh[4];
+    double s[4] = {0, 0, 0, 0};
+    for (int x = threadIdx.x; x < w; x += 256) {
+        s[0 + (x & 1)] += colsum[x];
+        s[2 + (x & 1)] += colsum[w + x];
+    }
+    double tot[4];
+    for (int q = 0; q < 4; ++q) tot[q] = block_sum_256(s[q], sh);
+    const double rows_e = (double)((h + 1) >> 1), rows_o = (double)(h >> 1);
+    const double cols_e = (double)((w + 1) >> 1), cols_o = (double)(w >> 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        g[0] = tot[0] / (rows_e * cols_e);
+        g[1] = tot[1] / (rows_e * cols_o);
+        g[2] = tot[2] / (rows_o * cols_e);
+        g[3] = tot[3] / (rows_o * cols_o);
+    }
+    for (int x = threadIdx.x; x < w; x += 256) {
+        colsum[x] /= rows_e;
+        colsum[w + x] /= rows_o;
+    }
+}
+
+// out = x - colmean[pi][x] - rowmean[pj][y] + g[pi][pj]: columns-then-rows
+// zero-mean of zeromeantotalfilter.cu in closed form.
+__device__ __forceinline__ float zm_value(float x, int y, int xx, int h, int w, const double *colmean,
+                                          const double *rowmean, const double *g) {
+    const int pi = y & 1, pj = xx & 1;
+    return (float)((double)x - colmean[pi * w + xx] - rowmean[pj * h + y] + g[pi * 2 + pj]);
+}
+
+__global__ __launch_bounds__(256) void zm_apply_kernel(float *img, int h, int w, const double *colmean,
+                                                       const double *rowmean, const double *g) {
+    const int64_t n = (int64_t)h * w;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int y = (int)(i / w), x = (int)(i - (int64_t)y * w);
+        img[i] = zm_value(img[i], y, x, h, w, colmean, rowmean, g);
+    }
+}
+
+// ------------------------------------------------------------------ Wiener
+// Row i: (optional zero-mean) -> sum of squares (FP64) -> forward FFT (w).
+__global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img, int h,
+                                                                      float2 *A, double *rowsq,
+                                                                      const double *colmean, const double *rowmean,
+                                                                      const double *g) {
+    extern __shared__ float2 buf[];
+    __shared__ double sh[4];
+    const int i = blockIdx.x;
+    const int w = d.n;
+    double sq = 0.0;
+    for (int j = threadIdx.x; j < w; j += kFftThreads) {
+        float v = img[(int64_t)i * w + j];
+        if (colmean) v = zm_value(v, i, j, h, w, colmean, rowmean, g);
+        sq += (double)v * (double)v;
+        buf[j] = make_float2(v, 0.f);
+    }
+    sq = block_sum_256(sq, sh);
+    if (threadIdx.x == 0) rowsq[i] = sq;
+    __syncthreads();
+    fft_lds<-1>(buf, d);
+    for (int j = threadIdx.x; j < w; j += kFftThreads) A[(int64_t)i * w + j] = buf[j];
+}
+
+// In-place forward FFT of the rows of B (length d.n) and |F|^2 into P.
+__global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, float2 *B, float *P) {
+    extern __shared__ float2 buf[];
+    const int r = blockIdx.x;
+    const int n = d.n;
+    float2 *row = B + (int64_t)r * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
+    __syncthreads();
+    fft_lds<-1>(buf, d);
+    for (int j = threadIdx.x; j < n; j += kFftThreads) {
+        const float2 f = buf[j];
+        row[j] = f;
+        P[(int64_t)r * n + j] = f.x * f.x + f.y * f.y;
+    }
+}
+
+// var = n/(n-1) * sum(x^2) over the row partials, FP64, fixed order.
+__global__ __launch_bounds__(256) void wiener_var_kernel(const double *rowsq, int h, int64_t n, double *var) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < h; i += 256) s += rowsq[i];
+    s = block_sum_256(s, sh);
+    if (threadIdx.x == 0) var[0] = s * (double)n / (double)(n - 1);
+}
+
+// Variance estimate: min over f in {3,5,7,9} of the zero-padded f x f mean of
+// P (wienerfilter.cu:164-247), then S = var / max(var, varest).
+constexpr int kVT = 32, kVB = 4;
+__global__ __launch_bounds__(256) void varest_scale_kernel(const float *P, float *S, int rows, int cols,
+                                                           const double *var) {
+    __shared__ float T[kVT + 2 * kVB][kVT + 2 * kVB + 1];
+    const int c0 = blockIdx.x * kVT, r0 = blockIdx.y * kVT;
+    for (int idx = threadIdx.x; idx < (kVT + 2 * kVB) * (kVT + 2 * kVB); idx += 256) {
+        const int ly = idx / (kVT + 2 * kVB), lx = idx % (kVT + 2 * kVB);
+        const int gy = r0 + ly - kVB, gx = c0 + lx - kVB;
+        T[ly][lx] = (gy >= 0 && gy < rows && gx >= 0 && gx < cols) ? P[(int64_t)gy * cols + gx] : 0.f;
+    }
+    __syncthreads();
+    const float v = (float)var[0];
+    for (int idx = threadIdx.x; idx < kVT * kVT; idx += 256) {
+        const int ly = idx / kVT, lx = idx % kVT;
+        const int gy = r0 + ly, gx = c0 + lx;
+        if (gy >= rows || gx >= cols) continue;
+        const int cy = ly + kVB, cx = lx + kVB;
+        float sum = T[cy][cx];
+        float best = 3.40282347e+38f;
+#pragma unroll
+        for (int r = 1; r <= kVB; ++r) {
+            const int f = 2 * r + 1;
+            for (int k = -r; k <= r; ++k) {
+                sum += T[cy - r][cx + k];
+                sum += T[cy + r][cx + k];
+            }
+            for (int k = -r + 1; k <= r - 1; ++k) {
+                sum += T[cy + k][cx - r];
+                sum += T[cy + k][cx + r];
+            }
+            const float avg = sum / (float)(f * f);
+            best = avg < best ? avg : best;
+        }
+        S[(int64_t)gy * cols + gx] = v / fmaxf(v, best);
+    }
+}
+
+// Scale row r of B by S and run the inverse FFT (length d.n) in place.
+__global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d, float2 *B, const float *S) {
+    extern __shared__ float2 buf[];
+    const int r = blockIdx.x;
+    const int n = d.n;
+    float2 *row = B + (int64_t)r * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = cscale(row[j], S[(int64_t)r * n + j]);
+    __syncthreads();
+    fft_lds<1>(buf, d);
+    for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
+}
+
+// Inverse FFT (length w) of row i of A; out = Re / (w*h)  (normalizeToReal).
+__global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, const float2 *A, float *out,
+                                                                     float inv_n) {
+    extern __shared__ float2 buf[];
+    const int i = blockIdx.x;
+    const int w = d.n;
+    for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = A[(int64_t)i * w + j];
+    __syncthreads();
+    fft_lds<1>(buf, d);
+    for (int j = threadIdx.x; j < w; j += kFftThreads) out[(int64_t)i * w + j] = buf[j].x * inv_n;
+}
+
+// ---------------------------------------------------------------- spectrum
+// Forward FFT (length w) of real row i; the first w/2+1 bins go to A (h x wh).
+__global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img, float2 *A) {
+    extern __shared__ float2 buf[];
+    const int i = blockIdx.x;
+    const int w = d.n, wh = w / 2 + 1;
+    for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = make_float2(img[(int64_t)i * w + j], 0.f);
+    __syncthreads();
+    fft_lds<-1>(buf, d);
+    for (int j = threadIdx.x; j < wh; j += kFftThreads) A[(int64_t)i * wh + j] = buf[j];
+}
+
+template <int DIR>
+__global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d, float2 *B) {
+    extern __shared__ float2 buf[];
+    const int n = d.n;
+    float2 *row = B + (int64_t)blockIdx.x * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
+    __syncthreads();
+    fft_lds<DIR>(buf, d);
+    for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
+}
+
+// ------------------------------------------------------------- workspace
+struct ExtractWs {
+    float2 *A, *B;
+    float *P, *S;
+    double *rowsq, *var, *colpart, *rowpart, *colmean, *rowmean, *g;
+    int tiles_x, tiles_y;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+int get_ws(Context *c, ExtractWs &e) {
+    const int h = c->h, w = c->w;
+    const int64_t n = c->pixels();
+    e.tiles_x = (w + kFnTX - 1) / kFnTX;
+    e.tiles_y = (h + kFnTY - 1) / kFnTY;
+    const size_t sz[] = {align256(sizeof(float2) * n), align256(sizeof(float2) * n), align256(sizeof(float) * n),
+                         align256(sizeof(float) * n), align256(sizeof(double) * h), align256(sizeof(double) * 2),
+                         align256(sizeof(double) * 2 * w * (size_t)e.tiles_y),
+                         align256(sizeof(double) * 2 * h * (size_t)e.tiles_x), align256(sizeof(double) * 2 * w),
+                         align256(sizeof(double) * 2 * h), align256(sizeof(double) * 4)};
+    size_t total = 0;
+    for (size_t s : sz) total += s;
+    PRNU_TRY(c->ws_filter.ensure(total));
+    char *p = static_cast<char *>(c->ws_filter.ptr);
+    void **slots[] = {(void **)&e.A,       (void **)&e.B,       (void **)&e.P,       (void **)&e.S,
+                      (void **)&e.rowsq,   (void **)&e.var,     (void **)&e.colpart, (void **)&e.rowpart,
+                      (void **)&e.colmean, (void **)&e.rowmean, (void **)&e.g};
+    for (int k = 0; k < 11; ++k) {
+        *slots[k] = p;
+        p += sz[k];
+    }
+    return PRNU_OK;
+}
+
+size_t fft_smem(const FftDesc &d) { return sizeof(float2) * d.n; }
+
+template <int FMT>
+int launch_fastnoise_fmt(const void *in, float *out, int h, int w, double *colpart, double *rowpart,
+                         hipStream_t st) {
+    dim3 grid((w + kFnTX - 1) / kFnTX, (h + kFnTY - 1) / kFnTY);
+    hipLaunchKernelGGL(fastnoise_kernel<FMT>, grid, dim3(256), 0, st, in, out, h, w, colpart, rowpart);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+int launch_fastnoise(int fmt, const void *in, float *out, int h, int w, double *colpart, double *rowpart,
+                     hipStream_t st) {
+    switch (fmt) {
+        case PRNU_INPUT_RGB8: return launch_fastnoise_fmt<PRNU_INPUT_RGB8>(in, out, h, w, colpart, rowpart, st);
+        case PRNU_INPUT_GRAY8: return launch_fastnoise_fmt<PRNU_INPUT_GRAY8>(in, out, h, w, colpart, rowpart, st);
+        default: return launch_fastnoise_fmt<PRNU_INPUT_FLOAT32>(in, out, h, w, colpart, rowpart, st);
+    }
+}
+
+int launch_zm_means(Context *c, ExtractWs &e, hipStream_t st) {
+    const int h = c->h, w = c->w;
+    const int m = h > w ? h : w;
+    hipLaunchKernelGGL(zm_reduce_kernel, dim3((m + 255) / 256), dim3(256), 0, st, e.colpart, e.rowpart, h, w,
+                       e.tiles_y, e.tiles_x, e.colmean, e.rowmean);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(zm_lattice_kernel, dim3(1), dim3(256), 0, st, e.colmean, h, w, e.g);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+// Wiener filter of img (h x w) into out; with zero-mean means when zm.
+int launch_wiener(Context *c, ExtractWs &e, const float *img, float *out, bool zm, hipStream_t st) {
+    const int h = c->h, w = c->w;
+    const FftDesc &dw = fft_desc(c->fft_w);
+    const FftDesc &dh = fft_desc(c->fft_h);
+    hipLaunchKernelGGL(wiener_rows_fwd_kernel, dim3(h), dim3(kFftThreads), fft_smem(dw), st, dw, img, h, e.A,
+                       e.rowsq, zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
+    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(launch_transpose_c(e.A, e.B, h, w, 1, 0, st));
+    hipLaunchKernelGGL(rows_fft_power_kernel, dim3(w), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.P);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(wiener_var_kernel, dim3(1), dim3(256), 0, st, e.rowsq, h, c->pixels(), e.var);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(varest_scale_kernel, dim3((h + kVT - 1) / kVT, (w + kVT - 1) / kVT), dim3(256), 0, st, e.P,
+                       e.S, w, h, e.var);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rows_ifft_scaled_kernel, dim3(w), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.S);
+    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(launch_transpose_c(e.B, e.A, w, h, 1, 0, st));
+    hipLaunchKernelGGL(rows_ifft_real_kernel, dim3(h), dim3(kFftThreads), fft_smem(dw), st, dw, e.A, out,
+                       1.0f / (float)(w * h));
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+int launch_spectrum(Context *c, ExtractWs &e, const float *img, float2 *out, hipStream_t st) {
+    const int h = c->h, w = c->w, wh = w / 2 + 1;
+    const FftDesc &dw = fft_desc(c->fft_w);
+    const FftDesc &dh = fft_desc(c->fft_h);
+    hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h), dim3(kFftThreads), fft_smem(dw), st, dw, img, e.A);
+    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, 1, 0, st));
+    hipLaunchKernelGGL(rows_fft_inplace_kernel<-1>, dim3(wh), dim3(kFftThreads), fft_smem(dh), st, dh, e.B);
+    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(launch_transpose_c(e.B, out, wh, h, 1, 0, st));
+    return PRNU_OK;
+}
+
+int input_bytes_per_pixel(int fmt) {
+    return fmt == PRNU_INPUT_RGB8 ? 3 : fmt == PRNU_INPUT_GRAY8 ? 1 : fmt == PRNU_INPUT_FLOAT32 ? 4 : 0;
+}
+
+}  // namespace
+
+// Whole PRNUFilter chain (PRNUFilter.java:124-138) into pattern (NCC mode,
+// h*w FP32) or spectrum (PCE mode).
+int run_preprocess(Context *c, int fmt, const void *in, size_t in_bytes, void *out, size_t out_cap,
+                   size_t *out_bytes, hipStream_t st) {
+    const int bpp = input_bytes_per_pixel(fmt);
+    PRNU_REQUIRE(bpp > 0, "preprocess: unknown input format %d", fmt);
+    PRNU_REQUIRE(in_bytes >= (size_t)bpp * c->pixels(), "preprocess: input has %zu bytes, needs %lld", in_bytes,
+                 (long long)(bpp * c->pixels()));
+    const size_t need = c->mode == PRNU_MODE_PCE ? 8 * (size_t)c->spectrum_elems() : 4 * (size_t)c->pixels();
+    PRNU_REQUIRE(out_cap >= need, "preprocess: output has %zu bytes, needs %zu", out_cap, need);
+    PRNU_TRY(ensure_fft_plans(c));
+    ExtractWs e;
+    PRNU_TRY(get_ws(c, e));
+    // gray + FastNoise into S (as an FP32 image), zero-mean sums on the side
+    float *noise = e.S;
+    PRNU_TRY(launch_fastnoise(fmt, in, noise, c->h, c->w, e.colpart, e.rowpart, st));
+    PRNU_TRY(launch_zm_means(c, e, st));
+    if (c->mode == PRNU_MODE_NCC) {
+        time_begin(c, st);
+        PRNU_TRY(launch_wiener(c, e, noise, static_cast<float *>(out), true, st));
+        time_end(c, st);
+    } else {
+        // Wiener output into P (FP32 h*w), then its spectrum
+        PRNU_TRY(launch_wiener(c, e, noise, e.P, true, st));
+        PRNU_TRY(launch_spectrum(c, e, e.P, static_cast<float2 *>(out), st));
+    }
+    if (out_bytes) *out_bytes = need;
+    return PRNU_OK;
+}
+
+int run_filter_grayscale(Context *c, int fmt, const void *in, size_t in_bytes, float *out, hipStream_t st) {
+    const int bpp = input_bytes_per_pixel(fmt);
+    PRNU_REQUIRE(bpp > 0, "grayscale: unknown input format %d", fmt);
+    // GrayscaleFilter.java:72-74
+    PRNU_REQUIRE(in_bytes >= (size_t)bpp * c->pixels(), "grayscale: input has %zu bytes, needs %lld", in_bytes,
+                 (long long)(bpp * c->pixels()));
+    const int64_t n = c->pixels();
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    if (fmt == PRNU_INPUT_RGB8)
+        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_RGB8>, dim3(blocks), dim3(256), 0, st, in, out, n);
+    else if (fmt == PRNU_INPUT_GRAY8)
+        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_GRAY8>, dim3(blocks), dim3(256), 0, st, in, out, n);
+    else
+        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_FLOAT32>, dim3(blocks), dim3(256), 0, st, in, out, n);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+int run_filter_fastnoise(Context *c, float *img, hipStream_t st) {
+    // the kernel reads a halo around each tile, so it cannot write in place
+    ExtractWs e;
+    PRNU_TRY(get_ws(c, e));
+    PRNU_TRY(launch_fastnoise(PRNU_INPUT_FLOAT32, img, e.P, c->h, c->w, nullptr, nullptr, st));
+    PRNU_CHECK_HIP(hipMemcpyAsync(img, e.P, 4 * (size_t)c->pixels(), hipMemcpyDeviceToDevice, st));
+    return PRNU_OK;
+}
+
+int run_filter_zeromean(Context *c, float *img, hipStream_t st) {
+    ExtractWs e;
+    PRNU_TRY(get_ws(c, e));
+    dim3 grid(e.tiles_x, e.tiles_y);
+    hipLaunchKernelGGL(zm_partials_kernel, grid, dim3(256), 0, st, img, c->h, c->w, e.colpart, e.rowpart);
+    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(launch_zm_means(c, e, st));
+    const int64_t n = c->pixels();
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(zm_apply_kernel, dim3(blocks), dim3(256), 0, st, img, c->h, c->w, e.colmean, e.rowmean,
+                       e.g);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+int run_filter_wiener(Context *c, float *img, hipStream_t st) {
+    PRNU_TRY(ensure_fft_plans(c));
+    ExtractWs e;
+    PRNU_TRY(get_ws(c, e));
+    return launch_wiener(c, e, img, img, false, st);
+}
+
+int run_filter_spectral(Context *c, const float *img, float *out, size_t out_cap, hipStream_t st) {
+    // SpectralFilter.java:45-47
+    PRNU_REQUIRE(out_cap >= 8 * (size_t)c->spectrum_elems(), "spectral: output has %zu bytes, needs %lld", out_cap,
+                 (long long)(8 * c->spectrum_elems()));
+    PRNU_TRY(ensure_fft_plans(c));
+    ExtractWs e;
+    PRNU_TRY(get_ws(c, e));
+    return launch_spectrum(c, e, img, reinterpret_cast<float2 *>(out), st);
+}
+
+}  // namespace prnu

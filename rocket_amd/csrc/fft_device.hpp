@@ -1,0 +1,173 @@
+// Mixed-radix (2, 3, 4, 5) Stockham FFT of one row held in LDS.
+//
+// A length-n transform, n = prod(radix[p]), runs as `passes` autosorting
+// passes.  Pass p with radix R and Ns = radix[0]*...*radix[p-1] does, for each
+// butterfly j in [0, n/R), with k = j mod Ns:
+//     a[r] = buf[j + r*n/R] * w^(r*k),   w = exp(DIR * 2*pi*i / (Ns*R))
+//     b    = DFT_R(a)
+//     buf[(j/Ns)*Ns*R + k + q*Ns] = b[q]
+// which leaves the result in natural order (Stockham autosort).  Twiddles come
+// from one table tw[t] = exp(-2*pi*i*t/n) (computed in FP64 on the host):
+// w^(r*k) = tw[r*k*n/(Ns*R)] (conjugated for the inverse).  DIR = -1 is the
+// forward transform, +1 the unnormalised inverse.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace prnu {
+
+constexpr int kFftMaxPasses = 16;
+constexpr int kFftMaxN = 8192;   // longest row held in LDS (64 KiB)
+constexpr int kFftThreads = 256;
+
+struct FftDesc {
+    int n;
+    int passes;
+    int radix[kFftMaxPasses];
+    const float2 *tw;  // n twiddles, device memory
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// i * a
+__device__ __forceinline__ float2 cmuli(float2 a) { return make_float2(-a.y, a.x); }
+
+template <int DIR>
+__device__ __forceinline__ float2 twiddle(const float2 *tw, int idx) {
+    float2 t = tw[idx];
+    if (DIR > 0) t.y = -t.y;
+    return t;
+}
+
+// In-register DFT of size R with exponent sign DIR.
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2 *a);
+
+template <>
+__device__ __forceinline__ void dft<2, -1>(float2 *a) {
+    const float2 t = a[1];
+    a[1] = csub(a[0], t);
+    a[0] = cadd(a[0], t);
+}
+template <>
+__device__ __forceinline__ void dft<2, 1>(float2 *a) {
+    dft<2, -1>(a);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2 *a) {
+    const float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+    const float2 s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
+    // DIR * i * (a1 - a3)
+    const float2 r = DIR < 0 ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
+    a[0] = cadd(s02, s13);
+    a[2] = csub(s02, s13);
+    a[1] = cadd(d02, r);
+    a[3] = csub(d02, r);
+}
+template <>
+__device__ __forceinline__ void dft<4, -1>(float2 *a) { dft4<-1>(a); }
+template <>
+__device__ __forceinline__ void dft<4, 1>(float2 *a) { dft4<1>(a); }
+
+template <int DIR>
+__device__ __forceinline__ void dft3(float2 *a) {
+    constexpr float kS = 0.866025403784438646763723170752936183f;  // sqrt(3)/2
+    const float2 t1 = cadd(a[1], a[2]);
+    const float2 t2 = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+    const float2 d = csub(a[1], a[2]);
+    // DIR * i * sqrt(3)/2 * (a1 - a2)
+    const float2 r = DIR < 0 ? make_float2(kS * d.y, -kS * d.x) : make_float2(-kS * d.y, kS * d.x);
+    a[0] = cadd(a[0], t1);
+    a[1] = cadd(t2, r);
+    a[2] = csub(t2, r);
+}
+template <>
+__device__ __forceinline__ void dft<3, -1>(float2 *a) { dft3<-1>(a); }
+template <>
+__device__ __forceinline__ void dft<3, 1>(float2 *a) { dft3<1>(a); }
+
+template <int DIR>
+__device__ __forceinline__ void dft5(float2 *a) {
+    constexpr float c1 = 0.309016994374947424102293417182819059f;   // cos(2pi/5)
+    constexpr float c2 = -0.809016994374947424102293417182819059f;  // cos(4pi/5)
+    constexpr float s1 = 0.951056516295153572116439333379382143f;   // sin(2pi/5)
+    constexpr float s2 = 0.587785252292473129168705954639072769f;   // sin(4pi/5)
+    const float2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+    const float2 t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
+    const float2 m1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+    const float2 m2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+    // n1 = s1*t3 + s2*t4, n2 = s2*t3 - s1*t4; multiplied by DIR * i
+    const float2 n1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const float2 n2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    const float2 r1 = DIR < 0 ? make_float2(n1.y, -n1.x) : make_float2(-n1.y, n1.x);
+    const float2 r2 = DIR < 0 ? make_float2(n2.y, -n2.x) : make_float2(-n2.y, n2.x);
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(m1, r1);
+    a[4] = csub(m1, r1);
+    a[2] = cadd(m2, r2);
+    a[3] = csub(m2, r2);
+}
+template <>
+__device__ __forceinline__ void dft<5, -1>(float2 *a) { dft5<-1>(a); }
+template <>
+__device__ __forceinline__ void dft<5, 1>(float2 *a) { dft5<1>(a); }
+
+// One Stockham pass of radix R over buf[0..n) with kFftThreads threads.
+template <int R, int DIR>
+__device__ __forceinline__ void fft_pass(float2 *buf, int n, int Ns, const float2 *tw) {
+    constexpr int kMaxB = (kFftMaxN / R + kFftThreads - 1) / kFftThreads;
+    const int nb = n / R;
+    const int twstep = n / (Ns * R);
+    const int tid = threadIdx.x;
+    float2 v[kMaxB][R];
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nb) {
+            const int k = j % Ns;
+            v[b][0] = buf[j];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[j + r * nb], twiddle<DIR>(tw, r * k * twstep));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nb) {
+            dft<R, DIR>(v[b]);
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[base + q * Ns] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Full in-LDS transform of buf[0..d.n).  Callers synchronise before (data in
+// LDS) and may read buf after return (the last pass ends with a barrier).
+template <int DIR>
+__device__ inline void fft_lds(float2 *buf, const FftDesc &d) {
+    int Ns = 1;
+    for (int p = 0; p < d.passes; ++p) {
+        const int R = d.radix[p];
+        if (R == 4)
+            fft_pass<4, DIR>(buf, d.n, Ns, d.tw);
+        else if (R == 2)
+            fft_pass<2, DIR>(buf, d.n, Ns, d.tw);
+        else if (R == 3)
+            fft_pass<3, DIR>(buf, d.n, Ns, d.tw);
+        else
+            fft_pass<5, DIR>(buf, d.n, Ns, d.tw);
+        Ns *= R;
+    }
+}
+
+}  // namespace prnu

@@ -1,0 +1,19 @@
+// Host-side FFT plan helpers shared by the extraction and PCE translation units.
+#pragma once
+
+#include "fft_device.hpp"
+#include "prnu_internal.hpp"
+
+namespace prnu {
+
+bool fft_factor(int n, int *radix, int *passes);
+int fft_plan_create(int n, FftPlan **out);
+void fft_plan_destroy(FftPlan *p);
+const FftDesc &fft_desc(const FftPlan *p);
+// Create the context's length-w and length-h plans on first use.
+int ensure_fft_plans(Context *c);
+// out[b][c][r] = in[b][r][c], complex, batch items batch_stride elements apart.
+int launch_transpose_c(const float2 *in, float2 *out, int rows, int cols, int batch, int64_t batch_stride,
+                       hipStream_t st);
+
+}  // namespace prnu
