@@ -176,6 +176,35 @@ PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int
                   const double *d_sum, const double *d_sumsq, const float *d_absmax,
                   double *d_ncc, prnu_stream stream);
 
+/* --------------------------------------------------------- pair scheduling
+ * Host-only (no device call).  Upper-triangle index spaces of
+ * R/indexspace/HilbertCurveIndexSpace.java:31-123 and
+ * R/indexspace/TilingIndexSpace.java:27-91 and the pair count of
+ * R/util/Util.java:51-77.  A node covers rows (left) [left_offset,
+ * left_offset+left_length) x columns (right) [right_offset, ...); pair (i, j)
+ * with i < j (or i <= j with include_diagonal) belongs to it. */
+typedef enum { PRNU_INDEX_HILBERT = 0, PRNU_INDEX_TILING = 1 } prnu_index_kind;
+
+typedef struct {
+    int32_t left_offset, left_length;
+    int32_t right_offset, right_length;
+    int32_t rule; /* Hilbert rule 0..3 (A..D); 0 for tiling */
+} prnu_index_node;
+
+/* Util.calculateTriangleSize(rowBegin, rowEnd, colBegin, colEnd, includeDiagonal) */
+PRNU_EXPORT int64_t prnu_triangle_size(int row_begin, int row_end, int col_begin, int col_end,
+                                       int include_diagonal);
+/* IndexSpace.divide(IndexSpaceDivision) of one node: `param` is minSplitSize
+ * (Hilbert) or tileSize (tiling).  Children to spawn go to `children`, entries
+ * the step emits directly go to `pairs` as (i, j) int32 pairs. */
+PRNU_EXPORT int prnu_index_divide(int kind, const prnu_index_node *node, int param, int include_diagonal,
+                                  prnu_index_node *children, int children_cap, int *n_children,
+                                  int32_t *pairs, int64_t pairs_cap, int64_t *n_pairs);
+/* Every pair of a `count`-key space in depth-first (curve) order.  With
+ * pairs == NULL only *n_pairs is computed. */
+PRNU_EXPORT int prnu_index_pairs(int kind, int count, int param, int include_diagonal, int32_t *pairs,
+                                 int64_t pairs_cap, int64_t *n_pairs);
+
 /* --------------------------------------------------------------- timing
  * No reference counterpart (the reference times whole tasks on the host,
  * R/RocketLauncher.java:115-203).  When enabled, the context records HIP

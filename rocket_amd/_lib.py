@@ -19,6 +19,7 @@ MODE_NCC, MODE_PCE = 0, 1
 INPUT_RGB8, INPUT_GRAY8, INPUT_FLOAT32 = 0, 1, 2
 PREC_FP32, PREC_SPLIT_BF16, PREC_FP8 = 0, 1, 2
 PCE_REFERENCE, PCE_TEXTBOOK = 0, 1
+INDEX_HILBERT, INDEX_TILING = 0, 1
 
 PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp8": PREC_FP8}
 
@@ -71,6 +72,10 @@ _SIGNATURES = {
     "prnu_filter_wiener": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
     "prnu_filter_spectral": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_size, _c_void_p]),
     "prnu_pattern_stats": (_c_int, [_c_void_p, _c_int, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "prnu_triangle_size": (_c_i64, [_c_int, _c_int, _c_int, _c_int, _c_int]),
+    "prnu_index_divide": (_c_int, [_c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, ctypes.POINTER(_c_int),
+                                   _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
+    "prnu_index_pairs": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
     "prnu_ctx_set_timing": (_c_int, [_c_void_p, _c_int]),
     "prnu_ctx_last_kernel_ms": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "prnu_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,

@@ -51,7 +51,9 @@ def test_fastnoise(h, w, rng):
     f = csi.PRNUFilter(h, w)
     t = cuda(x.ravel())
     f.get_fast_noise_filter().apply_gpu(t)
-    assert rel_err(t.cpu().numpy().reshape(h, w), O.fastnoise(x)) <= 2e-6
+    # the output is a sum of differences of normalised gradients, each in
+    # (-1, 1): FP32 rounding is bounded absolutely (outputs may cancel to ~0)
+    assert np.abs(t.cpu().numpy().reshape(h, w) - O.fastnoise(x)).max() <= 2e-6
     f.cleanup()
 
 

@@ -1,0 +1,70 @@
+"""Host-side pair scheduling (Rocket's index spaces) against brute force and
+the oracle's restatement of Util.calculateTriangleSize."""
+import numpy as np
+import pytest
+
+from oracle import prnu_oracle as O
+from rocket_amd import indexspace as ix
+
+
+@pytest.mark.parametrize("rb,re,cb,ce", [(0, 10, 0, 10), (0, 4, 6, 10), (3, 9, 2, 7), (5, 5, 0, 3), (0, 1000, 0, 1000)])
+@pytest.mark.parametrize("diag", [False, True])
+def test_triangle_size_matches_oracle(rb, re, cb, ce, diag):
+    assert ix.calculate_triangle_size(rb, re, cb, ce, diag) == O.triangle_size_range(rb, re, cb, ce, diag)
+
+
+def brute(n, diag):
+    return {(i, j) for i in range(n) for j in range(n) if i < j or (diag and i == j)}
+
+
+@pytest.mark.parametrize("kind,param", [("hilbert", 1), ("hilbert", 64), ("tiling", 1), ("tiling", 7), ("tiling", 50)])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 17, 64, 101])
+@pytest.mark.parametrize("diag", [False, True])
+def test_all_pairs_exactly_once(kind, param, n, diag):
+    pairs = ix.all_pairs(n, kind, param, diag)
+    got = [tuple(p) for p in pairs]
+    assert len(got) == len(set(got)) == O.triangle_size_range(0, n, 0, n, diag)
+    assert set(got) == brute(n, diag)
+
+
+def test_hilbert_order_is_local():
+    # on a power-of-two square the curve moves one cell at a time except where
+    # it skips over the lower triangle
+    pairs = ix.all_pairs(64, "hilbert", 1, True)
+    steps = np.abs(np.diff(pairs, axis=0)).sum(axis=1)
+    assert np.mean(steps == 1) > 0.95
+
+
+def test_divide_recursion_covers_space():
+    keys = [f"img{k}" for k in range(37)]
+    root = ix.HilbertCurveIndexSpace(keys, min_split_size=16, include_diagonal=False)
+    assert root.size() == 37 * 36 // 2
+    seen, stack, steps = [], [root], 0
+    while stack:
+        sp = stack.pop()
+        d = ix.IndexSpaceDivision()
+        sp.divide(d)
+        assert sum(c.size() for c in d.get_children()) + len(d.get_entries()) == sp.size()
+        stack.extend(d.get_children())
+        seen.extend(d.get_entries())
+        steps += 1
+    assert steps > 1
+    assert len(seen) == len(set(seen)) == 37 * 36 // 2
+    assert all(keys.index(a) < keys.index(b) for a, b in seen)
+
+
+def test_tiling_divide_tiles_then_entries():
+    keys = list(range(10))
+    root = ix.TilingIndexSpace(4, keys, include_diagonal=False)
+    d = ix.IndexSpaceDivision()
+    root.divide(d)
+    assert not d.get_entries()
+    # 3 x 3 tiles of the 10 x 10 matrix, minus the 3 below the diagonal
+    assert len(d.get_children()) == 6
+    total = 0
+    for c in d.get_children():
+        dd = ix.IndexSpaceDivision()
+        c.divide(dd)
+        assert not dd.get_children()
+        total += len(dd.get_entries())
+    assert total == 45
