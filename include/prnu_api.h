@@ -176,6 +176,25 @@ PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int
                   const double *d_sum, const double *d_sumsq, const float *d_absmax,
                   double *d_ncc, prnu_stream stream);
 
+/* Launch shape of the Gram tile kernel (no reference counterpart).
+ * slabs: K slabs per tile (0 = automatic); xcd_remap: 1 keeps the tiles of one
+ * K slab on one XCD so they share it through that XCD's L2. */
+typedef struct {
+    int32_t slabs;
+    int32_t xcd_remap;
+} prnu_gram_config;
+
+/* Set (cfg != NULL) or reset to automatic (cfg == NULL) the Gram launch shape
+ * of `precision` for this context. */
+PRNU_EXPORT int prnu_gram_set_config(prnu_ctx *ctx, int precision, const prnu_gram_config *cfg);
+/* The shape prnu_gram_ncc would use for `count` patterns. */
+PRNU_EXPORT int prnu_gram_get_config(prnu_ctx *ctx, int precision, int count, prnu_gram_config *out);
+/* Autotuner: times the tile kernel on the caller's patterns for candidate K
+ * splits and XCD mappings (`reps` timed launches each, minimum kept), keeps
+ * the fastest in the context and reports it.  Synchronous on `stream`. */
+PRNU_EXPORT int prnu_gram_autotune(prnu_ctx *ctx, const float *const *d_patterns, int count, int precision,
+                                   int reps, prnu_gram_config *best, float *best_ms, prnu_stream stream);
+
 /* --------------------------------------------------------- pair scheduling
  * Host-only (no device call).  Upper-triangle index spaces of
  * R/indexspace/HilbertCurveIndexSpace.java:31-123 and

@@ -72,6 +72,10 @@ _SIGNATURES = {
     "prnu_filter_wiener": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
     "prnu_filter_spectral": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_size, _c_void_p]),
     "prnu_pattern_stats": (_c_int, [_c_void_p, _c_int, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "prnu_gram_set_config": (_c_int, [_c_void_p, _c_int, _c_void_p]),
+    "prnu_gram_get_config": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p]),
+    "prnu_gram_autotune": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_void_p,
+                                    ctypes.POINTER(ctypes.c_float), _c_void_p]),
     "prnu_triangle_size": (_c_i64, [_c_int, _c_int, _c_int, _c_int, _c_int]),
     "prnu_index_divide": (_c_int, [_c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, ctypes.POINTER(_c_int),
                                    _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),

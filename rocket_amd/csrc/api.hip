@@ -185,6 +185,7 @@ int prnu_ctx_destroy(prnu_ctx *ctx) {
     c->ws_gram.release();
     c->ws_small.release();
     c->ws_pce.release();
+    c->ws_fp8.release();
     if (c->fft_w) fft_plan_destroy(c->fft_w);
     if (c->fft_h && c->fft_h != c->fft_w) fft_plan_destroy(c->fft_h);
     if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
@@ -249,6 +250,34 @@ int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int count, int 
     hipStream_t st = pick_stream(c, stream);
     PRNU_TRY(launch_gram_ncc(c, d_patterns, count, precision, d_sum, d_sumsq, d_absmax, d_ncc, st));
     return finish(c, stream);
+}
+
+int prnu_gram_set_config(prnu_ctx *ctx, int precision, const prnu_gram_config *cfg) {
+    PRNU_REQUIRE(ctx, "gram_set_config: null context");
+    PRNU_REQUIRE(precision >= 0 && precision <= 2, "gram_set_config: unknown precision %d", precision);
+    Context *c = reinterpret_cast<Context *>(ctx);
+    if (!cfg) {
+        c->gram_cfg[precision] = prnu_gram_config{0, 1};
+        return PRNU_OK;
+    }
+    PRNU_REQUIRE(cfg->slabs >= 0 && (cfg->xcd_remap == 0 || cfg->xcd_remap == 1),
+                 "gram_set_config: slabs must be >= 0 and xcd_remap 0 or 1");
+    c->gram_cfg[precision] = *cfg;
+    return PRNU_OK;
+}
+
+int prnu_gram_get_config(prnu_ctx *ctx, int precision, int count, prnu_gram_config *out) {
+    PRNU_REQUIRE(ctx && out, "gram_get_config: null pointer");
+    PRNU_REQUIRE(precision >= 0 && precision <= 2 && count >= 1, "gram_get_config: bad precision or count");
+    return gram_resolved_config(reinterpret_cast<Context *>(ctx), count, precision, out);
+}
+
+int prnu_gram_autotune(prnu_ctx *ctx, const float *const *d_patterns, int count, int precision, int reps,
+                       prnu_gram_config *best, float *best_ms, prnu_stream stream) {
+    PRNU_REQUIRE(ctx && d_patterns, "gram_autotune: null pointer");
+    Context *c = reinterpret_cast<Context *>(ctx);
+    PRNU_REQUIRE(c->mode == PRNU_MODE_NCC, "gram_autotune: context is not in NCC mode");
+    return autotune_gram(c, d_patterns, count, precision, reps, best, best_ms, pick_stream(c, stream));
 }
 
 }  // extern "C"

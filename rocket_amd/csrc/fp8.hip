@@ -1,0 +1,311 @@
+// FP8-e4m3 NCC on gfx950: per-pattern scaled quantisation and an MFMA
+// (v_mfma_scale_f32_16x16x128_f8f6f4) tile kernel for C = A B^T over K, in
+// two shapes: the upper triangle of one set (all-pairs Gram) and the full
+// rectangle of a query block against a store (serving scan).
+//
+// Quantisation: scale_i = max|x_i| / 448 (e4m3 max normal), q = rne(x/scale).
+// G_ij = scale_i * scale_j * sum_k q_ik q_jk, accumulated in FP32 per K slab
+// and summed over slabs in FP64.  Representation error is <= 2^-4 relative per
+// element; the FP32 re-score of the serving path removes it for the top hits.
+#include "prnu_internal.hpp"
+
+#include <algorithm>
+
+namespace prnu {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
+
+constexpr float kE4M3Max = 448.0f;
+constexpr int kKT8 = 128;          // K bytes per stage = one MFMA K
+constexpr int kThreads8 = 512;     // 8 waves
+constexpr int kBTile = 256;        // B rows per tile
+
+// ------------------------------------------------------------ quantisation
+// grid (ceil(stride/4096), count), block 256; each thread packs 16 values.
+__global__ __launch_bounds__(256) void quantize_fp8_kernel(const float *const *ptrs, int64_t n, int64_t stride,
+                                                           const float *absmax, uint8_t *q, float *scale) {
+    const int p = blockIdx.y;
+    const float am = absmax[p];
+    const float inv = am > 0.f ? kE4M3Max / am : 1.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scale[p] = am > 0.f ? am / kE4M3Max : 1.0f;
+    const float *x = ptrs[p];
+    const int64_t k0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 16;
+    if (k0 >= stride) return;
+    int packed[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t k = k0 + 4 * g + e;
+            v[e] = k < n ? x[k] * inv : 0.f;
+        }
+        int w = 0;
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+        packed[g] = w;
+    }
+    *reinterpret_cast<int4 *>(q + (int64_t)p * stride + k0) = make_int4(packed[0], packed[1], packed[2], packed[3]);
+}
+
+// ------------------------------------------------------------- tile kernel
+struct Fp8Args {
+    const uint8_t *A;
+    const uint8_t *B;
+    int64_t lda, ldb;  // row strides in bytes (multiples of 128)
+    int ma, nb;        // valid rows
+    int64_t K;         // padded K (multiple of 128)
+    int64_t kchunk;    // K per slab (multiple of 128)
+    int tri;           // 1: upper triangle of A with itself (B == A)
+    int nta, ntb, T, S;
+    int xcd_remap;
+    float *partials;
+};
+
+__device__ inline int2 tri_tile8(int t, int nt) {
+    int I = 0;
+    while (t >= nt - I) {
+        t -= nt - I;
+        ++I;
+    }
+    return make_int2(I, I + t);
+}
+
+// 16-byte chunks of a 128-byte row, XOR-swizzled by row (conflict-free
+// ds_read_b128 for 8 consecutive rows).
+__device__ __forceinline__ int q_off(int row, int chunk) { return row * 128 + ((chunk ^ (row & 7)) << 4); }
+
+template <int AR>
+__global__ __launch_bounds__(kThreads8, 1) void fp8_tile_kernel(Fp8Args a) {
+    constexpr int kRowsL = AR + kBTile;
+    constexpr int kBuf = kRowsL * kKT8;                  // bytes per stage buffer
+    constexpr int WM = AR == 256 ? 2 : 1, WN = 8 / WM;   // wave grid
+    constexpr int RB = AR / WM / 16, CB = kBTile / WN / 16;
+    constexpr int kLoads = kRowsL * 8 / kThreads8;       // 16-byte chunks per thread per stage
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBuf];
+
+    const int bid = blockIdx.x;
+    int t, s;
+    if (a.xcd_remap) {
+        const int local = bid >> 3;
+        t = local % a.T;
+        s = (local / a.T) * 8 + (bid & 7);
+    } else {
+        t = bid % a.T;
+        s = bid / a.T;
+    }
+    if (s >= a.S) return;
+    int I, J;
+    if (a.tri) {
+        const int2 ij = tri_tile8(t, a.nta);
+        I = ij.x;
+        J = ij.y;
+    } else {
+        I = t / a.ntb;
+        J = t % a.ntb;
+    }
+    const bool diag = a.tri && I == J;
+    const int nrows = diag ? AR : kRowsL;
+    const int bbase = diag ? 0 : AR;
+    const int64_t k_begin = (int64_t)s * a.kchunk;
+    const int64_t k_end = min(a.K, k_begin + a.kchunk);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wrow0 = (wave / WN) * (AR / WM);
+    const int wcol0 = (wave % WN) * (kBTile / WN);
+
+    // loader: chunk c = tid + 512*it -> row c >> 3, 16-byte column c & 7.
+    // Row addresses are recomputed per load (row clamped into the set) to
+    // keep registers for the accumulators; absent rows are zeroed on store.
+    const int r0 = tid >> 3;
+    const int col = (tid & 7) << 4;
+    uint32_t valid = 0;
+#pragma unroll
+    for (int it = 0; it < kLoads; ++it) {
+        const int r = r0 + 64 * it;
+        const bool v = (r < AR) ? (I * AR + r < a.ma) : (r < nrows && J * kBTile + (r - AR) < a.nb);
+        valid |= (uint32_t)v << it;
+    }
+    auto row_ptr = [&](int it) -> const uint8_t * {
+        const int r = r0 + 64 * it;
+        if (r < AR) {
+            const int g = min(I * AR + r, a.ma - 1);
+            return a.A + (int64_t)g * a.lda + col;
+        }
+        const int g = min(J * kBTile + (r - AR), a.nb - 1);
+        return a.B + (int64_t)g * a.ldb + col;
+    };
+
+    floatx4 acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    int4 st[kLoads];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int it = 0; it < kLoads; ++it) {
+            typedef const __attribute__((address_space(1))) int __attribute__((ext_vector_type(4))) *gi4;
+            const auto v = *(gi4)(row_ptr(it) + k0);
+            st[it] = make_int4(v[0], v[1], v[2], v[3]);
+        }
+    };
+    auto store = [&](uint8_t *buf) {
+#pragma unroll
+        for (int it = 0; it < kLoads; ++it) {
+            const int c = tid + kThreads8 * it;
+            const int r = c >> 3;
+            if (r < nrows) {
+                int4 v = st[it];
+                if (!((valid >> it) & 1)) v = make_int4(0, 0, 0, 0);
+                *reinterpret_cast<int4 *>(buf + q_off(r, c & 7)) = v;
+            }
+        }
+    };
+    auto compute = [&](const uint8_t *buf) {
+        const int frow = lane & 15, ch = (lane >> 4) * 2;
+        intx8 bf[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            const int r = bbase + wcol0 + cb * 16 + frow;
+            const int4 lo = *reinterpret_cast<const int4 *>(buf + q_off(r, ch));
+            const int4 hi = *reinterpret_cast<const int4 *>(buf + q_off(r, ch + 1));
+            bf[cb] = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int r = wrow0 + rb * 16 + frow;
+            const int4 lo = *reinterpret_cast<const int4 *>(buf + q_off(r, ch));
+            const int4 hi = *reinterpret_cast<const int4 *>(buf + q_off(r, ch + 1));
+            const intx8 af = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af, bf[cb], acc[rb][cb], 0, 0, 0, 127,
+                                                                               0, 127);
+        }
+    };
+
+    const int64_t nst = (k_end - k_begin) / kKT8;
+    load(k_begin);
+    store(lds);
+    __syncthreads();
+    for (int64_t q = 0; q < nst; ++q) {
+        const bool next = q + 1 < nst;
+        if (next) load(k_begin + (q + 1) * kKT8);
+        compute(lds + (q & 1) * kBuf);
+        if (next) store(lds + ((q + 1) & 1) * kBuf);
+        __syncthreads();
+    }
+
+    float *out = a.partials + ((int64_t)t * a.S + s) * (AR * kBTile);
+    const int ocol = lane & 15, orow = (lane >> 4) * 4;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(wrow0 + rb * 16 + orow + r) * kBTile + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
+}
+
+// Sum the slabs in FP64, undo the scales and apply the NCC epilogue
+// G / (sqrt(Sxx_i - Sx_i^2) sqrt(Sxx_j - Sx_j^2)).  grid (AR*256/256, T).
+template <int AR>
+__global__ __launch_bounds__(256) void fp8_reduce_kernel(Fp8Args a, const float *scale_a, const float *scale_b,
+                                                         const double *sum_a, const double *sumsq_a,
+                                                         const double *sum_b, const double *sumsq_b, double *out,
+                                                         int64_t ld_out) {
+    const int t = blockIdx.y;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    int I, J;
+    if (a.tri) {
+        const int2 ij = tri_tile8(t, a.nta);
+        I = ij.x;
+        J = ij.y;
+    } else {
+        I = t / a.ntb;
+        J = t % a.ntb;
+    }
+    const int i = I * AR + e / kBTile;
+    const int j = J * kBTile + e % kBTile;
+    if (i >= a.ma || j >= a.nb) return;
+    if (a.tri && I == J && i > j) return;
+    const float *p = a.partials + (int64_t)t * a.S * (AR * kBTile) + e;
+    double g = 0.0;
+    for (int q = 0; q < a.S; ++q) g += (double)p[(int64_t)q * (AR * kBTile)];
+    g *= (double)scale_a[i] * (double)scale_b[j];
+    const double di = sqrt(sumsq_a[i] - sum_a[i] * sum_a[i]);
+    const double dj = sqrt(sumsq_b[j] - sum_b[j] * sum_b[j]);
+    const double v = g / (di * dj);
+    out[(int64_t)i * ld_out + j] = v;
+    if (a.tri) out[(int64_t)j * ld_out + i] = v;
+}
+
+}  // namespace
+
+int64_t fp8_row_stride(int64_t n) { return (n + kKT8 - 1) / kKT8 * kKT8; }
+
+int launch_quantize_fp8(const float *const *d_ptrs, int count, int64_t n, const float *d_absmax, uint8_t *d_q,
+                        float *d_scale, hipStream_t st) {
+    PRNU_REQUIRE(count >= 1 && count <= 65535 && n >= 1, "quantize_fp8: bad count or length");
+    const int64_t stride = fp8_row_stride(n);
+    dim3 grid((unsigned)((stride + 4095) / 4096), (unsigned)count);
+    hipLaunchKernelGGL(quantize_fp8_kernel, grid, dim3(256), 0, st, d_ptrs, n, stride, d_absmax, d_q, d_scale);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+// C = A B^T in FP8 with NCC epilogue.  tri: B is A and only I <= J tiles run.
+// a_rows: 256 (Gram) or 64 (query blocks).
+int launch_fp8_ncc(Workspace &ws, int a_rows, bool tri, const uint8_t *A, int64_t lda, int ma, const float *scale_a,
+                   const double *sum_a, const double *sumsq_a, const uint8_t *B, int64_t ldb, int nb,
+                   const float *scale_b, const double *sum_b, const double *sumsq_b, int64_t K, double *out,
+                   int64_t ld_out, int slabs, hipStream_t st, Context *timing_ctx) {
+    PRNU_REQUIRE(a_rows == 256 || a_rows == 64, "fp8: a_rows must be 256 or 64");
+    PRNU_REQUIRE(K % kKT8 == 0 && lda % kKT8 == 0 && ldb % kKT8 == 0, "fp8: K and strides must be multiples of 128");
+    Fp8Args a;
+    a.A = A;
+    a.B = tri ? A : B;
+    a.lda = lda;
+    a.ldb = tri ? lda : ldb;
+    a.ma = ma;
+    a.nb = tri ? ma : nb;
+    a.K = K;
+    a.tri = tri ? 1 : 0;
+    a.nta = (ma + a_rows - 1) / a_rows;
+    a.ntb = (a.nb + kBTile - 1) / kBTile;
+    PRNU_REQUIRE(!tri || a_rows == kBTile, "fp8: the triangle needs square tiles");
+    a.T = tri ? a.nta * (a.nta + 1) / 2 : a.nta * a.ntb;
+    const int64_t stages = K / kKT8;
+    int64_t S = slabs > 0 ? slabs : std::max<int64_t>(1, (256 * 4 + a.T - 1) / a.T);
+    S = std::min<int64_t>(S, std::max<int64_t>(1, stages / 8));
+    const int64_t tile_bytes = (int64_t)a_rows * kBTile * sizeof(float);
+    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * a.T)));
+    a.kchunk = ((stages + S - 1) / S) * kKT8;
+    a.S = (int)((K + a.kchunk - 1) / a.kchunk);
+    a.xcd_remap = 1;
+    PRNU_TRY(ws.ensure((size_t)tile_bytes * a.T * a.S));
+    a.partials = static_cast<float *>(ws.ptr);
+    const dim3 grid((unsigned)(a.T * ((a.S + 7) / 8 * 8)));
+    if (timing_ctx) time_begin(timing_ctx, st);
+    if (a_rows == 256)
+        hipLaunchKernelGGL(fp8_tile_kernel<256>, grid, dim3(kThreads8), 0, st, a);
+    else
+        hipLaunchKernelGGL(fp8_tile_kernel<64>, grid, dim3(kThreads8), 0, st, a);
+    PRNU_CHECK_LAUNCH();
+    if (timing_ctx) time_end(timing_ctx, st);
+    const dim3 rgrid((unsigned)(a_rows * kBTile / 256), (unsigned)a.T);
+    if (a_rows == 256)
+        hipLaunchKernelGGL(fp8_reduce_kernel<256>, rgrid, dim3(256), 0, st, a, scale_a, tri ? scale_a : scale_b,
+                           sum_a, sumsq_a, tri ? sum_a : sum_b, tri ? sumsq_a : sumsq_b, out, ld_out);
+    else
+        hipLaunchKernelGGL(fp8_reduce_kernel<64>, rgrid, dim3(256), 0, st, a, scale_a, scale_b, sum_a, sumsq_a,
+                           sum_b, sumsq_b, out, ld_out);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+}  // namespace prnu

@@ -216,6 +216,7 @@ struct GramArgs {
     int nt;  // tiles per side
     int T;   // tiles in the upper triangle, nt*(nt+1)/2
     int S;   // K slabs
+    int xcd_remap;  // 1: the T tiles of a slab run on one XCD
 };
 
 // Linear index over the upper triangle (row-major, I <= J) -> (I, J).
@@ -382,12 +383,19 @@ template <int PREC>
 __global__ __launch_bounds__(kGramThreads, 1) void gram_tile_kernel(GramArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t lds[2 * kRows * 32];  // 2 x 64 KiB stage buffers
 
-    // ---- workgroup -> (tile, slab), XCD-aware: the T tiles of a slab share an XCD
+    // ---- workgroup -> (tile, slab).  Workgroups are dealt round-robin to the
+    // 8 XCDs (bid mod 8); with xcd_remap the T tiles of slab s all have
+    // bid = s mod 8 (mod 8) and run back to back on one XCD.
     const int bid = blockIdx.x;
-    const int xcd = bid & 7;
-    const int local = bid >> 3;
-    const int t = local % a.T;
-    const int s = (local / a.T) * 8 + xcd;
+    int t, s;
+    if (a.xcd_remap) {
+        const int local = bid >> 3;
+        t = local % a.T;
+        s = (local / a.T) * 8 + (bid & 7);
+    } else {
+        t = bid % a.T;
+        s = bid / a.T;
+    }
     if (s >= a.S) return;
     const int2 tile = tri_tile(t, a.nt);
     const bool diag = tile.x == tile.y;
@@ -500,51 +508,149 @@ int launch_pair_ncc(const float *x, const float *y, int64_t n, double *d_out, Wo
     return PRNU_OK;
 }
 
-int launch_gram_ncc(Context *c, const float *const *d_ptrs, int count, int precision, const double *d_sum,
-                    const double *d_sumsq, const float *d_absmax, double *d_ncc, hipStream_t st) {
-    (void)d_absmax;
-    PRNU_REQUIRE(count >= 1, "gram: count must be >= 1");
-    PRNU_REQUIRE(precision == PRNU_PREC_FP32 || precision == PRNU_PREC_SPLIT_BF16,
-                 "gram: precision %d not supported by this entry point", precision);
+namespace {
+
+struct GramPlan {
+    int nt, T;
+    int64_t S, kchunk;
+    int xcd_remap;
+};
+
+// K split of the Gram: `slabs` per tile (0: about 5 workgroups per CU, >= 16
+// stages per slab), partials capped at 2 GiB.
+GramPlan plan_gram(const Context *c, int count, int precision) {
+    GramPlan p;
     const int64_t K = c->pixels();
-    const int nt = (count + kTile - 1) / kTile;
-    const int T = nt * (nt + 1) / 2;
-
-    // K split: about 5 waves of workgroups over 256 CUs, >= 16 stages per slab,
-    // partials capped at 2 GiB.
+    p.nt = (count + kTile - 1) / kTile;
+    p.T = p.nt * (p.nt + 1) / 2;
     const int64_t stages = (K + kKT - 1) / kKT;
-    int64_t S = std::max<int64_t>(1, (256 * 5 + T - 1) / T);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, stages / 16));
+    int64_t S = c->gram_cfg[precision].slabs;
+    if (S <= 0) {
+        S = std::max<int64_t>(1, (256 * 5 + p.T - 1) / p.T);
+        S = std::min<int64_t>(S, std::max<int64_t>(1, stages / 16));
+    }
+    S = std::min<int64_t>(S, stages);
     const int64_t tile_bytes = (int64_t)kTile * kTile * sizeof(float);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * T)));
-    const int64_t kchunk = ((stages + S - 1) / S) * kKT;
-    S = (K + kchunk - 1) / kchunk;
+    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * p.T)));
+    p.kchunk = ((stages + S - 1) / S) * kKT;
+    p.S = (K + p.kchunk - 1) / p.kchunk;
+    p.xcd_remap = c->gram_cfg[precision].xcd_remap;
+    return p;
+}
 
-    const size_t part_bytes = (size_t)tile_bytes * T * S;
-    PRNU_TRY(c->ws_gram.ensure(part_bytes));
-    float *partials = static_cast<float *>(c->ws_gram.ptr);
-
+int launch_gram_tiles(Context *c, const GramPlan &p, const float *const *d_ptrs, int count, int precision,
+                      float *partials, hipStream_t st) {
     GramArgs a;
     a.ptrs = d_ptrs;
     a.partials = partials;
-    a.K = K;
-    a.kchunk = kchunk;
+    a.K = c->pixels();
+    a.kchunk = p.kchunk;
     a.M = count;
-    a.nt = nt;
-    a.T = T;
-    a.S = (int)S;
-    const int64_t S8 = (S + 7) / 8 * 8;
-    const dim3 grid((unsigned)(T * S8));
-    time_begin(c, st);
+    a.nt = p.nt;
+    a.T = p.T;
+    a.S = (int)p.S;
+    a.xcd_remap = p.xcd_remap;
+    const int64_t S8 = p.xcd_remap ? (p.S + 7) / 8 * 8 : p.S;
+    const dim3 grid((unsigned)(p.T * S8));
     if (precision == PRNU_PREC_SPLIT_BF16)
         hipLaunchKernelGGL(gram_tile_kernel<PRNU_PREC_SPLIT_BF16>, grid, dim3(kGramThreads), 0, st, a);
     else
         hipLaunchKernelGGL(gram_tile_kernel<PRNU_PREC_FP32>, grid, dim3(kGramThreads), 0, st, a);
     PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+}  // namespace
+
+int launch_gram_ncc(Context *c, const float *const *d_ptrs, int count, int precision, const double *d_sum,
+                    const double *d_sumsq, const float *d_absmax, double *d_ncc, hipStream_t st) {
+    PRNU_REQUIRE(count >= 1, "gram: count must be >= 1");
+    PRNU_REQUIRE(precision == PRNU_PREC_FP32 || precision == PRNU_PREC_SPLIT_BF16 || precision == PRNU_PREC_FP8,
+                 "gram: unknown precision %d", precision);
+    if (precision == PRNU_PREC_FP8) {
+        // e4m3 copy of the set (rows padded to 128 bytes), then the FP8 triangle
+        PRNU_REQUIRE(count <= 65535, "gram: FP8 supports at most 65535 patterns");
+        const int64_t stride = fp8_row_stride(c->pixels());
+        const size_t qbytes = (size_t)stride * count;
+        PRNU_TRY(c->ws_fp8.ensure(qbytes + 256 + sizeof(float) * count));
+        uint8_t *q = static_cast<uint8_t *>(c->ws_fp8.ptr);
+        float *scale = reinterpret_cast<float *>(q + ((qbytes + 255) & ~size_t(255)));
+        PRNU_TRY(launch_quantize_fp8(d_ptrs, count, c->pixels(), d_absmax, q, scale, st));
+        return launch_fp8_ncc(c->ws_gram, 256, true, q, stride, count, scale, d_sum, d_sumsq, nullptr, 0, 0,
+                              nullptr, nullptr, nullptr, stride, d_ncc, count, c->gram_cfg[precision].slabs, st, c);
+    }
+    const GramPlan p = plan_gram(c, count, precision);
+    const size_t part_bytes = (size_t)kTile * kTile * sizeof(float) * p.T * p.S;
+    PRNU_TRY(c->ws_gram.ensure(part_bytes));
+    float *partials = static_cast<float *>(c->ws_gram.ptr);
+    time_begin(c, st);
+    PRNU_TRY(launch_gram_tiles(c, p, d_ptrs, count, precision, partials, st));
     time_end(c, st);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(kTile * kTile / kThreads, T), dim3(kThreads), 0, st, partials,
-                       (int)S, nt, count, d_sum, d_sumsq, d_ncc);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(kTile * kTile / kThreads, p.T), dim3(kThreads), 0, st, partials,
+                       (int)p.S, p.nt, count, d_sum, d_sumsq, d_ncc);
     PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+int gram_resolved_config(Context *c, int count, int precision, prnu_gram_config *out) {
+    const GramPlan p = plan_gram(c, count, precision);
+    out->slabs = (int32_t)p.S;
+    out->xcd_remap = p.xcd_remap;
+    return PRNU_OK;
+}
+
+// Time the tile kernel over candidate K splits and XCD mappings on the
+// caller's data; keep the fastest in the context.
+int autotune_gram(Context *c, const float *const *d_ptrs, int count, int precision, int reps,
+                  prnu_gram_config *best, float *best_ms, hipStream_t st) {
+    PRNU_REQUIRE(precision == PRNU_PREC_FP32 || precision == PRNU_PREC_SPLIT_BF16,
+                 "gram_autotune: precision %d not supported", precision);
+    PRNU_REQUIRE(count >= 1 && reps >= 1, "gram_autotune: count and reps must be positive");
+    const prnu_gram_config saved = c->gram_cfg[precision];
+    c->gram_cfg[precision] = prnu_gram_config{0, 1};
+    const int64_t s0 = plan_gram(c, count, precision).S;
+    const int64_t cand_s[] = {s0, std::max<int64_t>(8, s0 / 2), s0 * 2, s0 * 4};
+    hipEvent_t e0, e1;
+    PRNU_CHECK_HIP(hipEventCreate(&e0));
+    PRNU_CHECK_HIP(hipEventCreate(&e1));
+    float best_t = 3.4e38f;
+    prnu_gram_config best_cfg = saved;
+    int status = PRNU_OK;
+    for (int64_t S : cand_s) {
+        for (int xcd = 1; xcd >= 0 && status == PRNU_OK; --xcd) {
+            c->gram_cfg[precision] = prnu_gram_config{(int32_t)S, xcd};
+            const GramPlan p = plan_gram(c, count, precision);
+            const size_t part_bytes = (size_t)kTile * kTile * sizeof(float) * p.T * p.S;
+            status = c->ws_gram.ensure(part_bytes);
+            if (status != PRNU_OK) break;
+            float *partials = static_cast<float *>(c->ws_gram.ptr);
+            status = launch_gram_tiles(c, p, d_ptrs, count, precision, partials, st);  // warm-up
+            float tmin = 3.4e38f;
+            for (int r = 0; r < reps && status == PRNU_OK; ++r) {
+                (void)hipEventRecord(e0, st);
+                status = launch_gram_tiles(c, p, d_ptrs, count, precision, partials, st);
+                (void)hipEventRecord(e1, st);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    set_error("gram_autotune: event wait failed");
+                    status = PRNU_EDEVICE;
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                tmin = std::min(tmin, ms);
+            }
+            if (status == PRNU_OK && tmin < best_t) {
+                best_t = tmin;
+                best_cfg = prnu_gram_config{(int32_t)p.S, xcd};
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->gram_cfg[precision] = (status == PRNU_OK) ? best_cfg : saved;
+    if (status != PRNU_OK) return status;
+    if (best) *best = best_cfg;
+    if (best_ms) *best_ms = best_t;
     return PRNU_OK;
 }
 

@@ -61,8 +61,10 @@ struct Context {
     Workspace ws_gram;      // Gram partial tiles
     Workspace ws_small;     // reductions, pointer tables, scalars
     Workspace ws_pce;       // PCE correlation planes
+    Workspace ws_fp8;       // FP8 copy of a pattern set and its scales
     FftPlan *fft_w = nullptr;       // length-w plan
     FftPlan *fft_h = nullptr;       // length-h plan
+    prnu_gram_config gram_cfg[3] = {{0, 1}, {0, 1}, {0, 1}};  // per precision; slabs 0 = automatic
     bool timing = false;            // record events around the dominant kernel
     bool timed = false;             // ev_begin/ev_end hold a measurement
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -100,5 +102,16 @@ int launch_pair_ncc(const float *x, const float *y, int64_t n, double *d_out, Wo
 int launch_gram_ncc(Context *c, const float *const *d_ptrs, int count, int precision,
                     const double *d_sum, const double *d_sumsq, const float *d_absmax,
                     double *d_ncc, hipStream_t st);
+// FP8 (fp8.hip)
+int64_t fp8_row_stride(int64_t n);
+int launch_quantize_fp8(const float *const *d_ptrs, int count, int64_t n, const float *d_absmax, uint8_t *d_q,
+                        float *d_scale, hipStream_t st);
+int launch_fp8_ncc(Workspace &ws, int a_rows, bool tri, const uint8_t *A, int64_t lda, int ma, const float *scale_a,
+                   const double *sum_a, const double *sumsq_a, const uint8_t *B, int64_t ldb, int nb,
+                   const float *scale_b, const double *sum_b, const double *sumsq_b, int64_t K, double *out,
+                   int64_t ld_out, int slabs, hipStream_t st, Context *timing_ctx);
+int gram_resolved_config(Context *c, int count, int precision, prnu_gram_config *out);
+int autotune_gram(Context *c, const float *const *d_ptrs, int count, int precision, int reps,
+                  prnu_gram_config *best, float *best_ms, hipStream_t st);
 
 }  // namespace prnu

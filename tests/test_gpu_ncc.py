@@ -142,3 +142,46 @@ def test_gram_full_size_properties():
     err = np.abs(bf[np.ix_(idx, idx)] - want).max()
     assert err <= NCC_TOL, f"max |dNCC| = {err:.3e}"
     gram.close()
+
+
+@pytest.mark.parametrize("count,h,w", [(64, 512, 512), (300, 200, 321), (7, 16, 9)])
+def test_gram_fp8_within_e4m3_bound(count, h, w):
+    """FP8-e4m3 Gram with per-pattern scale.  e4m3 rounds to 3 mantissa bits
+    (relative error <= 2^-4, rms ~0.036 for uniform mantissas), so an
+    off-diagonal NCC of n-sample patterns errs by ~0.05/sqrt(n) rms; the bound
+    is 6 sigma.  The diagonal carries the E[eps^2] bias (~1.3e-3)."""
+    p = patterns(count, h, w, 4, seed=30 + count, gain=0.2)
+    gram = csi.GramNCC(h, w)
+    got = gram.ncc(p, "fp8").cpu().numpy()
+    want = O.ncc_matrix(p.cpu().numpy())
+    off = ~np.eye(count, dtype=bool)
+    n = h * w
+    assert np.abs(got - want)[off].max() <= 6 * 0.05 / np.sqrt(n)
+    assert np.abs(np.diag(got) - np.diag(want)).max() <= 5e-3
+    assert np.array_equal(got, got.T)
+    gram.close()
+
+
+def test_gram_config_and_autotune():
+    h, w, count = 256, 256, 300
+    p = patterns(count, h, w, 4, seed=41)
+    gram = csi.GramNCC(h, w)
+    ptrs = csi.pattern_pointer_table(p)
+    lib = _lib.load()
+    cfg = (_lib.ctypes.c_int32 * 2)()
+    _lib.check(lib.prnu_gram_get_config(gram._ctx.handle, _lib.PREC_SPLIT_BF16, count, cfg))
+    assert cfg[0] >= 1 and cfg[1] == 1
+    ref = gram.ncc(p, "split_bf16").cpu().numpy()
+    for slabs, xcd in [(8, 0), (16, 1), (3, 0)]:
+        c = (_lib.ctypes.c_int32 * 2)(slabs, xcd)
+        _lib.check(lib.prnu_gram_set_config(gram._ctx.handle, _lib.PREC_SPLIT_BF16, c))
+        got = gram.ncc(p, "split_bf16").cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-6          # only the summation order changes
+    best = (_lib.ctypes.c_int32 * 2)()
+    ms = _lib.ctypes.c_float()
+    _lib.check(lib.prnu_gram_autotune(gram._ctx.handle, ptrs.data_ptr(), count, _lib.PREC_SPLIT_BF16, 2, best,
+                                      _lib.ctypes.byref(ms), torch.cuda.current_stream().cuda_stream))
+    assert best[0] >= 1 and ms.value > 0
+    got = gram.ncc(p, "split_bf16").cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-6
+    gram.close()
