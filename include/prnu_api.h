@@ -176,12 +176,42 @@ PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int
                   const double *d_sum, const double *d_sumsq, const float *d_absmax,
                   double *d_ncc, prnu_stream stream);
 
+/* ----------------------------------------------------------- pattern store
+ * Device-resident pattern set for the batched path, the counterpart of the
+ * device cache slots Rocket keeps preprocessed patterns in
+ * (R/cache/DeviceCache.java:19-44), laid out for the Gram: each pattern is cut
+ * into blocks of 32 values and block b of all slots is contiguous, stored as
+ * split-BF16 (hi, lo) or FP32 (`format` = PRNU_PREC_SPLIT_BF16 or
+ * PRNU_PREC_FP32).  Inserting computes the per-pattern FP64 statistics. */
+typedef struct prnu_store prnu_store;
+
+PRNU_EXPORT int prnu_store_create(int64_t n, int capacity, int format, prnu_store **out);
+PRNU_EXPORT int prnu_store_destroy(prnu_store *store);
+PRNU_EXPORT int prnu_store_info(const prnu_store *store, int64_t *n, int *capacity, int *format, size_t *bytes);
+/* Copy `count` FP32 patterns (DEVICE array of pointers, n floats each) into
+ * slots [first_slot, first_slot + count) and record sum x, sum fl32(x*x)
+ * (FP64) and max|x| of each. */
+PRNU_EXPORT int prnu_store_put(prnu_store *store, const float *const *d_patterns, int count, int first_slot,
+                               prnu_stream stream);
+/* Device arrays (capacity entries) of the statistics recorded by put. */
+PRNU_EXPORT int prnu_store_stats(const prnu_store *store, const double **d_sum, const double **d_sumsq,
+                                 const float **d_absmax);
+/* All-pairs NCC of slots [first_slot, first_slot + count) in the store's
+ * format, same epilogue and output as prnu_gram_ncc.  `ctx` supplies the
+ * workspace, the launch shape (its prnu_gram_config of the store's format)
+ * and kernel timing. */
+PRNU_EXPORT int prnu_store_gram_ncc(prnu_ctx *ctx, prnu_store *store, int first_slot, int count, double *d_ncc,
+                                    prnu_stream stream);
+
 /* Launch shape of the Gram tile kernel (no reference counterpart).
  * slabs: K slabs per tile (0 = automatic); xcd_remap: 1 keeps the tiles of one
- * K slab on one XCD so they share it through that XCD's L2. */
+ * K slab on one XCD so they share it through that XCD's L2; variant (split-BF16
+ * over pointer tables only): 0 = 8 waves, one K stage in flight; 1 = 4 waves
+ * with the full register file each, two K stages in flight. */
 typedef struct {
     int32_t slabs;
     int32_t xcd_remap;
+    int32_t variant;
 } prnu_gram_config;
 
 /* Set (cfg != NULL) or reset to automatic (cfg == NULL) the Gram launch shape

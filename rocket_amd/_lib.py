@@ -72,6 +72,14 @@ _SIGNATURES = {
     "prnu_filter_wiener": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
     "prnu_filter_spectral": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_size, _c_void_p]),
     "prnu_pattern_stats": (_c_int, [_c_void_p, _c_int, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "prnu_store_create": (_c_int, [_c_i64, _c_int, _c_int, ctypes.POINTER(_c_void_p)]),
+    "prnu_store_destroy": (_c_int, [_c_void_p]),
+    "prnu_store_info": (_c_int, [_c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                                 ctypes.POINTER(_c_size)]),
+    "prnu_store_put": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p]),
+    "prnu_store_stats": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p), ctypes.POINTER(_c_void_p),
+                                  ctypes.POINTER(_c_void_p)]),
+    "prnu_store_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p]),
     "prnu_gram_set_config": (_c_int, [_c_void_p, _c_int, _c_void_p]),
     "prnu_gram_get_config": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p]),
     "prnu_gram_autotune": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_void_p,

@@ -369,3 +369,58 @@ class GramNCC:
 
     def close(self) -> None:
         self._ctx.close()
+
+
+class PatternStore:
+    """Device-resident, K-blocked pattern set for the batched NCC path (see
+    ``prnu_store_*`` in include/prnu_api.h): ``format`` "split_bf16" or "fp32".
+    The role of Rocket's device cache slots (``R/cache/DeviceCache.java``),
+    laid out so that one Gram K stage reads contiguous memory."""
+
+    def __init__(self, n: int, capacity: int, format: str = "split_bf16"):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.prnu_store_create(int(n), int(capacity), _lib.PRECISIONS[format], ctypes.byref(h)),
+                   "prnu_store_create")
+        self.handle = h
+        self.n, self.capacity, self.format = int(n), int(capacity), format
+
+    def put(self, patterns, first_slot: int = 0, sync: bool = True) -> None:
+        ptrs = pattern_pointer_table(patterns)
+        _lib.call("prnu_store_put", self.handle, ptrs.data_ptr(), int(ptrs.numel()), int(first_slot), _stream())
+        _sync(sync)
+
+    def put_ptrs(self, ptrs, count: int, first_slot: int = 0, sync: bool = True) -> None:
+        _lib.call("prnu_store_put", self.handle, ptrs.data_ptr(), int(count), int(first_slot), _stream())
+        _sync(sync)
+
+    def stats(self, count: int, first_slot: int = 0):
+        """(sum, sumsq, absmax) of slots [first, first+count) as CPU numpy arrays."""
+        import numpy as np
+        s, ss, am = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.call("prnu_store_stats", self.handle, ctypes.byref(s), ctypes.byref(ss), ctypes.byref(am))
+        out = []
+        for ptr, dt, size in [(s, np.float64, 8), (ss, np.float64, 8), (am, np.float32, 4)]:
+            host = np.empty(count, dtype=dt)
+            _lib.call("prnu_memcpy_d2h", host.ctypes.data, ptr.value + size * first_slot, size * count, None)
+            out.append(host)
+        return tuple(out)
+
+    def gram(self, gram_ctx: "GramNCC", count: int, first_slot: int = 0, out=None, sync: bool = True):
+        if out is None:
+            out = torch.empty((count, count), dtype=torch.float64, device="cuda")
+        _lib.call("prnu_store_gram_ncc", gram_ctx._ctx.handle, self.handle, int(first_slot), int(count),
+                  out.data_ptr(), _stream())
+        _sync(sync)
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().prnu_store_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -257,11 +257,12 @@ int prnu_gram_set_config(prnu_ctx *ctx, int precision, const prnu_gram_config *c
     PRNU_REQUIRE(precision >= 0 && precision <= 2, "gram_set_config: unknown precision %d", precision);
     Context *c = reinterpret_cast<Context *>(ctx);
     if (!cfg) {
-        c->gram_cfg[precision] = prnu_gram_config{0, 1};
+        c->gram_cfg[precision] = prnu_gram_config{0, 1, 0};
         return PRNU_OK;
     }
-    PRNU_REQUIRE(cfg->slabs >= 0 && (cfg->xcd_remap == 0 || cfg->xcd_remap == 1),
-                 "gram_set_config: slabs must be >= 0 and xcd_remap 0 or 1");
+    PRNU_REQUIRE(cfg->slabs >= 0 && (cfg->xcd_remap == 0 || cfg->xcd_remap == 1) &&
+                     (cfg->variant == 0 || cfg->variant == 1),
+                 "gram_set_config: slabs must be >= 0, xcd_remap and variant 0 or 1");
     c->gram_cfg[precision] = *cfg;
     return PRNU_OK;
 }

@@ -448,6 +448,183 @@ __global__ __launch_bounds__(kGramThreads, 1) void gram_tile_kernel(GramArgs a) 
                 out[(wrow0 + rb * 16 + orow + r) * kTile + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
 }
 
+// Variant 1 of the split-BF16 tile kernel: 4 waves (one per SIMD, so each
+// has the whole 512-entry register file), wave tile 128 x 128 with the
+// accumulators in AGPRs, and two K stages of global loads in flight (the
+// latency of a K stage that misses L2 is longer than one stage of MFMA work).
+// Row pointers live in LDS.  Same LDS layout and math as gram_tile_kernel.
+constexpr int kG4Threads = 256;
+
+template <bool FAST>
+__device__ __forceinline__ void g4_load(float4 (&st)[16], const float *const *rowp, const uint16_t *rowv, int tid,
+                                        int64_t k0, int64_t k_end, int64_t K) {
+    const int r_lo = tid >> 3, c4 = (tid & 7) * 4;
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const float *p = rowp[r_lo + 32 * it];
+        if (FAST) {
+            const int64_t kk = k0 + c4;
+            const int64_t kc = kk < K ? kk : K - 4;
+            typedef const __attribute__((address_space(1))) floatx4 *gf4;
+            const floatx4 v = *(gf4)(p + kc);
+            st[it] = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+            const bool ok = rowv[r_lo + 32 * it] != 0;
+            float e[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t kk = k0 + c4 + q;
+                e[q] = (ok && kk < k_end) ? p[kk] : 0.f;
+            }
+            st[it] = make_float4(e[0], e[1], e[2], e[3]);
+        }
+    }
+}
+
+template <bool FAST>
+__device__ __forceinline__ void g4_store(const float4 (&st)[16], uint32_t *buf, const uint16_t *rowv, int nrows,
+                                         int tid, int64_t k0, int64_t k_end) {
+    const int r_lo = tid >> 3, c4 = (tid & 7) * 4;
+    const bool kin = k0 + c4 < k_end;
+    uint16_t *hi = reinterpret_cast<uint16_t *>(buf);
+    uint16_t *lo = hi + kRows * 32;
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int r = r_lo + 32 * it;
+        if (r < nrows) {
+            float4 v = st[it];
+            if (FAST && !(rowv[r] && kin)) v = make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint16_t hx = bf16_bits(v.x), hy = bf16_bits(v.y);
+            const uint16_t hz = bf16_bits(v.z), hw = bf16_bits(v.w);
+            const uint16_t lx = bf16_bits(v.x - bf16_val(hx));
+            const uint16_t ly = bf16_bits(v.y - bf16_val(hy));
+            const uint16_t lz = bf16_bits(v.z - bf16_val(hz));
+            const uint16_t lw = bf16_bits(v.w - bf16_val(hw));
+            const int off = bf_off(r, c4 >> 3) + (c4 & 4);
+            *reinterpret_cast<uint2 *>(hi + off) =
+                make_uint2((uint32_t)hx | ((uint32_t)hy << 16), (uint32_t)hz | ((uint32_t)hw << 16));
+            *reinterpret_cast<uint2 *>(lo + off) =
+                make_uint2((uint32_t)lx | ((uint32_t)ly << 16), (uint32_t)lz | ((uint32_t)lw << 16));
+        }
+    }
+}
+
+__device__ __forceinline__ void g4_compute(const uint32_t *buf, int bbase, int wrow0, int wcol0, int lane,
+                                           floatx4 (&acc)[8][8]) {
+    const uint16_t *hi = reinterpret_cast<const uint16_t *>(buf);
+    const uint16_t *lo = hi + kRows * 32;
+    const int frow = lane & 15, chunk = lane >> 4;
+    bf16x8 bh[8], bl[8];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) {
+        const int off = bf_off(bbase + wcol0 + cb * 16 + frow, chunk);
+        bh[cb] = *reinterpret_cast<const bf16x8 *>(hi + off);
+        bl[cb] = *reinterpret_cast<const bf16x8 *>(lo + off);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb) {
+        const int off = bf_off(wrow0 + rb * 16 + frow, chunk);
+        const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(hi + off);
+        const bf16x8 al = *reinterpret_cast<const bf16x8 *>(lo + off);
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+        }
+    }
+}
+
+template <bool FAST>
+__device__ __forceinline__ void g4_mainloop(uint32_t *lds, const float *const *rowp, const uint16_t *rowv, int nrows,
+                                            int bbase, int wrow0, int wcol0, int64_t k_begin, int64_t k_end,
+                                            int64_t K, floatx4 (&acc)[8][8]) {
+    constexpr int kBuf = kRows * 32;
+    const int tid = threadIdx.x, lane = tid & 63;
+    float4 s0[16], s1[16];
+    const int64_t nst = (k_end - k_begin + kKT - 1) / kKT;
+    // prologue: stage 0 into LDS buffer 0, stage 1 in flight in s1
+    g4_load<FAST>(s0, rowp, rowv, tid, k_begin, k_end, K);
+    if (nst > 1) g4_load<FAST>(s1, rowp, rowv, tid, k_begin + kKT, k_end, K);
+    g4_store<FAST>(s0, lds, rowv, nrows, tid, k_begin, k_end);
+    __syncthreads();
+    // iteration s: LDS buf[s&1] holds stage s, stage s+1 is in flight in the
+    // other register set; issue stage s+2, compute s, then land s+1.
+    for (int64_t s = 0; s < nst; s += 2) {
+        {
+            const int64_t k2 = k_begin + (s + 2) * kKT;
+            if (s + 2 < nst) g4_load<FAST>(s0, rowp, rowv, tid, k2, k_end, K);
+            g4_compute(lds, bbase, wrow0, wcol0, lane, acc);
+            if (s + 1 < nst) g4_store<FAST>(s1, lds + kBuf, rowv, nrows, tid, k_begin + (s + 1) * kKT, k_end);
+            __syncthreads();
+        }
+        if (s + 1 < nst) {
+            const int64_t k3 = k_begin + (s + 3) * kKT;
+            if (s + 3 < nst) g4_load<FAST>(s1, rowp, rowv, tid, k3, k_end, K);
+            g4_compute(lds + kBuf, bbase, wrow0, wcol0, lane, acc);
+            if (s + 2 < nst) g4_store<FAST>(s0, lds, rowv, nrows, tid, k_begin + (s + 2) * kKT, k_end);
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kG4Threads, 1) void gram_tile4_kernel(GramArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[2 * kRows * 32];  // 2 x 64 KiB stage buffers
+    __shared__ const float *rowp[kRows];                                     // row pointers of this tile
+    __shared__ uint16_t rowv[kRows];                                         // row present
+    const int bid = blockIdx.x;
+    int t, s;
+    if (a.xcd_remap) {
+        const int local = bid >> 3;
+        t = local % a.T;
+        s = (local / a.T) * 8 + (bid & 7);
+    } else {
+        t = bid % a.T;
+        s = bid / a.T;
+    }
+    if (s >= a.S) return;
+    const int2 tile = tri_tile(t, a.nt);
+    const bool diag = tile.x == tile.y;
+    const int nrows = diag ? kTile : kRows;
+    const int64_t k_begin = (int64_t)s * a.kchunk;
+    const int64_t k_end = min(a.K, k_begin + a.kchunk);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wrow0 = (wave >> 1) * 128, wcol0 = (wave & 1) * 128;
+    const int bbase = diag ? 0 : kTile;
+
+    const float *any_row = a.ptrs[min(tile.x * kTile, a.M - 1)];
+    int aligned = (a.K & 3) == 0;
+    for (int r = tid; r < kRows; r += kG4Threads) {
+        const int g = (r < kTile) ? tile.x * kTile + r : tile.y * kTile + (r - kTile);
+        const bool v = r < nrows && g < a.M;
+        const float *p = v ? a.ptrs[g] : any_row;
+        rowp[r] = p;
+        rowv[r] = v;
+        if (reinterpret_cast<uintptr_t>(p) & 15) aligned = 0;
+    }
+    aligned = __syncthreads_and(aligned);
+
+    floatx4 acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (aligned)
+        g4_mainloop<true>(lds, rowp, rowv, nrows, bbase, wrow0, wcol0, k_begin, k_end, a.K, acc);
+    else
+        g4_mainloop<false>(lds, rowp, rowv, nrows, bbase, wrow0, wcol0, k_begin, k_end, a.K, acc);
+
+    float *out = a.partials + ((int64_t)t * a.S + s) * (kTile * kTile);
+    const int ocol = lane & 15, orow = (lane >> 4) * 4;
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(wrow0 + rb * 16 + orow + r) * kTile + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
+}
+
 // Sum the S partials of each element in FP64 (fixed order) and apply the NCC
 // epilogue of normalizedcrosscorrelation.cu:121-123.  grid (256, T), block 256.
 __global__ __launch_bounds__(kThreads) void gram_reduce_kernel(const float *__restrict__ partials, int S,
@@ -514,6 +691,7 @@ struct GramPlan {
     int nt, T;
     int64_t S, kchunk;
     int xcd_remap;
+    int variant;
 };
 
 // K split of the Gram: `slabs` per tile (0: about 5 workgroups per CU, >= 16
@@ -535,6 +713,7 @@ GramPlan plan_gram(const Context *c, int count, int precision) {
     p.kchunk = ((stages + S - 1) / S) * kKT;
     p.S = (K + p.kchunk - 1) / p.kchunk;
     p.xcd_remap = c->gram_cfg[precision].xcd_remap;
+    p.variant = precision == PRNU_PREC_SPLIT_BF16 ? c->gram_cfg[precision].variant : 0;
     return p;
 }
 
@@ -552,7 +731,9 @@ int launch_gram_tiles(Context *c, const GramPlan &p, const float *const *d_ptrs,
     a.xcd_remap = p.xcd_remap;
     const int64_t S8 = p.xcd_remap ? (p.S + 7) / 8 * 8 : p.S;
     const dim3 grid((unsigned)(p.T * S8));
-    if (precision == PRNU_PREC_SPLIT_BF16)
+    if (precision == PRNU_PREC_SPLIT_BF16 && p.variant == 1)
+        hipLaunchKernelGGL(gram_tile4_kernel, grid, dim3(kG4Threads), 0, st, a);
+    else if (precision == PRNU_PREC_SPLIT_BF16)
         hipLaunchKernelGGL(gram_tile_kernel<PRNU_PREC_SPLIT_BF16>, grid, dim3(kGramThreads), 0, st, a);
     else
         hipLaunchKernelGGL(gram_tile_kernel<PRNU_PREC_FP32>, grid, dim3(kGramThreads), 0, st, a);
@@ -596,6 +777,7 @@ int gram_resolved_config(Context *c, int count, int precision, prnu_gram_config 
     const GramPlan p = plan_gram(c, count, precision);
     out->slabs = (int32_t)p.S;
     out->xcd_remap = p.xcd_remap;
+    out->variant = p.variant;
     return PRNU_OK;
 }
 
@@ -607,7 +789,7 @@ int autotune_gram(Context *c, const float *const *d_ptrs, int count, int precisi
                  "gram_autotune: precision %d not supported", precision);
     PRNU_REQUIRE(count >= 1 && reps >= 1, "gram_autotune: count and reps must be positive");
     const prnu_gram_config saved = c->gram_cfg[precision];
-    c->gram_cfg[precision] = prnu_gram_config{0, 1};
+    c->gram_cfg[precision] = prnu_gram_config{0, 1, 0};
     const int64_t s0 = plan_gram(c, count, precision).S;
     const int64_t cand_s[] = {s0, std::max<int64_t>(8, s0 / 2), s0 * 2, s0 * 4};
     hipEvent_t e0, e1;
@@ -616,9 +798,11 @@ int autotune_gram(Context *c, const float *const *d_ptrs, int count, int precisi
     float best_t = 3.4e38f;
     prnu_gram_config best_cfg = saved;
     int status = PRNU_OK;
+    const int nvar = precision == PRNU_PREC_SPLIT_BF16 ? 2 : 1;
+    for (int var = 0; var < nvar; ++var)
     for (int64_t S : cand_s) {
         for (int xcd = 1; xcd >= 0 && status == PRNU_OK; --xcd) {
-            c->gram_cfg[precision] = prnu_gram_config{(int32_t)S, xcd};
+            c->gram_cfg[precision] = prnu_gram_config{(int32_t)S, xcd, var};
             const GramPlan p = plan_gram(c, count, precision);
             const size_t part_bytes = (size_t)kTile * kTile * sizeof(float) * p.T * p.S;
             status = c->ws_gram.ensure(part_bytes);
@@ -641,7 +825,7 @@ int autotune_gram(Context *c, const float *const *d_ptrs, int count, int precisi
             }
             if (status == PRNU_OK && tmin < best_t) {
                 best_t = tmin;
-                best_cfg = prnu_gram_config{(int32_t)p.S, xcd};
+                best_cfg = prnu_gram_config{(int32_t)p.S, xcd, var};
             }
         }
     }
