@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--precision", default="split_bf16", choices=["split_bf16", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the query-serving leg")
+    ap.add_argument("--serving-seconds", type=float, default=8.0)
+    ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     return ap.parse_args()
 
 
@@ -114,6 +117,130 @@ def cpu_baseline(torch, p):
     return {"value": count / dt, "unit": "pairs/s", "cores": int(threads), "kind": "port",
             "sample": f"{count} pair evaluations over the {len(uniq)} pairs of {sample} of the 1024 patterns "
                       f"(2048x1536), C port of NormalizedCrossCorrelation.applyCPU, OpenMP, {dt:.1f} s"}
+
+
+SERVE_H, SERVE_W, SERVE_STORE, SERVE_RATE = 1024, 1024, 20000, 200.0
+
+
+def synth_query_images(torch, fingerprints, owners, seed):
+    """u8 (len(owners), h, w) images on the GPU: I = clamp(round(S*(1+0.02*K) + N(0, 2^2)), 0, 255)
+    with S a sum of 8 random-phase cosines (frequencies <= 4 per frame) rescaled
+    to [40, 215] and K the owner's fingerprint (a fresh N(0,1) field for -1)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    h, w = SERVE_H, SERVE_W
+    y = torch.arange(h, device="cuda", dtype=torch.float32)[:, None]
+    x = torch.arange(w, device="cuda", dtype=torch.float32)[None, :]
+    out = torch.empty((len(owners), h, w), dtype=torch.uint8, device="cuda")
+    for i, o in enumerate(owners):
+        f = torch.randint(0, 5, (8, 2), generator=g, device="cuda").float()
+        f[(f[:, 0] == 0) & (f[:, 1] == 0), 0] = 1.0
+        ph = torch.rand(8, generator=g, device="cuda") * 6.283185307
+        am = 0.5 + 0.5 * torch.rand(8, generator=g, device="cuda")
+        s = sum(am[k] * torch.cos(6.283185307 * (f[k, 0] * x / w + f[k, 1] * y / h) + ph[k]) for k in range(8))
+        s = 40.0 + 175.0 * (s - s.min()) / (s.max() - s.min())
+        k = fingerprints[o].view(h, w) if o >= 0 else torch.randn((h, w), generator=g, device="cuda")
+        img = s * (1.0 + 0.02 * k) + 2.0 * torch.randn((h, w), generator=g, device="cuda")
+        out[i] = img.round().clamp_(0, 255).to(torch.uint8)
+    return out
+
+
+def serving_bench(torch, duration: float):
+    """Poisson bursts (uniform 1..64 queries, mean rate 200 q/s) against a
+    20,000-fingerprint FP8 store at 1024x1024; p50/p99 latency from submit
+    to result, queries/s over the run."""
+    from rocket_amd import serving
+    n = SERVE_H * SERVE_W
+    t0 = time.time()
+    store = serving.FingerprintStore(SERVE_H, SERVE_W, SERVE_STORE)
+    g = torch.Generator(device="cuda").manual_seed(4242)
+    known = None
+    chunk = 500
+    for b in range(0, SERVE_STORE, chunk):
+        f = torch.randn((chunk, n), generator=g, device="cuda")
+        f -= f.mean(dim=1, keepdim=True)
+        store.put(f, first_slot=b)
+        if known is None:
+            known = f[:64].clone()          # cameras the queries may come from
+        del f
+    torch.cuda.synchronize()
+    log(f"serving: store of {SERVE_STORE} fingerprints ({store.info()['bytes'] / 1e9:.0f} GB) in {time.time() - t0:.1f} s")
+    rng = np.random.default_rng(20261202)
+    bursts, t, total = [], 0.0, 0
+    while t < duration:
+        t += rng.exponential(32.5 / SERVE_RATE)
+        m = int(rng.integers(1, 65))
+        bursts.append((t, m))
+        total += m
+    owners = [int(rng.integers(0, 64)) if rng.random() < 0.5 else -1 for _ in range(total)]
+    imgs = []
+    for b in range(0, total, 128):
+        imgs.append(synth_query_images(torch, known, owners[b:b + 128], seed=b).cpu().numpy())
+    imgs = np.concatenate(imgs)
+    srv = serving.Server(store, k=10, max_batch=64, slots=2).start()
+    reqs = []
+    start = time.perf_counter()
+    q = 0
+    for (tb, m) in bursts:
+        now = time.perf_counter() - start
+        if tb > now:
+            time.sleep(tb - now)
+        for _ in range(m):
+            reqs.append(srv.submit(imgs[q]))
+            q += 1
+    for r in reqs:
+        r.wait(120)
+    wall = max(r.t_done for r in reqs) - start
+    srv.stop()
+    lat = np.array([r.latency for r in reqs]) * 1000.0
+    hit = [r.idx[0] == o for r, o in zip(reqs, owners) if o >= 0]
+    res = {"queries": len(reqs), "bursts": len(bursts), "offered_qps": len(reqs) / bursts[-1][0],
+           "qps": len(reqs) / wall, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_batch": float(np.mean(srv.batches)), "max_batch": int(max(srv.batches)),
+           "top1_known_camera": float(np.mean(hit)) if hit else None,
+           "store": f"{SERVE_STORE} x {SERVE_W}x{SERVE_H} FP8-e4m3 + FP32 originals", "k": 10}
+    store.close()
+    return res
+
+
+def extraction_bench(torch, batch=32, pce_spectra=32):
+    """PRNU extraction of RGB8 frames of the workload size (one batched call per
+    `batch` frames) in images/s, and the PCE of all pairs of `pce_spectra`
+    extracted spectra (reference variant) in pairs/s."""
+    from rocket_amd import _lib, csi
+    lib = _lib.load()
+    n = HEIGHT * WIDTH
+    g = torch.Generator(device="cuda").manual_seed(77)
+    imgs = torch.randint(0, 256, (batch, 3 * n), generator=g, device="cuda", dtype=torch.uint8)
+    res = {}
+    for mode, name in ((_lib.MODE_NCC, "ncc"), (_lib.MODE_PCE, "pce")):
+        f = csi.PRNUFilter(HEIGHT, WIDTH, apply_spectral_filter=(mode == _lib.MODE_PCE))
+        out = torch.empty((batch, f.get_output_size() // 4), device="cuda")
+        f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8)            # warm-up, sizes the workspace
+        reps = 3
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8, sync=False)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / reps
+        res[f"{name}_images_per_s"] = batch / dt
+        res[f"{name}_ms_per_image"] = 1000.0 * dt / batch
+        if mode == _lib.MODE_PCE:
+            m = min(pce_spectra, batch)
+            cmp = csi.PeakToCorrelationEnergy(HEIGHT, WIDTH, _lib.PCE_REFERENCE, _ctx=f._ctx)
+            pairs = np.array([(i, j) for i in range(m) for j in range(i + 1, m)], dtype=np.int32)
+            cmp.pairs(out[:m], pairs)                             # warm-up
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            cmp.pairs(out[:m], pairs, sync=False)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            res["pce_pairs_per_s"] = len(pairs) / dt
+            res["pce_pairs"] = int(len(pairs))
+        f.cleanup()
+        del out
+    res.update({"batch": batch, "input": f"RGB8 {WIDTH}x{HEIGHT}"})
+    return res
 
 
 def main():
@@ -221,6 +348,23 @@ def main():
         if cpu:
             log(f"CPU: {cpu['value']:.4g} pairs/s on {cpu['cores']} threads")
 
+    del p                                               # free 12.9 GB before the other legs
+    torch.cuda.empty_cache()
+    extract = None
+    if not args.no_extract:
+        log("extraction / PCE ...")
+        extract = extraction_bench(torch)
+        log(f"extraction: {extract['ncc_images_per_s']:.1f} images/s (NCC patterns), "
+            f"{extract['pce_images_per_s']:.1f} images/s (PCE spectra); PCE {extract['pce_pairs_per_s']:.1f} pairs/s")
+
+    serve = None
+    if not args.no_serving:
+        gram.close()
+        torch.cuda.empty_cache()
+        log("serving ...")
+        serve = serving_bench(torch, args.serving_seconds)
+        log(f"serving: p50 {serve['p50_ms']:.1f} ms, p99 {serve['p99_ms']:.1f} ms, {serve['qps']:.0f} q/s")
+
     line = {
         "metric": "NCC pairs/s, split-BF16 MFMA vs FP32 at |dNCC|<=1e-5; query p50/p99 latency",
         "value": value,
@@ -240,6 +384,8 @@ def main():
         "max_abs_dncc_vs_fp64": dncc,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serve,
+        "extraction": extract,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

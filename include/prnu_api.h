@@ -121,6 +121,16 @@ PRNU_EXPORT int64_t prnu_max_output_size(const prnu_ctx *ctx);
 PRNU_EXPORT int prnu_preprocess(prnu_ctx *ctx, int input_format, const void *d_in, size_t in_bytes,
                     void *d_out, size_t out_capacity, size_t *out_bytes, prnu_stream stream);
 
+/* Batched preprocessInputGPU: `count` images, in_stride BYTES apart, into
+ * `count` patterns, out_stride BYTES apart, with one launch per filter stage
+ * for the whole batch (the reference calls preprocessInputGPU once per file,
+ * R/scheduler/ApplicationContext.java:66; this is the same computation for
+ * many files).  Each output equals prnu_preprocess of that image alone.
+ * in_bytes / out_capacity bound the whole strided range. */
+PRNU_EXPORT int prnu_preprocess_batch(prnu_ctx *ctx, int input_format, const void *d_in, size_t in_bytes,
+                                      int64_t in_stride, int count, void *d_out, size_t out_capacity,
+                                      int64_t out_stride, prnu_stream stream);
+
 /* correlateGPU(K, CudaMem left, K, CudaMem right, CudaMem output)
  * (R/scheduler/ApplicationContext.java:81, CommonSourceIdentificationContext.java:111-115,
  * PatternComparator.applyGPU at CSI/kernels/compare/PatternComparator.java:45):
@@ -136,6 +146,14 @@ PRNU_EXPORT int prnu_correlate(prnu_ctx *ctx, const void *d_left, size_t left_by
  * PeakToCorrelationEnergy.compare (CSI/kernels/compare/PeakToCorrelationEnergy.java:202-232). */
 PRNU_EXPORT int prnu_correlate_pce(prnu_ctx *ctx, int variant, const void *d_left, const void *d_right,
                        double *d_out, prnu_stream stream);
+
+/* PCE of `npairs` pairs in batches (no reference counterpart: replaces
+ * `npairs` correlateGPU calls in PCE mode).  d_spectra is a DEVICE array of
+ * pointers to R2C half spectra (as prnu_preprocess writes them in PCE mode),
+ * d_pairs a DEVICE array of 2*npairs int32 indices into it, d_out npairs
+ * doubles.  PCE contexts only. */
+PRNU_EXPORT int prnu_pce_pairs(prnu_ctx *ctx, int variant, const void *const *d_spectra, const int32_t *d_pairs,
+                               int npairs, double *d_out, prnu_stream stream);
 
 /* ------------------------------------------------------- single filters
  * The individual filter objects of PRNUFilter (PRNUFilter.java:157-175), for
@@ -202,6 +220,31 @@ PRNU_EXPORT int prnu_store_stats(const prnu_store *store, const double **d_sum, 
  * and kernel timing. */
 PRNU_EXPORT int prnu_store_gram_ncc(prnu_ctx *ctx, prnu_store *store, int first_slot, int count, double *d_ncc,
                                     prnu_stream stream);
+
+/* ------------------------------------------------------ fingerprint serving
+ * No reference counterpart (the reference has no serving mode).  A device-
+ * resident store of camera fingerprints: an FP8-e4m3 copy with a per-slot
+ * FP32 scale for scanning, the FP32 originals for exact re-scoring, and FP64
+ * sum / sum of squares per slot. */
+typedef struct prnu_fpstore prnu_fpstore;
+
+PRNU_EXPORT int prnu_fpstore_create(int64_t n, int capacity, prnu_fpstore **out);
+PRNU_EXPORT int prnu_fpstore_destroy(prnu_fpstore *store);
+/* size = 1 + the highest slot filled; bytes = device memory held */
+PRNU_EXPORT int prnu_fpstore_info(const prnu_fpstore *store, int64_t *n, int *capacity, int *size, size_t *bytes);
+/* Quantise and copy `count` FP32 fingerprints (DEVICE pointer table) into
+ * slots [first_slot, first_slot + count). */
+PRNU_EXPORT int prnu_fpstore_put(prnu_fpstore *store, const float *const *d_patterns, int count, int first_slot,
+                                 prnu_stream stream);
+/* Top-k (1 <= k <= 16) fingerprints of each of `nq` query patterns (DEVICE
+ * pointer table, the context's h*w floats each): FP8 MFMA scan of every
+ * filled slot, then the k best re-scored exactly (FP32 products, FP64 sums,
+ * NCC of normalizedcrosscorrelation.cu:121-123) and sorted by that score.
+ * d_idx / d_score: nq*k int32 / doubles.  d_scan (optional, nq*size doubles)
+ * receives the FP8 NCC estimates.  A query's result does not depend on the
+ * other queries of the call. */
+PRNU_EXPORT int prnu_fpstore_query(prnu_ctx *ctx, prnu_fpstore *store, const float *const *d_queries, int nq, int k,
+                                   int32_t *d_idx, double *d_score, double *d_scan, prnu_stream stream);
 
 /* Launch shape of the Gram tile kernel (no reference counterpart).
  * slabs: K slabs per tile (0 = automatic); xcd_remap: 1 keeps the tiles of one

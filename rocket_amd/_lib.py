@@ -63,9 +63,12 @@ _SIGNATURES = {
     "prnu_max_output_size": (_c_i64, [_c_void_p]),
     "prnu_preprocess": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_size, _c_void_p, _c_size,
                                  ctypes.POINTER(_c_size), _c_void_p]),
+    "prnu_preprocess_batch": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_size, _c_i64, _c_int, _c_void_p, _c_size,
+                                       _c_i64, _c_void_p]),
     "prnu_correlate": (_c_int, [_c_void_p, _c_void_p, _c_size, _c_void_p, _c_size, _c_void_p, _c_size,
                                 ctypes.POINTER(_c_size), _c_void_p]),
     "prnu_correlate_pce": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "prnu_pce_pairs": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_void_p]),
     "prnu_filter_grayscale": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_size, _c_void_p, _c_void_p]),
     "prnu_filter_fastnoise": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
     "prnu_filter_zeromean": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
@@ -80,6 +83,13 @@ _SIGNATURES = {
     "prnu_store_stats": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p), ctypes.POINTER(_c_void_p),
                                   ctypes.POINTER(_c_void_p)]),
     "prnu_store_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p]),
+    "prnu_fpstore_create": (_c_int, [_c_i64, _c_int, ctypes.POINTER(_c_void_p)]),
+    "prnu_fpstore_destroy": (_c_int, [_c_void_p]),
+    "prnu_fpstore_info": (_c_int, [_c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int),
+                                   ctypes.POINTER(_c_int), ctypes.POINTER(_c_size)]),
+    "prnu_fpstore_put": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p]),
+    "prnu_fpstore_query": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p,
+                                    _c_void_p, _c_void_p]),
     "prnu_gram_set_config": (_c_int, [_c_void_p, _c_int, _c_void_p]),
     "prnu_gram_get_config": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p]),
     "prnu_gram_autotune": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_void_p,

@@ -177,6 +177,29 @@ class PRNUFilter:
         _sync(sync)
         return out_bytes.value
 
+    def apply_gpu_batch(self, images, outputs, input_format: int = _lib.INPUT_RGB8, sync: bool = True) -> None:
+        """``apply_gpu`` of every row of ``images`` (B, ...) into the rows of
+        ``outputs`` (B, ...) with one launch per filter stage
+        (``prnu_preprocess_batch``); row strides are taken from the tensors."""
+        count = int(images.shape[0])
+        if int(outputs.shape[0]) < count:
+            raise ValueError(f"outputs hold {outputs.shape[0]} patterns, need {count}")
+        if count == 0:
+            return
+        for t in (images, outputs):
+            if not (hasattr(t, "is_cuda") and t.is_cuda):
+                raise ValueError("device buffer must be a CUDA tensor")
+            if not t[0].is_contiguous():
+                raise ValueError("every image / pattern row must be contiguous")
+        ip, op = images.data_ptr(), outputs.data_ptr()
+        in_stride = int(images.stride(0)) * images.element_size()
+        out_stride = int(outputs.stride(0)) * outputs.element_size()
+        ib = in_stride * (count - 1) + int(images[0].numel()) * images.element_size()
+        ob = out_stride * (int(outputs.shape[0]) - 1) + int(outputs[0].numel()) * outputs.element_size()
+        _lib.call("prnu_preprocess_batch", self._ctx.handle, input_format, ip, ib, in_stride, count, op, ob,
+                  out_stride, _stream())
+        _sync(sync)
+
     # getters, PRNUFilter.java:157-175
     def get_grayscale_filter(self):
         return self.grayscale_filter
@@ -263,6 +286,19 @@ class PeakToCorrelationEnergy(PatternComparator):
 
     def compare(self, left, right, result, sync: bool = True) -> None:
         self.apply_gpu(left, right, result, sync)
+
+    def pairs(self, spectra, pairs, out=None, sync: bool = True):
+        """PCE of every (i, j) in ``pairs`` (int array (P, 2)) over a list of
+        spectra (CUDA tensors) or a (M, ...) CUDA tensor, batched on the GPU."""
+        ptrs = pattern_pointer_table(spectra)
+        pr = torch.as_tensor(pairs, dtype=torch.int32).reshape(-1, 2).contiguous().cuda()
+        n = int(pr.shape[0])
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device="cuda")
+        _lib.call("prnu_pce_pairs", self._ctx.handle, self.variant, ptrs.data_ptr(), pr.data_ptr(), n,
+                  out.data_ptr(), _stream())
+        _sync(sync)
+        return out
 
     def cleanup(self) -> None:
         if self._own:

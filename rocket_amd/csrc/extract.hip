@@ -1,6 +1,8 @@
 // PRNU extraction on gfx950: grayscale + FastNoise (one LDS halo-tile
 // kernel), zero-mean total, frequency-domain Wiener filter and the R2C
-// spectrum of the PCE strategy.
+// spectrum of the PCE strategy.  Every kernel takes a batch of images on the
+// last grid axis (per-image strides), so a batch of B images is one launch
+// per stage.
 //
 // Reference semantics (paths under CSI/kernels/filter/):
 //   grayscalefilter.cu:47-66      out = 0.299*c.z + 0.587*c.y + 0.114*c.x
@@ -18,6 +20,8 @@
 #include "prnu_internal.hpp"
 #include "fft_device.hpp"
 #include "fft_host.hpp"
+
+#include <algorithm>
 
 namespace prnu {
 namespace {
@@ -41,21 +45,21 @@ __device__ inline double block_sum_256(double v, double *sh) {
 
 // ---------------------------------------------------------------- grayscale
 template <int FMT>
-__device__ __forceinline__ float gray_at(const void *in, int64_t idx) {
+__device__ __forceinline__ float gray_at(const uint8_t *in, int64_t idx) {
     if (FMT == PRNU_INPUT_RGB8) {
-        const uint8_t *p = static_cast<const uint8_t *>(in) + 3 * idx;
+        const uint8_t *p = in + 3 * idx;
         // byte 2 (c.z) carries 0.299, byte 0 (c.x) 0.114
         return kWR * (float)p[2] + kWG * (float)p[1] + kWB * (float)p[0];
     } else if (FMT == PRNU_INPUT_GRAY8) {
-        const float g = (float)static_cast<const uint8_t *>(in)[idx];
+        const float g = (float)in[idx];
         return kWR * g + kWG * g + kWB * g;
     } else {
-        return static_cast<const float *>(in)[idx];
+        return reinterpret_cast<const float *>(in)[idx];
     }
 }
 
 template <int FMT>
-__global__ __launch_bounds__(256) void gray_kernel(const void *in, float *out, int64_t n) {
+__global__ __launch_bounds__(256) void gray_kernel(const uint8_t *in, float *out, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         out[i] = gray_at<FMT>(in, i);
 }
@@ -70,17 +74,50 @@ __device__ __forceinline__ float grad1(float left, float center, float right, in
     return 0.5f * (right - left);
 }
 
+// Partial sums of the zero-mean pass for one 64 x 16 output tile held in O:
+// per column the sums of even and odd rows, per row the sums of even and odd
+// columns (FP64).
+__device__ __forceinline__ void zm_tile_sums(const float (*O)[kFnTX + 1], int x0, int y0, int h, int w,
+                                             double *colpart, double *rowpart) {
+    const int tid = threadIdx.x;
+    if (tid < kFnTX) {
+        const int gx = x0 + tid;
+        double se = 0.0, so = 0.0;
+        for (int ly = 0; ly < kFnTY; ++ly) {
+            if (((y0 + ly) & 1) == 0) se += O[ly][tid];
+            else so += O[ly][tid];
+        }
+        if (gx < w) {
+            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 0] = se;
+            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 1] = so;
+        }
+    } else if (tid >= 64 && tid < 64 + kFnTY) {
+        const int ly = tid - 64, gy = y0 + ly;
+        double se = 0.0, so = 0.0;
+        for (int lx = 0; lx < kFnTX; ++lx) {
+            if (((x0 + lx) & 1) == 0) se += O[ly][lx];
+            else so += O[ly][lx];
+        }
+        if (gy < h) {
+            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 0] = se;
+            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 1] = so;
+        }
+    }
+}
+
 // Gray conversion + FastNoise for a 64 x 16 tile with a 2-pixel halo, and
-// optionally the zero-mean partial sums of the output tile: per column the
-// sums of even and odd rows (colpart), per row the sums of even and odd
-// columns (rowpart), FP64.
+// optionally the zero-mean partial sums of the output tile.
+// grid (tiles_x, tiles_y, batch).
 template <int FMT>
-__global__ __launch_bounds__(256) void fastnoise_kernel(const void *in, float *out, int h, int w,
-                                                        double *colpart, double *rowpart) {
+__global__ __launch_bounds__(256) void fastnoise_kernel(const uint8_t *in_all, int64_t in_stride, float *out_all,
+                                                        int64_t out_stride, int h, int w, double *colpart_all,
+                                                        int64_t cp_stride, double *rowpart_all, int64_t rp_stride) {
     __shared__ float G[kFnTY + 4][kFnTX + 4];
     __shared__ float U[kFnTY + 2][kFnTX + 2];
     __shared__ float V[kFnTY + 2][kFnTX + 2];
     __shared__ float O[kFnTY][kFnTX + 1];
+    const uint8_t *in = in_all + (int64_t)blockIdx.z * in_stride;
+    float *out = out_all + (int64_t)blockIdx.z * out_stride;
     const int x0 = blockIdx.x * kFnTX, y0 = blockIdx.y * kFnTY;
     const int tid = threadIdx.x;
 
@@ -122,74 +159,41 @@ __global__ __launch_bounds__(256) void fastnoise_kernel(const void *in, float *o
         }
         O[ly][lx] = o;
     }
-    if (!colpart) return;
+    if (!colpart_all) return;
     __syncthreads();
-    if (tid < kFnTX) {
-        const int gx = x0 + tid;
-        double se = 0.0, so = 0.0;
-        for (int ly = 0; ly < kFnTY; ++ly) {
-            if (((y0 + ly) & 1) == 0) se += O[ly][tid];
-            else so += O[ly][tid];
-        }
-        if (gx < w) {
-            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 0] = se;
-            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 1] = so;
-        }
-    } else if (tid >= 64 && tid < 64 + kFnTY) {
-        const int ly = tid - 64, gy = y0 + ly;
-        double se = 0.0, so = 0.0;
-        for (int lx = 0; lx < kFnTX; ++lx) {
-            if (((x0 + lx) & 1) == 0) se += O[ly][lx];
-            else so += O[ly][lx];
-        }
-        if (gy < h) {
-            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 0] = se;
-            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 1] = so;
-        }
-    }
+    zm_tile_sums(O, x0, y0, h, w, colpart_all + (int64_t)blockIdx.z * cp_stride,
+                 rowpart_all + (int64_t)blockIdx.z * rp_stride);
 }
 
 // -------------------------------------------------------------- zero-mean
-// Partial sums of an arbitrary image in the fastnoise tiling.
-__global__ __launch_bounds__(256) void zm_partials_kernel(const float *img, int h, int w, double *colpart,
-                                                          double *rowpart) {
+// Partial sums of an arbitrary image in the fastnoise tiling.  grid (tx, ty, B)
+__global__ __launch_bounds__(256) void zm_partials_kernel(const float *img_all, int64_t img_stride, int h, int w,
+                                                          double *colpart_all, int64_t cp_stride,
+                                                          double *rowpart_all, int64_t rp_stride) {
     __shared__ float O[kFnTY][kFnTX + 1];
+    const float *img = img_all + (int64_t)blockIdx.z * img_stride;
     const int x0 = blockIdx.x * kFnTX, y0 = blockIdx.y * kFnTY;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < kFnTY * kFnTX; idx += 256) {
+    for (int idx = threadIdx.x; idx < kFnTY * kFnTX; idx += 256) {
         const int ly = idx / kFnTX, lx = idx % kFnTX;
         const int gy = y0 + ly, gx = x0 + lx;
         O[ly][lx] = (gy < h && gx < w) ? img[(int64_t)gy * w + gx] : 0.f;
     }
     __syncthreads();
-    if (tid < kFnTX) {
-        const int gx = x0 + tid;
-        double se = 0.0, so = 0.0;
-        for (int ly = 0; ly < kFnTY; ++ly) {
-            if (((y0 + ly) & 1) == 0) se += O[ly][tid];
-            else so += O[ly][tid];
-        }
-        if (gx < w) {
-            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 0] = se;
-            colpart[((int64_t)blockIdx.y * w + gx) * 2 + 1] = so;
-        }
-    } else if (tid >= 64 && tid < 64 + kFnTY) {
-        const int ly = tid - 64, gy = y0 + ly;
-        double se = 0.0, so = 0.0;
-        for (int lx = 0; lx < kFnTX; ++lx) {
-            if (((x0 + lx) & 1) == 0) se += O[ly][lx];
-            else so += O[ly][lx];
-        }
-        if (gy < h) {
-            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 0] = se;
-            rowpart[((int64_t)blockIdx.x * h + gy) * 2 + 1] = so;
-        }
-    }
+    zm_tile_sums(O, x0, y0, h, w, colpart_all + (int64_t)blockIdx.z * cp_stride,
+                 rowpart_all + (int64_t)blockIdx.z * rp_stride);
 }
 
 // colsum[p][x] = sum over tile rows; rowmean[p][y] = row sums / column counts.
-__global__ __launch_bounds__(256) void zm_reduce_kernel(const double *colpart, const double *rowpart, int h, int w,
-                                                        int tiles_y, int tiles_x, double *colsum, double *rowmean) {
+// grid (ceil(max(h,w)/256), B)
+__global__ __launch_bounds__(256) void zm_reduce_kernel(const double *colpart_all, int64_t cp_stride,
+                                                        const double *rowpart_all, int64_t rp_stride, int h, int w,
+                                                        int tiles_y, int tiles_x, double *colsum_all,
+                                                        double *rowmean_all) {
+    const int b = blockIdx.y;
+    const double *colpart = colpart_all + (int64_t)b * cp_stride;
+    const double *rowpart = rowpart_all + (int64_t)b * rp_stride;
+    double *colsum = colsum_all + (int64_t)b * 2 * w;
+    double *rowmean = rowmean_all + (int64_t)b * 2 * h;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < w) {
         double se = 0.0, so = 0.0;
@@ -211,10 +215,12 @@ __global__ __launch_bounds__(256) void zm_reduce_kernel(const double *colpart, c
     }
 }
 
-// g[pi*2+pj] = mean of the (row parity pi, column parity pj) sub-lattice.
-// One block of 256 threads; also turns colsum into column means in place.
-__global__ __launch_bounds__(256) void zm_lattice_kernel(double *colsum, int h, int w, double *g) {
+// g[pi*2+pj] = mean of the (row parity pi, column parity pj) sub-lattice; also
+// turns colsum into column means in place.  grid (B), block 256.
+__global__ __launch_bounds__(256) void zm_lattice_kernel(double *colsum_all, int h, int w, double *g_all) {
     __shared__ double sh[4];
+    double *colsum = colsum_all + (int64_t)blockIdx.x * 2 * w;
+    double *g = g_all + (int64_t)blockIdx.x * 4;
     double s[4] = {0, 0, 0, 0};
     for (int x = threadIdx.x; x < w; x += 256) {
         s[0 + (x & 1)] += colsum[x];
@@ -245,8 +251,15 @@ __device__ __forceinline__ float zm_value(float x, int y, int xx, int h, int w, 
     return (float)((double)x - colmean[pi * w + xx] - rowmean[pj * h + y] + g[pi * 2 + pj]);
 }
 
-__global__ __launch_bounds__(256) void zm_apply_kernel(float *img, int h, int w, const double *colmean,
-                                                       const double *rowmean, const double *g) {
+// grid (blocks, B)
+__global__ __launch_bounds__(256) void zm_apply_kernel(float *img_all, int64_t img_stride, int h, int w,
+                                                       const double *colmean_all, const double *rowmean_all,
+                                                       const double *g_all) {
+    const int b = blockIdx.y;
+    float *img = img_all + (int64_t)b * img_stride;
+    const double *colmean = colmean_all + (int64_t)b * 2 * w;
+    const double *rowmean = rowmean_all + (int64_t)b * 2 * h;
+    const double *g = g_all + (int64_t)b * 4;
     const int64_t n = (int64_t)h * w;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int y = (int)(i / w), x = (int)(i - (int64_t)y * w);
@@ -255,60 +268,75 @@ __global__ __launch_bounds__(256) void zm_apply_kernel(float *img, int h, int w,
 }
 
 // ------------------------------------------------------------------ Wiener
-// Row i: (optional zero-mean) -> sum of squares (FP64) -> forward FFT (w).
-__global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img, int h,
-                                                                      float2 *A, double *rowsq,
-                                                                      const double *colmean, const double *rowmean,
-                                                                      const double *g) {
+// Row i of image b: (optional zero-mean) -> sum of squares (FP64) -> forward
+// FFT (w).  grid (h, B).
+__global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img_all,
+                                                                      int64_t img_stride, int h, float2 *A_all,
+                                                                      double *rowsq_all, const double *colmean_all,
+                                                                      const double *rowmean_all, const double *g_all) {
     extern __shared__ float2 buf[];
     __shared__ double sh[4];
-    const int i = blockIdx.x;
+    const int i = blockIdx.x, b = blockIdx.y;
     const int w = d.n;
+    const int64_t n = (int64_t)h * w;
+    const float *img = img_all + (int64_t)b * img_stride;
+    const bool zm = colmean_all != nullptr;
+    const double *colmean = zm ? colmean_all + (int64_t)b * 2 * w : nullptr;
+    const double *rowmean = zm ? rowmean_all + (int64_t)b * 2 * h : nullptr;
+    const double *g = zm ? g_all + (int64_t)b * 4 : nullptr;
     double sq = 0.0;
     for (int j = threadIdx.x; j < w; j += kFftThreads) {
         float v = img[(int64_t)i * w + j];
-        if (colmean) v = zm_value(v, i, j, h, w, colmean, rowmean, g);
+        if (zm) v = zm_value(v, i, j, h, w, colmean, rowmean, g);
         sq += (double)v * (double)v;
         buf[j] = make_float2(v, 0.f);
     }
     sq = block_sum_256(sq, sh);
-    if (threadIdx.x == 0) rowsq[i] = sq;
+    if (threadIdx.x == 0) rowsq_all[(int64_t)b * h + i] = sq;
     __syncthreads();
     fft_lds<-1>(buf, d);
+    float2 *A = A_all + (int64_t)b * n;
     for (int j = threadIdx.x; j < w; j += kFftThreads) A[(int64_t)i * w + j] = buf[j];
 }
 
 // In-place forward FFT of the rows of B (length d.n) and |F|^2 into P.
-__global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, float2 *B, float *P) {
+// grid (rows, batch); planes `plane` elements apart.
+__global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, float2 *B, float *P, int64_t plane) {
     extern __shared__ float2 buf[];
     const int r = blockIdx.x;
     const int n = d.n;
-    float2 *row = B + (int64_t)r * n;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)r * n;
     for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
     __syncthreads();
     fft_lds<-1>(buf, d);
     for (int j = threadIdx.x; j < n; j += kFftThreads) {
         const float2 f = buf[j];
         row[j] = f;
-        P[(int64_t)r * n + j] = f.x * f.x + f.y * f.y;
+        prow[j] = f.x * f.x + f.y * f.y;
     }
 }
 
-// var = n/(n-1) * sum(x^2) over the row partials, FP64, fixed order.
-__global__ __launch_bounds__(256) void wiener_var_kernel(const double *rowsq, int h, int64_t n, double *var) {
+// var = n/(n-1) * sum(x^2) over the row partials, FP64, fixed order.  grid (B)
+__global__ __launch_bounds__(256) void wiener_var_kernel(const double *rowsq_all, int h, int64_t n, double *var_all) {
     __shared__ double sh[4];
+    const double *rowsq = rowsq_all + (int64_t)blockIdx.x * h;
     double s = 0.0;
     for (int i = threadIdx.x; i < h; i += 256) s += rowsq[i];
     s = block_sum_256(s, sh);
-    if (threadIdx.x == 0) var[0] = s * (double)n / (double)(n - 1);
+    if (threadIdx.x == 0) var_all[blockIdx.x] = s * (double)n / (double)(n - 1);
 }
 
 // Variance estimate: min over f in {3,5,7,9} of the zero-padded f x f mean of
 // P (wienerfilter.cu:164-247), then S = var / max(var, varest).
+// grid (ceil(cols/32), ceil(rows/32), B)
 constexpr int kVT = 32, kVB = 4;
-__global__ __launch_bounds__(256) void varest_scale_kernel(const float *P, float *S, int rows, int cols,
-                                                           const double *var) {
+__global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, float *S_all, int rows, int cols,
+                                                           const double *var_all) {
     __shared__ float T[kVT + 2 * kVB][kVT + 2 * kVB + 1];
+    const int64_t plane = (int64_t)rows * cols;
+    const float *P = P_all + (int64_t)blockIdx.z * plane;
+    float *S = S_all + (int64_t)blockIdx.z * plane;
     const int c0 = blockIdx.x * kVT, r0 = blockIdx.y * kVT;
     for (int idx = threadIdx.x; idx < (kVT + 2 * kVB) * (kVT + 2 * kVB); idx += 256) {
         const int ly = idx / (kVT + 2 * kVB), lx = idx % (kVT + 2 * kVB);
@@ -316,7 +344,7 @@ __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P, float
         T[ly][lx] = (gy >= 0 && gy < rows && gx >= 0 && gx < cols) ? P[(int64_t)gy * cols + gx] : 0.f;
     }
     __syncthreads();
-    const float v = (float)var[0];
+    const float v = (float)var_all[blockIdx.z];
     for (int idx = threadIdx.x; idx < kVT * kVT; idx += 256) {
         const int ly = idx / kVT, lx = idx % kVT;
         const int gy = r0 + ly, gx = c0 + lx;
@@ -343,46 +371,59 @@ __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P, float
 }
 
 // Scale row r of B by S and run the inverse FFT (length d.n) in place.
-__global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d, float2 *B, const float *S) {
+// grid (rows, B); planes `plane` apart.
+__global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d, float2 *B, const float *S,
+                                                                       int64_t plane) {
     extern __shared__ float2 buf[];
     const int r = blockIdx.x;
     const int n = d.n;
-    float2 *row = B + (int64_t)r * n;
-    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = cscale(row[j], S[(int64_t)r * n + j]);
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = cscale(row[j], srow[j]);
     __syncthreads();
     fft_lds<1>(buf, d);
     for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
 }
 
-// Inverse FFT (length w) of row i of A; out = Re / (w*h)  (normalizeToReal).
-__global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, const float2 *A, float *out,
-                                                                     float inv_n) {
+// Inverse FFT (length w) of row i of A; out = Re / (w*h) (normalizeToReal).
+// grid (h, B)
+__global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, const float2 *A, int64_t plane,
+                                                                     float *out_all, int64_t out_stride, float inv_n) {
     extern __shared__ float2 buf[];
     const int i = blockIdx.x;
     const int w = d.n;
-    for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = A[(int64_t)i * w + j];
+    const float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * w;
+    float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i * w;
+    for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = arow[j];
     __syncthreads();
     fft_lds<1>(buf, d);
-    for (int j = threadIdx.x; j < w; j += kFftThreads) out[(int64_t)i * w + j] = buf[j].x * inv_n;
+    for (int j = threadIdx.x; j < w; j += kFftThreads) out[j] = buf[j].x * inv_n;
 }
 
 // ---------------------------------------------------------------- spectrum
 // Forward FFT (length w) of real row i; the first w/2+1 bins go to A (h x wh).
-__global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img, float2 *A) {
+// grid (h, B)
+__global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img_all,
+                                                                    int64_t img_stride, float2 *A, int64_t plane,
+                                                                    int h) {
     extern __shared__ float2 buf[];
     const int i = blockIdx.x;
     const int w = d.n, wh = w / 2 + 1;
+    const float *img = img_all + (int64_t)blockIdx.y * img_stride;
     for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = make_float2(img[(int64_t)i * w + j], 0.f);
     __syncthreads();
     fft_lds<-1>(buf, d);
-    for (int j = threadIdx.x; j < wh; j += kFftThreads) A[(int64_t)i * wh + j] = buf[j];
+    float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * wh;
+    for (int j = threadIdx.x; j < wh; j += kFftThreads) arow[j] = buf[j];
+    (void)h;
 }
 
+// grid (rows, B)
 template <int DIR>
-__global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d, float2 *B) {
+__global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d, float2 *B, int64_t plane) {
     extern __shared__ float2 buf[];
     const int n = d.n;
-    float2 *row = B + (int64_t)blockIdx.x * n;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * n;
     for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
     __syncthreads();
     fft_lds<DIR>(buf, d);
@@ -390,25 +431,40 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d
 }
 
 // ------------------------------------------------------------- workspace
+// Per-image buffers, each an array over the batch with the given stride.
 struct ExtractWs {
+    int nb;                 // images the workspace holds
+    int64_t n;              // plane stride (elements) of A, B, P, S
     float2 *A, *B;
     float *P, *S;
     double *rowsq, *var, *colpart, *rowpart, *colmean, *rowmean, *g;
+    int64_t cp_stride, rp_stride;
     int tiles_x, tiles_y;
 };
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-int get_ws(Context *c, ExtractWs &e) {
+size_t per_image_bytes(int h, int w) {
+    const int64_t n = (int64_t)h * w;
+    const int tx = (w + kFnTX - 1) / kFnTX, ty = (h + kFnTY - 1) / kFnTY;
+    return 24 * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) + 11 * 256;
+}
+
+int get_ws(Context *c, int nb, ExtractWs &e) {
     const int h = c->h, w = c->w;
     const int64_t n = c->pixels();
+    e.nb = nb;
+    e.n = n;
     e.tiles_x = (w + kFnTX - 1) / kFnTX;
     e.tiles_y = (h + kFnTY - 1) / kFnTY;
-    const size_t sz[] = {align256(sizeof(float2) * n), align256(sizeof(float2) * n), align256(sizeof(float) * n),
-                         align256(sizeof(float) * n), align256(sizeof(double) * h), align256(sizeof(double) * 2),
-                         align256(sizeof(double) * 2 * w * (size_t)e.tiles_y),
-                         align256(sizeof(double) * 2 * h * (size_t)e.tiles_x), align256(sizeof(double) * 2 * w),
-                         align256(sizeof(double) * 2 * h), align256(sizeof(double) * 4)};
+    e.cp_stride = 2 * (int64_t)w * e.tiles_y;
+    e.rp_stride = 2 * (int64_t)h * e.tiles_x;
+    const size_t sz[] = {align256(sizeof(float2) * n * nb),   align256(sizeof(float2) * n * nb),
+                         align256(sizeof(float) * n * nb),    align256(sizeof(float) * n * nb),
+                         align256(sizeof(double) * h * nb),   align256(sizeof(double) * nb),
+                         align256(sizeof(double) * e.cp_stride * nb), align256(sizeof(double) * e.rp_stride * nb),
+                         align256(sizeof(double) * 2 * w * nb), align256(sizeof(double) * 2 * h * nb),
+                         align256(sizeof(double) * 4 * nb)};
     size_t total = 0;
     for (size_t s : sz) total += s;
     PRNU_TRY(c->ws_filter.ensure(total));
@@ -426,69 +482,87 @@ int get_ws(Context *c, ExtractWs &e) {
 size_t fft_smem(const FftDesc &d) { return sizeof(float2) * d.n; }
 
 template <int FMT>
-int launch_fastnoise_fmt(const void *in, float *out, int h, int w, double *colpart, double *rowpart,
-                         hipStream_t st) {
-    dim3 grid((w + kFnTX - 1) / kFnTX, (h + kFnTY - 1) / kFnTY);
-    hipLaunchKernelGGL(fastnoise_kernel<FMT>, grid, dim3(256), 0, st, in, out, h, w, colpart, rowpart);
+int launch_fastnoise_fmt(const void *in, int64_t in_stride, float *out, int64_t out_stride, int h, int w, int nb,
+                         double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, hipStream_t st) {
+    dim3 grid((w + kFnTX - 1) / kFnTX, (h + kFnTY - 1) / kFnTY, nb);
+    hipLaunchKernelGGL(fastnoise_kernel<FMT>, grid, dim3(256), 0, st, static_cast<const uint8_t *>(in), in_stride,
+                       out, out_stride, h, w, colpart, cp_stride, rowpart, rp_stride);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
 
-int launch_fastnoise(int fmt, const void *in, float *out, int h, int w, double *colpart, double *rowpart,
-                     hipStream_t st) {
+int launch_fastnoise(int fmt, const void *in, int64_t in_stride, float *out, int64_t out_stride, int h, int w, int nb,
+                     double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, hipStream_t st) {
     switch (fmt) {
-        case PRNU_INPUT_RGB8: return launch_fastnoise_fmt<PRNU_INPUT_RGB8>(in, out, h, w, colpart, rowpart, st);
-        case PRNU_INPUT_GRAY8: return launch_fastnoise_fmt<PRNU_INPUT_GRAY8>(in, out, h, w, colpart, rowpart, st);
-        default: return launch_fastnoise_fmt<PRNU_INPUT_FLOAT32>(in, out, h, w, colpart, rowpart, st);
+        case PRNU_INPUT_RGB8:
+            return launch_fastnoise_fmt<PRNU_INPUT_RGB8>(in, in_stride, out, out_stride, h, w, nb, colpart, cp_stride,
+                                                         rowpart, rp_stride, st);
+        case PRNU_INPUT_GRAY8:
+            return launch_fastnoise_fmt<PRNU_INPUT_GRAY8>(in, in_stride, out, out_stride, h, w, nb, colpart,
+                                                          cp_stride, rowpart, rp_stride, st);
+        default:
+            return launch_fastnoise_fmt<PRNU_INPUT_FLOAT32>(in, in_stride, out, out_stride, h, w, nb, colpart,
+                                                            cp_stride, rowpart, rp_stride, st);
     }
 }
 
-int launch_zm_means(Context *c, ExtractWs &e, hipStream_t st) {
+int launch_zm_means(Context *c, ExtractWs &e, int nb, hipStream_t st) {
     const int h = c->h, w = c->w;
     const int m = h > w ? h : w;
-    hipLaunchKernelGGL(zm_reduce_kernel, dim3((m + 255) / 256), dim3(256), 0, st, e.colpart, e.rowpart, h, w,
-                       e.tiles_y, e.tiles_x, e.colmean, e.rowmean);
+    hipLaunchKernelGGL(zm_reduce_kernel, dim3((m + 255) / 256, nb), dim3(256), 0, st, e.colpart, e.cp_stride,
+                       e.rowpart, e.rp_stride, h, w, e.tiles_y, e.tiles_x, e.colmean, e.rowmean);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(zm_lattice_kernel, dim3(1), dim3(256), 0, st, e.colmean, h, w, e.g);
+    hipLaunchKernelGGL(zm_lattice_kernel, dim3(nb), dim3(256), 0, st, e.colmean, h, w, e.g);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
 
-// Wiener filter of img (h x w) into out; with zero-mean means when zm.
-int launch_wiener(Context *c, ExtractWs &e, const float *img, float *out, bool zm, hipStream_t st) {
+// Wiener filter of nb images (img, img_stride floats apart) into out.
+int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float *out,
+                  int64_t out_stride, bool zm, hipStream_t st) {
     const int h = c->h, w = c->w;
+    const int64_t n = c->pixels();
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    hipLaunchKernelGGL(wiener_rows_fwd_kernel, dim3(h), dim3(kFftThreads), fft_smem(dw), st, dw, img, h, e.A,
-                       e.rowsq, zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
+    hipLaunchKernelGGL(wiener_rows_fwd_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img, img_stride,
+                       h, e.A, e.rowsq, zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(e.A, e.B, h, w, 1, 0, st));
-    hipLaunchKernelGGL(rows_fft_power_kernel, dim3(w), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.P);
+    PRNU_TRY(launch_transpose_c(e.A, e.B, h, w, nb, n, st));
+    hipLaunchKernelGGL(rows_fft_power_kernel, dim3(w, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.P, n);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(wiener_var_kernel, dim3(1), dim3(256), 0, st, e.rowsq, h, c->pixels(), e.var);
+    hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(varest_scale_kernel, dim3((h + kVT - 1) / kVT, (w + kVT - 1) / kVT), dim3(256), 0, st, e.P,
+    hipLaunchKernelGGL(varest_scale_kernel, dim3((h + kVT - 1) / kVT, (w + kVT - 1) / kVT, nb), dim3(256), 0, st, e.P,
                        e.S, w, h, e.var);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(rows_ifft_scaled_kernel, dim3(w), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.S);
+    hipLaunchKernelGGL(rows_ifft_scaled_kernel, dim3(w, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.S, n);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(e.B, e.A, w, h, 1, 0, st));
-    hipLaunchKernelGGL(rows_ifft_real_kernel, dim3(h), dim3(kFftThreads), fft_smem(dw), st, dw, e.A, out,
-                       1.0f / (float)(w * h));
+    PRNU_TRY(launch_transpose_c(e.B, e.A, w, h, nb, n, st));
+    hipLaunchKernelGGL(rows_ifft_real_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, e.A, n, out,
+                       out_stride, 1.0f / (float)(w * h));
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
 
-int launch_spectrum(Context *c, ExtractWs &e, const float *img, float2 *out, hipStream_t st) {
+// R2C spectrum of nb images into out (h x (w/2+1) complex each, out_stride float2 apart).
+int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float2 *out,
+                    int64_t out_stride, hipStream_t st) {
     const int h = c->h, w = c->w, wh = w / 2 + 1;
+    const int64_t n = c->pixels();
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h), dim3(kFftThreads), fft_smem(dw), st, dw, img, e.A);
+    hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img, img_stride,
+                       e.A, n, h);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, 1, 0, st));
-    hipLaunchKernelGGL(rows_fft_inplace_kernel<-1>, dim3(wh), dim3(kFftThreads), fft_smem(dh), st, dh, e.B);
+    PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
+    hipLaunchKernelGGL(rows_fft_inplace_kernel<-1>, dim3(wh, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, n);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(e.B, out, wh, h, 1, 0, st));
+    if (nb == 1 || out_stride == n) {
+        PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st));
+    } else {
+        for (int b = 0; b < nb; ++b)
+            PRNU_TRY(launch_transpose_c(e.B + (int64_t)b * n, out + (int64_t)b * out_stride, wh, h, 1, 0, st));
+    }
     return PRNU_OK;
 }
 
@@ -498,34 +572,63 @@ int input_bytes_per_pixel(int fmt) {
 
 }  // namespace
 
-// Whole PRNUFilter chain (PRNUFilter.java:124-138) into pattern (NCC mode,
-// h*w FP32) or spectrum (PCE mode).
-int run_preprocess(Context *c, int fmt, const void *in, size_t in_bytes, void *out, size_t out_cap,
-                   size_t *out_bytes, hipStream_t st) {
+// Whole PRNUFilter chain (PRNUFilter.java:124-138) for `count` images
+// in_stride bytes apart into patterns out_stride bytes apart: NCC mode writes
+// h*w FP32, PCE mode the h*(w/2+1) complex spectrum.
+int run_preprocess_batch(Context *c, int fmt, const void *in, size_t in_bytes, int64_t in_stride, int count,
+                         void *out, size_t out_cap, int64_t out_stride, size_t *out_bytes, hipStream_t st) {
     const int bpp = input_bytes_per_pixel(fmt);
     PRNU_REQUIRE(bpp > 0, "preprocess: unknown input format %d", fmt);
-    PRNU_REQUIRE(in_bytes >= (size_t)bpp * c->pixels(), "preprocess: input has %zu bytes, needs %lld", in_bytes,
-                 (long long)(bpp * c->pixels()));
+    PRNU_REQUIRE(count >= 1, "preprocess: count must be >= 1");
+    const size_t img_bytes = (size_t)bpp * c->pixels();
     const size_t need = c->mode == PRNU_MODE_PCE ? 8 * (size_t)c->spectrum_elems() : 4 * (size_t)c->pixels();
-    PRNU_REQUIRE(out_cap >= need, "preprocess: output has %zu bytes, needs %zu", out_cap, need);
-    PRNU_TRY(ensure_fft_plans(c));
-    ExtractWs e;
-    PRNU_TRY(get_ws(c, e));
-    // gray + FastNoise into S (as an FP32 image), zero-mean sums on the side
-    float *noise = e.S;
-    PRNU_TRY(launch_fastnoise(fmt, in, noise, c->h, c->w, e.colpart, e.rowpart, st));
-    PRNU_TRY(launch_zm_means(c, e, st));
-    if (c->mode == PRNU_MODE_NCC) {
-        time_begin(c, st);
-        PRNU_TRY(launch_wiener(c, e, noise, static_cast<float *>(out), true, st));
-        time_end(c, st);
+    if (count > 1) {
+        PRNU_REQUIRE(in_stride >= (int64_t)img_bytes && out_stride >= (int64_t)need,
+                     "preprocess: strides (%lld, %lld) smaller than an image (%zu) or a pattern (%zu)",
+                     (long long)in_stride, (long long)out_stride, img_bytes, need);
+        PRNU_REQUIRE(in_stride % bpp == 0 && out_stride % (c->mode == PRNU_MODE_PCE ? 8 : 4) == 0,
+                     "preprocess: strides must be whole pixels");
     } else {
-        // Wiener output into P (FP32 h*w), then its spectrum
-        PRNU_TRY(launch_wiener(c, e, noise, e.P, true, st));
-        PRNU_TRY(launch_spectrum(c, e, e.P, static_cast<float2 *>(out), st));
+        in_stride = (int64_t)img_bytes;
+        out_stride = (int64_t)need;
     }
+    PRNU_REQUIRE(in_bytes >= (size_t)in_stride * (count - 1) + img_bytes,
+                 "preprocess: input has %zu bytes, needs %zu", in_bytes, (size_t)in_stride * (count - 1) + img_bytes);
+    PRNU_REQUIRE(out_cap >= (size_t)out_stride * (count - 1) + need, "preprocess: output has %zu bytes, needs %zu",
+                 out_cap, (size_t)out_stride * (count - 1) + need);
+    PRNU_TRY(ensure_fft_plans(c));
+    // images per pass: the whole batch if its workspace stays under 4 GiB
+    const size_t per = per_image_bytes(c->h, c->w);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)count, (size_t(4) << 30) / per));
+    ExtractWs e;
+    PRNU_TRY(get_ws(c, chunk, e));
+    const int64_t n = c->pixels();
+    time_begin(c, st);
+    for (int b0 = 0; b0 < count; b0 += chunk) {
+        const int nb = std::min(chunk, count - b0);
+        const uint8_t *src = static_cast<const uint8_t *>(in) + (size_t)b0 * in_stride;
+        uint8_t *dst = static_cast<uint8_t *>(out) + (size_t)b0 * out_stride;
+        // gray + FastNoise into S (FP32 images n apart), zero-mean sums on the side
+        float *noise = e.S;
+        PRNU_TRY(launch_fastnoise(fmt, src, in_stride, noise, n, c->h, c->w, nb,
+                                  e.colpart, e.cp_stride, e.rowpart, e.rp_stride, st));
+        PRNU_TRY(launch_zm_means(c, e, nb, st));
+        if (c->mode == PRNU_MODE_NCC) {
+            PRNU_TRY(launch_wiener(c, e, nb, noise, n, reinterpret_cast<float *>(dst), out_stride / 4, true, st));
+        } else {
+            // Wiener output into P (FP32 images n apart), then its spectrum
+            PRNU_TRY(launch_wiener(c, e, nb, noise, n, e.P, n, true, st));
+            PRNU_TRY(launch_spectrum(c, e, nb, e.P, n, reinterpret_cast<float2 *>(dst), out_stride / 8, st));
+        }
+    }
+    time_end(c, st);
     if (out_bytes) *out_bytes = need;
     return PRNU_OK;
+}
+
+int run_preprocess(Context *c, int fmt, const void *in, size_t in_bytes, void *out, size_t out_cap,
+                   size_t *out_bytes, hipStream_t st) {
+    return run_preprocess_batch(c, fmt, in, in_bytes, 0, 1, out, out_cap, 0, out_bytes, st);
 }
 
 int run_filter_grayscale(Context *c, int fmt, const void *in, size_t in_bytes, float *out, hipStream_t st) {
@@ -536,12 +639,13 @@ int run_filter_grayscale(Context *c, int fmt, const void *in, size_t in_bytes, f
                  (long long)(bpp * c->pixels()));
     const int64_t n = c->pixels();
     const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    const uint8_t *src = static_cast<const uint8_t *>(in);
     if (fmt == PRNU_INPUT_RGB8)
-        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_RGB8>, dim3(blocks), dim3(256), 0, st, in, out, n);
+        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_RGB8>, dim3(blocks), dim3(256), 0, st, src, out, n);
     else if (fmt == PRNU_INPUT_GRAY8)
-        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_GRAY8>, dim3(blocks), dim3(256), 0, st, in, out, n);
+        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_GRAY8>, dim3(blocks), dim3(256), 0, st, src, out, n);
     else
-        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_FLOAT32>, dim3(blocks), dim3(256), 0, st, in, out, n);
+        hipLaunchKernelGGL(gray_kernel<PRNU_INPUT_FLOAT32>, dim3(blocks), dim3(256), 0, st, src, out, n);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -549,22 +653,24 @@ int run_filter_grayscale(Context *c, int fmt, const void *in, size_t in_bytes, f
 int run_filter_fastnoise(Context *c, float *img, hipStream_t st) {
     // the kernel reads a halo around each tile, so it cannot write in place
     ExtractWs e;
-    PRNU_TRY(get_ws(c, e));
-    PRNU_TRY(launch_fastnoise(PRNU_INPUT_FLOAT32, img, e.P, c->h, c->w, nullptr, nullptr, st));
+    PRNU_TRY(get_ws(c, 1, e));
+    PRNU_TRY(launch_fastnoise(PRNU_INPUT_FLOAT32, img, 4 * c->pixels(), e.P, c->pixels(), c->h, c->w, 1, nullptr, 0,
+                              nullptr, 0, st));
     PRNU_CHECK_HIP(hipMemcpyAsync(img, e.P, 4 * (size_t)c->pixels(), hipMemcpyDeviceToDevice, st));
     return PRNU_OK;
 }
 
 int run_filter_zeromean(Context *c, float *img, hipStream_t st) {
     ExtractWs e;
-    PRNU_TRY(get_ws(c, e));
-    dim3 grid(e.tiles_x, e.tiles_y);
-    hipLaunchKernelGGL(zm_partials_kernel, grid, dim3(256), 0, st, img, c->h, c->w, e.colpart, e.rowpart);
+    PRNU_TRY(get_ws(c, 1, e));
+    dim3 grid(e.tiles_x, e.tiles_y, 1);
+    hipLaunchKernelGGL(zm_partials_kernel, grid, dim3(256), 0, st, img, c->pixels(), c->h, c->w, e.colpart,
+                       e.cp_stride, e.rowpart, e.rp_stride);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_zm_means(c, e, st));
+    PRNU_TRY(launch_zm_means(c, e, 1, st));
     const int64_t n = c->pixels();
     const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
-    hipLaunchKernelGGL(zm_apply_kernel, dim3(blocks), dim3(256), 0, st, img, c->h, c->w, e.colmean, e.rowmean,
+    hipLaunchKernelGGL(zm_apply_kernel, dim3(blocks, 1), dim3(256), 0, st, img, n, c->h, c->w, e.colmean, e.rowmean,
                        e.g);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
@@ -573,8 +679,8 @@ int run_filter_zeromean(Context *c, float *img, hipStream_t st) {
 int run_filter_wiener(Context *c, float *img, hipStream_t st) {
     PRNU_TRY(ensure_fft_plans(c));
     ExtractWs e;
-    PRNU_TRY(get_ws(c, e));
-    return launch_wiener(c, e, img, img, false, st);
+    PRNU_TRY(get_ws(c, 1, e));
+    return launch_wiener(c, e, 1, img, c->pixels(), img, c->pixels(), false, st);
 }
 
 int run_filter_spectral(Context *c, const float *img, float *out, size_t out_cap, hipStream_t st) {
@@ -583,8 +689,8 @@ int run_filter_spectral(Context *c, const float *img, float *out, size_t out_cap
                  (long long)(8 * c->spectrum_elems()));
     PRNU_TRY(ensure_fft_plans(c));
     ExtractWs e;
-    PRNU_TRY(get_ws(c, e));
-    return launch_spectrum(c, e, img, reinterpret_cast<float2 *>(out), st);
+    PRNU_TRY(get_ws(c, 1, e));
+    return launch_spectrum(c, e, 1, img, c->pixels(), reinterpret_cast<float2 *>(out), c->spectrum_elems(), st);
 }
 
 }  // namespace prnu

@@ -19,6 +19,8 @@
 #include "fft_device.hpp"
 #include "fft_host.hpp"
 
+#include <algorithm>
+
 namespace prnu {
 namespace {
 
@@ -71,12 +73,34 @@ __device__ inline void block_argmax_256(float &v, int &idx, float *shv, int *shi
     for (int k = 1; k < 4; ++k) argmax_merge(v, idx, shv[k], shi[k]);
 }
 
+// Pair p of a batch: spectra from a device table and an (i, j) pair list,
+// or the two pointers of a single pair.
+struct PceBatch {
+    const float2 *const *tab;  // spectra table (null: use x1, y1)
+    const int *pairs;          // 2 ints per pair
+    int base;                  // first pair of this batch
+    const float2 *x1, *y1;
+};
+
+__device__ __forceinline__ void pce_operands(const PceBatch &b, int p, const float2 *&X, const float2 *&Y) {
+    if (b.tab) {
+        X = b.tab[b.pairs[2 * (b.base + p)]];
+        Y = b.tab[b.pairs[2 * (b.base + p) + 1]];
+    } else {
+        X = b.x1;
+        Y = b.y1;
+    }
+}
+
 template <int VARIANT>
-__global__ __launch_bounds__(kFftThreads) void pce_rows_kernel(FftDesc d, int h, const float2 *__restrict__ X,
-                                                               const float2 *__restrict__ Y, float2 *__restrict__ A) {
+__global__ __launch_bounds__(kFftThreads) void pce_rows_kernel(FftDesc d, int h, PceBatch bt, float2 *__restrict__ Aall,
+                                                               int64_t pstride) {
     extern __shared__ float2 buf[];
     const int i = blockIdx.x;
     const int w = d.n, wh = w / 2 + 1;
+    const float2 *X, *Y;
+    pce_operands(bt, blockIdx.y, X, Y);
+    float2 *A = Aall + (int64_t)blockIdx.y * pstride;
     // rows of the half spectra feeding the right half of C
     const int mi = (VARIANT == PRNU_PCE_REFERENCE) ? i : (h - i) % h;
     for (int j = threadIdx.x; j < w; j += kFftThreads) {
@@ -95,15 +119,18 @@ __global__ __launch_bounds__(kFftThreads) void pce_rows_kernel(FftDesc d, int h,
     for (int j = threadIdx.x; j < w; j += kFftThreads) A[(int64_t)i * w + j] = buf[j];
 }
 
-// Row j of B is column j of the plane; inverse FFT over h.
-__global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w, const float2 *__restrict__ B,
-                                                               float *__restrict__ R, PeakPart *__restrict__ part) {
+// Row j of B is column j of the plane; inverse FFT over h.  grid (w, batch).
+__global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w, const float2 *__restrict__ Ball,
+                                                               float *__restrict__ Rall, PeakPart *__restrict__ partall,
+                                                               int64_t pstride, int64_t rstride) {
     extern __shared__ float2 buf[];
     __shared__ double shd[4];
     __shared__ float shv[4];
     __shared__ int shi[4];
     const int j = blockIdx.x;
     const int h = d.n;
+    const float2 *B = Ball + (int64_t)blockIdx.y * pstride;
+    float *R = Rall + (int64_t)blockIdx.y * rstride;
     for (int i = threadIdx.x; i < h; i += kFftThreads) buf[i] = B[(int64_t)j * h + i];
     __syncthreads();
     fft_lds<1>(buf, d);
@@ -123,18 +150,21 @@ __global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w,
         p.energy = e;
         p.absmax = best;
         p.index = bi;
-        part[j] = p;
+        partall[(int64_t)blockIdx.y * w + j] = p;
     }
 }
 
-// R is the correlation plane transposed: c[i][j] = R[j*h + i].
+// R is the correlation plane transposed: c[i][j] = R[j*h + i].  grid (batch).
 template <int VARIANT>
-__global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restrict__ part, const float *__restrict__ R,
-                                                        int h, int w, double *__restrict__ out) {
+__global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restrict__ partall,
+                                                        const float *__restrict__ Rall, int h, int w,
+                                                        double *__restrict__ outall, int base, int64_t rstride) {
     __shared__ double shd[4];
     __shared__ float shv[4];
     __shared__ int shi[4];
     __shared__ int s_peak;
+    const PeakPart *part = partall + (int64_t)blockIdx.x * w;
+    const float *R = Rall + (int64_t)blockIdx.x * rstride;
     double e = 0.0;
     float best = -1.0f;
     int bi = 0x7fffffff;
@@ -184,8 +214,44 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
             const double pk = R[(int64_t)px * h + py];
             pce = (pk < 0 ? -1.0 : 1.0) * pk * pk / energy;
         }
-        out[0] = pce;
+        outall[base + blockIdx.x] = pce;
     }
+}
+
+int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, hipStream_t st) {
+    const int h = c->h, w = c->w;
+    const int64_t n = c->pixels();
+    const size_t plane = ((sizeof(float2) * n) + 255) & ~size_t(255);
+    const size_t rbytes = ((sizeof(float) * n) + 255) & ~size_t(255);
+    const size_t pbytes = ((sizeof(PeakPart) * w) + 255) & ~size_t(255);
+    const int64_t pstride = (int64_t)(plane / sizeof(float2)), rstride = (int64_t)(rbytes / sizeof(float));
+    PRNU_TRY(c->ws_pce.ensure((2 * plane + rbytes + pbytes) * nb));
+    char *base = static_cast<char *>(c->ws_pce.ptr);
+    float2 *A = reinterpret_cast<float2 *>(base);
+    float2 *B = reinterpret_cast<float2 *>(base + plane * nb);
+    float *R = reinterpret_cast<float *>(base + 2 * plane * nb);
+    PeakPart *part = reinterpret_cast<PeakPart *>(base + (2 * plane + rbytes) * nb);
+    const FftDesc &dw = fft_desc(c->fft_w);
+    const FftDesc &dh = fft_desc(c->fft_h);
+    if (variant == PRNU_PCE_REFERENCE)
+        hipLaunchKernelGGL(pce_rows_kernel<PRNU_PCE_REFERENCE>, dim3(h, nb), dim3(kFftThreads), sizeof(float2) * w,
+                           st, dw, h, bt, A, pstride);
+    else
+        hipLaunchKernelGGL(pce_rows_kernel<PRNU_PCE_TEXTBOOK>, dim3(h, nb), dim3(kFftThreads), sizeof(float2) * w,
+                           st, dw, h, bt, A, pstride);
+    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(launch_transpose_c(A, B, h, w, nb, pstride, st));
+    hipLaunchKernelGGL(pce_cols_kernel, dim3(w, nb), dim3(kFftThreads), sizeof(float2) * h, st, dh, w, B, R, part,
+                       pstride, rstride);
+    PRNU_CHECK_LAUNCH();
+    if (variant == PRNU_PCE_REFERENCE)
+        hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_REFERENCE>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,
+                           bt.base, rstride);
+    else
+        hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_TEXTBOOK>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,
+                           bt.base, rstride);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
 }
 
 }  // namespace
@@ -193,36 +259,39 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
 int run_pce(Context *c, int variant, const void *x, const void *y, double *out, hipStream_t st) {
     PRNU_REQUIRE(variant == PRNU_PCE_REFERENCE || variant == PRNU_PCE_TEXTBOOK, "pce: unknown variant %d", variant);
     PRNU_TRY(ensure_fft_plans(c));
-    const int h = c->h, w = c->w;
-    const int64_t n = c->pixels();
-    const size_t plane = ((sizeof(float2) * n) + 255) & ~size_t(255);
-    const size_t rbytes = ((sizeof(float) * n) + 255) & ~size_t(255);
-    PRNU_TRY(c->ws_pce.ensure(2 * plane + rbytes + sizeof(PeakPart) * w));
-    char *base = static_cast<char *>(c->ws_pce.ptr);
-    float2 *A = reinterpret_cast<float2 *>(base);
-    float2 *B = reinterpret_cast<float2 *>(base + plane);
-    float *R = reinterpret_cast<float *>(base + 2 * plane);
-    PeakPart *part = reinterpret_cast<PeakPart *>(base + 2 * plane + rbytes);
-    const FftDesc &dw = fft_desc(c->fft_w);
-    const FftDesc &dh = fft_desc(c->fft_h);
-    const float2 *X = static_cast<const float2 *>(x);
-    const float2 *Y = static_cast<const float2 *>(y);
+    PceBatch bt;
+    bt.tab = nullptr;
+    bt.pairs = nullptr;
+    bt.base = 0;
+    bt.x1 = static_cast<const float2 *>(x);
+    bt.y1 = static_cast<const float2 *>(y);
     time_begin(c, st);
-    if (variant == PRNU_PCE_REFERENCE)
-        hipLaunchKernelGGL(pce_rows_kernel<PRNU_PCE_REFERENCE>, dim3(h), dim3(kFftThreads), sizeof(float2) * w, st,
-                           dw, h, X, Y, A);
-    else
-        hipLaunchKernelGGL(pce_rows_kernel<PRNU_PCE_TEXTBOOK>, dim3(h), dim3(kFftThreads), sizeof(float2) * w, st,
-                           dw, h, X, Y, A);
-    PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(A, B, h, w, 1, 0, st));
-    hipLaunchKernelGGL(pce_cols_kernel, dim3(w), dim3(kFftThreads), sizeof(float2) * h, st, dh, w, B, R, part);
-    PRNU_CHECK_LAUNCH();
-    if (variant == PRNU_PCE_REFERENCE)
-        hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_REFERENCE>, dim3(1), dim3(256), 0, st, part, R, h, w, out);
-    else
-        hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_TEXTBOOK>, dim3(1), dim3(256), 0, st, part, R, h, w, out);
-    PRNU_CHECK_LAUNCH();
+    PRNU_TRY(pce_batch(c, variant, bt, 1, out, st));
+    time_end(c, st);
+    return PRNU_OK;
+}
+
+// PCE of `npairs` (i, j) pairs over a device table of spectra, in batches
+// whose planes fit `batch` pairs at a time.
+int run_pce_pairs(Context *c, int variant, const void *const *d_tab, const int *d_pairs, int npairs, double *out,
+                  int batch, hipStream_t st) {
+    PRNU_REQUIRE(variant == PRNU_PCE_REFERENCE || variant == PRNU_PCE_TEXTBOOK, "pce: unknown variant %d", variant);
+    PRNU_REQUIRE(npairs >= 0, "pce_pairs: negative pair count");
+    PRNU_TRY(ensure_fft_plans(c));
+    if (batch <= 0) {
+        // about 1 GiB of planes
+        const int64_t per = 20 * c->pixels();
+        batch = (int)std::max<int64_t>(1, std::min<int64_t>(256, (int64_t(1) << 30) / per));
+    }
+    time_begin(c, st);
+    for (int b0 = 0; b0 < npairs; b0 += batch) {
+        PceBatch bt;
+        bt.tab = reinterpret_cast<const float2 *const *>(d_tab);
+        bt.pairs = d_pairs;
+        bt.base = b0;
+        bt.x1 = bt.y1 = nullptr;
+        PRNU_TRY(pce_batch(c, variant, bt, std::min(batch, npairs - b0), out, st));
+    }
     time_end(c, st);
     return PRNU_OK;
 }

@@ -4,12 +4,16 @@
 namespace prnu {
 int run_preprocess(Context *c, int fmt, const void *in, size_t in_bytes, void *out, size_t out_cap,
                    size_t *out_bytes, hipStream_t st);
+int run_preprocess_batch(Context *c, int fmt, const void *in, size_t in_bytes, int64_t in_stride, int count,
+                         void *out, size_t out_cap, int64_t out_stride, size_t *out_bytes, hipStream_t st);
 int run_filter_grayscale(Context *c, int fmt, const void *in, size_t in_bytes, float *out, hipStream_t st);
 int run_filter_fastnoise(Context *c, float *img, hipStream_t st);
 int run_filter_zeromean(Context *c, float *img, hipStream_t st);
 int run_filter_wiener(Context *c, float *img, hipStream_t st);
 int run_filter_spectral(Context *c, const float *img, float *out, size_t out_cap, hipStream_t st);
 int run_pce(Context *c, int variant, const void *x, const void *y, double *out, hipStream_t st);
+int run_pce_pairs(Context *c, int variant, const void *const *d_tab, const int *d_pairs, int npairs, double *out,
+                  int batch, hipStream_t st);
 }  // namespace prnu
 
 using namespace prnu;
@@ -26,6 +30,17 @@ int prnu_preprocess(prnu_ctx *ctx, int input_format, const void *d_in, size_t in
     PRNU_CTX(ctx);
     PRNU_REQUIRE(d_in && d_out, "preprocess: null buffer");
     PRNU_TRY(run_preprocess(c, input_format, d_in, in_bytes, d_out, out_capacity, out_bytes, st));
+    return finish(c, stream);
+}
+
+int prnu_preprocess_batch(prnu_ctx *ctx, int input_format, const void *d_in, size_t in_bytes, int64_t in_stride,
+                          int count, void *d_out, size_t out_capacity, int64_t out_stride, prnu_stream stream) {
+    PRNU_CTX(ctx);
+    PRNU_REQUIRE(d_in && d_out, "preprocess_batch: null buffer");
+    if (count == 0) return PRNU_OK;
+    PRNU_REQUIRE(count > 0, "preprocess_batch: negative count %d", count);
+    PRNU_TRY(run_preprocess_batch(c, input_format, d_in, in_bytes, in_stride, count, d_out, out_capacity,
+                                  out_stride, nullptr, st));
     return finish(c, stream);
 }
 
@@ -54,6 +69,15 @@ int prnu_correlate_pce(prnu_ctx *ctx, int variant, const void *d_left, const voi
     PRNU_REQUIRE(c->mode == PRNU_MODE_PCE, "correlate_pce: context is not in PCE mode");
     PRNU_REQUIRE(d_left && d_right && d_out, "correlate_pce: null buffer");
     PRNU_TRY(run_pce(c, variant, d_left, d_right, d_out, st));
+    return finish(c, stream);
+}
+
+int prnu_pce_pairs(prnu_ctx *ctx, int variant, const void *const *d_spectra, const int32_t *d_pairs, int npairs,
+                   double *d_out, prnu_stream stream) {
+    PRNU_CTX(ctx);
+    PRNU_REQUIRE(c->mode == PRNU_MODE_PCE, "pce_pairs: context is not in PCE mode");
+    PRNU_REQUIRE(d_spectra && d_pairs && d_out, "pce_pairs: null buffer");
+    PRNU_TRY(run_pce_pairs(c, variant, d_spectra, d_pairs, npairs, d_out, 0, st));
     return finish(c, stream);
 }
 

@@ -1,0 +1,325 @@
+// Camera-identification serving on gfx950: a device-resident fingerprint
+// store (FP8-e4m3 copy for scanning, FP32 originals for exact re-scoring,
+// FP64 statistics) and the query path
+//
+//   query patterns -> FP64 stats -> FP8 quantisation -> FP8 MFMA scan of the
+//   whole store (NCC estimate per (query, fingerprint)) -> per-query top-k
+//   -> exact NCC of the k candidates (FP32 products, FP64 sums, reference
+//   formula of normalizedcrosscorrelation.cu:121-123) -> sort.
+//
+// Every per-query result depends only on that query: the scan runs the
+// queries as independent rows of 64-row tiles with a K split fixed by the
+// store size, top-k and re-score are per query, so a query scores the same
+// alone and in any batch.
+#include "prnu_internal.hpp"
+
+#include <algorithm>
+#include <new>
+
+namespace prnu {
+
+struct FpStore {
+    int64_t n = 0, stride = 0;
+    int capacity = 0, size = 0;  // size: slots filled (1 + highest slot put)
+    uint8_t *q = nullptr;        // capacity x stride, e4m3
+    float *scale = nullptr;      // capacity
+    float *f32 = nullptr;        // capacity x n, exact copy
+    double *sum = nullptr, *sumsq = nullptr;
+    float *absmax = nullptr;
+    Workspace ws_stats;
+};
+
+namespace {
+
+constexpr int kMaxK = 16;
+constexpr int kQueryBlock = 64;  // queries per scan tile (fp8_tile_kernel<64>)
+
+// out[i] row copy: dst + (first + i) * n <- ptrs[i]
+__global__ __launch_bounds__(256) void copy_rows_kernel(const float *const *ptrs, int64_t n, float *dst, int first) {
+    const float *src = ptrs[blockIdx.y];
+    float *d = dst + (int64_t)(first + blockIdx.y) * n;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) d[i] = src[i];
+}
+
+// Top-k of one row of `scores` (length N), k <= kMaxK: one 256-thread block
+// per row.  Each thread keeps the best kMaxK of its strided share, sorted, so
+// the union of the heads contains the row's top k; then k rounds of a block
+// arg-max over the list heads.  Ties go to the smaller index (deterministic).
+__device__ __forceinline__ bool better(double a, int ia, double b, int ib) { return a > b || (a == b && ia < ib); }
+
+__global__ __launch_bounds__(256) void topk_kernel(const double *scores, int64_t ld, int N, int k, int *out_idx) {
+    __shared__ double sv[256 * kMaxK];
+    __shared__ int si[256 * kMaxK];
+    __shared__ double wv[4];
+    __shared__ int wi[4], wt[4];
+    const double *row = scores + (int64_t)blockIdx.x * ld;
+    double v[kMaxK];
+    int ix[kMaxK];
+#pragma unroll
+    for (int q = 0; q < kMaxK; ++q) {
+        v[q] = -1e300;
+        ix[q] = 0x7fffffff;
+    }
+    for (int j = threadIdx.x; j < N; j += 256) {
+        double cv = row[j];
+        int ci = j;
+        if (better(cv, ci, v[kMaxK - 1], ix[kMaxK - 1])) {
+#pragma unroll
+            for (int q = 0; q < kMaxK; ++q) {
+                if (better(cv, ci, v[q], ix[q])) {
+                    const double tv = v[q];
+                    const int ti = ix[q];
+                    v[q] = cv;
+                    ix[q] = ci;
+                    cv = tv;
+                    ci = ti;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxK; ++q) {
+        sv[threadIdx.x * kMaxK + q] = v[q];
+        si[threadIdx.x * kMaxK + q] = ix[q];
+    }
+    int head = 0;
+    __syncthreads();
+    for (int r = 0; r < k; ++r) {
+        double bv = head < kMaxK ? sv[threadIdx.x * kMaxK + head] : -1e300;
+        int bi = head < kMaxK ? si[threadIdx.x * kMaxK + head] : 0x7fffffff;
+        int bt = threadIdx.x;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bv, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            const int ot = __shfl_xor(bt, off, 64);
+            if (better(ov, oi, bv, bi)) {
+                bv = ov;
+                bi = oi;
+                bt = ot;
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wv[threadIdx.x >> 6] = bv;
+            wi[threadIdx.x >> 6] = bi;
+            wt[threadIdx.x >> 6] = bt;
+        }
+        __syncthreads();
+        bv = wv[0];
+        bi = wi[0];
+        bt = wt[0];
+        for (int q = 1; q < 4; ++q)
+            if (better(wv[q], wi[q], bv, bi)) {
+                bv = wv[q];
+                bi = wi[q];
+                bt = wt[q];
+            }
+        if (threadIdx.x == 0) out_idx[(int64_t)blockIdx.x * k + r] = bi < N ? bi : -1;
+        if (threadIdx.x == bt) ++head;
+        __syncthreads();
+    }
+}
+
+// Exact NCC of (query q, candidate r): grid (k, nq), block 256.
+__global__ __launch_bounds__(256) void rescore_kernel(const float *const *qptrs, const double *qsum,
+                                                      const double *qsumsq, const float *f32, int64_t n,
+                                                      const double *fsum, const double *fsumsq, const int *idx,
+                                                      int k, double *score) {
+    __shared__ double sh[4];
+    const int q = blockIdx.y, r = blockIdx.x;
+    const int c = idx[q * k + r];
+    if (c < 0) {
+        if (threadIdx.x == 0) score[q * k + r] = -1e300;
+        return;
+    }
+    const float *x = qptrs[q];
+    const float *y = f32 + (int64_t)c * n;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const float p = x[i] * y[i];  // FP32 product, FP64 sum
+        s += p;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double sxy = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        const double dx = sqrt(qsumsq[q] - qsum[q] * qsum[q]);
+        const double dy = sqrt(fsumsq[c] - fsum[c] * fsum[c]);
+        score[q * k + r] = sxy / (dx * dy);
+    }
+}
+
+// Sort each query's k (score, index) by descending exact score (ties: index).
+__global__ void sort_k_kernel(double *score, int *idx, int nq, int k) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double *s = score + (int64_t)q * k;
+    int *ix = idx + (int64_t)q * k;
+    for (int a = 1; a < k; ++a) {
+        const double v = s[a];
+        const int iv = ix[a];
+        int b = a - 1;
+        while (b >= 0 && better(v, iv, s[b], ix[b])) {
+            s[b + 1] = s[b];
+            ix[b + 1] = ix[b];
+            --b;
+        }
+        s[b + 1] = v;
+        ix[b + 1] = iv;
+    }
+}
+
+size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+int fpstore_create(int64_t n, int capacity, FpStore **out) {
+    *out = nullptr;
+    PRNU_REQUIRE(n >= 1 && capacity >= 1, "fpstore_create: n and capacity must be positive");
+    FpStore *s = new (std::nothrow) FpStore();
+    if (!s) {
+        set_error("fpstore_create: out of host memory");
+        return PRNU_ENOMEM;
+    }
+    s->n = n;
+    s->stride = fp8_row_stride(n);
+    s->capacity = capacity;
+    hipError_t e = hipMalloc(&s->q, (size_t)s->stride * capacity);
+    if (e == hipSuccess) e = hipMalloc(&s->f32, sizeof(float) * (size_t)n * capacity);
+    if (e == hipSuccess) e = hipMalloc(&s->scale, sizeof(float) * capacity);
+    if (e == hipSuccess) e = hipMalloc(&s->sum, sizeof(double) * capacity);
+    if (e == hipSuccess) e = hipMalloc(&s->sumsq, sizeof(double) * capacity);
+    if (e == hipSuccess) e = hipMalloc(&s->absmax, sizeof(float) * capacity);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("fpstore_create: device memory for %d fingerprints of %lld values: %s", capacity, (long long)n,
+                  hipGetErrorString(e));
+        for (void *p : {(void *)s->q, (void *)s->f32, (void *)s->scale, (void *)s->sum, (void *)s->sumsq,
+                        (void *)s->absmax})
+            if (p) (void)hipFree(p);
+        delete s;
+        return PRNU_ENOMEM;
+    }
+    *out = s;
+    return PRNU_OK;
+}
+
+void fpstore_destroy(FpStore *s) {
+    if (!s) return;
+    (void)hipDeviceSynchronize();
+    for (void *p : {(void *)s->q, (void *)s->f32, (void *)s->scale, (void *)s->sum, (void *)s->sumsq,
+                    (void *)s->absmax})
+        if (p) (void)hipFree(p);
+    s->ws_stats.release();
+    delete s;
+}
+
+int fpstore_put(FpStore *s, const float *const *d_ptrs, int count, int first, hipStream_t st) {
+    PRNU_REQUIRE(count >= 1 && first >= 0 && first + count <= s->capacity,
+                 "fpstore_put: slots [%d, %d) outside the capacity %d", first, first + count, s->capacity);
+    PRNU_TRY(launch_pattern_stats(d_ptrs, count, s->n, s->sum + first, s->sumsq + first, s->absmax + first,
+                                  s->ws_stats, st));
+    PRNU_TRY(launch_quantize_fp8(d_ptrs, count, s->n, s->absmax + first, s->q + (size_t)first * s->stride,
+                                 s->scale + first, st));
+    const unsigned gx = (unsigned)std::min<int64_t>(1024, (s->n + 255) / 256);
+    hipLaunchKernelGGL(copy_rows_kernel, dim3(gx, count), dim3(256), 0, st, d_ptrs, s->n, s->f32, first);
+    PRNU_CHECK_LAUNCH();
+    s->size = std::max(s->size, first + count);
+    return PRNU_OK;
+}
+
+// nq query patterns (device pointer table) against slots [0, size).
+int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, int k, int *d_idx, double *d_score,
+                  double *d_scan, hipStream_t st) {
+    PRNU_REQUIRE(nq >= 1 && nq <= 65535, "query: 1..65535 queries per call");
+    PRNU_REQUIRE(k >= 1 && k <= kMaxK, "query: k must be in [1, %d]", kMaxK);
+    PRNU_REQUIRE(s->size >= 1, "query: the store is empty");
+    PRNU_REQUIRE(c->pixels() == s->n, "query: context size %lld does not match the store (%lld)",
+                 (long long)c->pixels(), (long long)s->n);
+    const int N = s->size;
+    PRNU_REQUIRE(k <= N, "query: k=%d exceeds the %d stored fingerprints", k, N);
+    // workspace: query stats, FP8 queries + scales, scan scores
+    const size_t b_stats = a256(sizeof(double) * nq) * 2 + a256(sizeof(float) * nq);
+    const size_t b_q = a256((size_t)s->stride * nq) + a256(sizeof(float) * nq);
+    const size_t b_scan = d_scan ? 0 : a256(sizeof(double) * (size_t)nq * N);
+    PRNU_TRY(c->ws_fp8.ensure(b_stats + b_q + b_scan));
+    char *p = static_cast<char *>(c->ws_fp8.ptr);
+    double *qsum = reinterpret_cast<double *>(p);
+    double *qsumsq = reinterpret_cast<double *>(p + a256(sizeof(double) * nq));
+    float *qabs = reinterpret_cast<float *>(p + 2 * a256(sizeof(double) * nq));
+    uint8_t *qq = reinterpret_cast<uint8_t *>(p + b_stats);
+    float *qscale = reinterpret_cast<float *>(p + b_stats + a256((size_t)s->stride * nq));
+    double *scan = d_scan ? d_scan : reinterpret_cast<double *>(p + b_stats + b_q);
+
+    PRNU_TRY(launch_pattern_stats(d_qptrs, nq, s->n, qsum, qsumsq, qabs, c->ws_small, st));
+    PRNU_TRY(launch_quantize_fp8(d_qptrs, nq, s->n, qabs, qq, qscale, st));
+    time_begin(c, st);
+    for (int q0 = 0; q0 < nq; q0 += kQueryBlock) {
+        const int m = std::min(kQueryBlock, nq - q0);
+        PRNU_TRY(launch_fp8_ncc(c->ws_gram, kQueryBlock, false, qq + (size_t)q0 * s->stride, s->stride, m,
+                                qscale + q0, qsum + q0, qsumsq + q0, s->q, s->stride, N, s->scale, s->sum, s->sumsq,
+                                s->stride, scan + (size_t)q0 * N, N, c->gram_cfg[PRNU_PREC_FP8].slabs, st, nullptr));
+    }
+    time_end(c, st);
+    hipLaunchKernelGGL(topk_kernel, dim3(nq), dim3(256), 0, st, scan, (int64_t)N, N, k, d_idx);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rescore_kernel, dim3(k, nq), dim3(256), 0, st, d_qptrs, qsum, qsumsq, s->f32, s->n, s->sum,
+                       s->sumsq, d_idx, k, d_score);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sort_k_kernel, dim3((nq + 63) / 64), dim3(64), 0, st, d_score, d_idx, nq, k);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+}  // namespace prnu
+
+using namespace prnu;
+
+extern "C" {
+
+int prnu_fpstore_create(int64_t n, int capacity, prnu_fpstore **out) {
+    PRNU_REQUIRE(out, "fpstore_create: null output");
+    FpStore *s = nullptr;
+    PRNU_TRY(fpstore_create(n, capacity, &s));
+    *out = reinterpret_cast<prnu_fpstore *>(s);
+    return PRNU_OK;
+}
+
+int prnu_fpstore_destroy(prnu_fpstore *store) {
+    fpstore_destroy(reinterpret_cast<FpStore *>(store));
+    return PRNU_OK;
+}
+
+int prnu_fpstore_info(const prnu_fpstore *store, int64_t *n, int *capacity, int *size, size_t *bytes) {
+    PRNU_REQUIRE(store, "fpstore_info: null store");
+    const FpStore *s = reinterpret_cast<const FpStore *>(store);
+    if (n) *n = s->n;
+    if (capacity) *capacity = s->capacity;
+    if (size) *size = s->size;
+    if (bytes) *bytes = (size_t)s->capacity * (s->stride + 4 * s->n + 28);
+    return PRNU_OK;
+}
+
+int prnu_fpstore_put(prnu_fpstore *store, const float *const *d_patterns, int count, int first_slot,
+                     prnu_stream stream) {
+    PRNU_REQUIRE(store && d_patterns, "fpstore_put: null pointer");
+    FpStore *s = reinterpret_cast<FpStore *>(store);
+    PRNU_TRY(fpstore_put(s, d_patterns, count, first_slot, (hipStream_t)stream));
+    if (!stream) PRNU_CHECK_HIP(hipStreamSynchronize(nullptr));
+    return PRNU_OK;
+}
+
+int prnu_fpstore_query(prnu_ctx *ctx, prnu_fpstore *store, const float *const *d_queries, int nq, int k,
+                       int32_t *d_idx, double *d_score, double *d_scan, prnu_stream stream) {
+    PRNU_REQUIRE(ctx && store && d_queries && d_idx && d_score, "fpstore_query: null pointer");
+    Context *c = reinterpret_cast<Context *>(ctx);
+    FpStore *s = reinterpret_cast<FpStore *>(store);
+    hipStream_t st = pick_stream(c, stream);
+    PRNU_TRY(fpstore_query(c, s, d_queries, nq, k, d_idx, d_score, d_scan, st));
+    return finish(c, stream);
+}
+
+}  // extern "C"

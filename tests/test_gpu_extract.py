@@ -207,3 +207,63 @@ def test_end_to_end_ncc_prnu64_subset():
     assert same.min() > diff.max()
     gram.close()
     ctx.destroy()
+
+
+@pytest.mark.parametrize("variant", [_lib.PCE_REFERENCE, _lib.PCE_TEXTBOOK])
+@pytest.mark.parametrize("h,w", [(48, 64), (45, 60)])
+def test_pce_pairs_batched_equals_single(variant, h, w, rng):
+    k = rng.normal(size=(h, w))
+    pats = [(0.3 * k * (i % 2) + rng.normal(size=(h, w))).astype(np.float32) for i in range(6)]
+    spec = _spectra(h, w, pats)
+    cmp = csi.PeakToCorrelationEnergy(h, w, variant)
+    pairs = np.array([(i, j) for i in range(6) for j in range(i + 1, 6)], dtype=np.int32)
+    got = cmp.pairs(spec, pairs).cpu().numpy()
+    one = torch.zeros(1, dtype=torch.float64, device="cuda")
+    fn = O.pce_reference if variant == _lib.PCE_REFERENCE else O.pce_textbook
+    for q, (i, j) in enumerate(pairs):
+        cmp.apply_gpu(spec[i], spec[j], one)
+        assert got[q] == one.item()                    # same kernels, same order: bitwise
+        assert got[q] == pytest.approx(fn(pats[i].astype(np.float64), pats[j].astype(np.float64)), rel=1e-4)
+    cmp.cleanup()
+
+
+@pytest.mark.parametrize("h,w,fmt,spectral", [(45, 60, _lib.INPUT_RGB8, False), (80, 96, _lib.INPUT_GRAY8, False),
+                                              (48, 64, _lib.INPUT_GRAY8, True), (96, 80, _lib.INPUT_RGB8, True)])
+def test_preprocess_batch_equals_single(h, w, fmt, spectral, rng):
+    """prnu_preprocess_batch over strided rows (padding between images and
+    between patterns) is bitwise prnu_preprocess of each image alone, and
+    matches the oracle chain."""
+    count, bpp = 7, (3 if fmt == _lib.INPUT_RGB8 else 1)
+    imgs = rng.integers(0, 256, size=(count, h * w * bpp + 37), dtype=np.uint8)   # 37 pad bytes per row
+    f = csi.PRNUFilter(h, w, apply_spectral_filter=spectral)
+    plen = f.get_output_size() // 4
+    out = torch.full((count, plen + 16), -7.0, device="cuda")                      # 16 pad floats per row
+    d_imgs = cuda(imgs)
+    f.apply_gpu_batch(d_imgs[:, : h * w * bpp], out[:, :plen], fmt)
+    one = torch.empty(plen, device="cuda")
+    for i in range(count):
+        f.apply_gpu(d_imgs[i, : h * w * bpp].contiguous(), one, fmt)
+        assert torch.equal(out[i, :plen], one), f"image {i}"
+        assert torch.all(out[i, plen:] == -7.0)                                   # padding untouched
+    img0 = imgs[0, : h * w * bpp].reshape((h, w, 3) if bpp == 3 else (h, w))
+    want = O.prnu_pattern(img0)
+    got = out[0, :plen].cpu().numpy()
+    if spectral:
+        s_want = O.spectrum(want)
+        got = got.view(np.complex64).reshape(h, w // 2 + 1)
+        assert np.abs(got - s_want).max() <= 2e-5 * np.abs(s_want).max()
+    else:
+        assert rel_err(got.reshape(h, w), want) <= 2e-5
+    f.cleanup()
+
+
+def test_preprocess_batch_rejects_bad_strides():
+    h, w = 16, 16
+    f = csi.PRNUFilter(h, w)
+    imgs = torch.zeros((3, h * w * 3), dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError):                    # pattern rows shorter than a pattern
+        f.apply_gpu_batch(imgs, torch.empty((3, h * w - 1), device="cuda"))
+    with pytest.raises(ValueError):                    # image rows shorter than an RGB image
+        f.apply_gpu_batch(imgs[:, : h * w], torch.empty((3, h * w), device="cuda"))
+    f.apply_gpu_batch(imgs[:0], torch.empty((0, h * w), device="cuda"))   # empty batch: no-op
+    f.cleanup()

@@ -1,0 +1,139 @@
+"""GPU tests of the fingerprint store and the continuous-batching server.
+
+* exact re-scores match the FP64 oracle NCC of the reference formula
+  (FP32 products summed in FP64 on both sides -> relative 1e-10);
+* the top-k is the oracle's exact top-k wherever the FP8 scan can separate
+  candidates (queries derived from a stored fingerprint find it first);
+* a query's result is bitwise the same alone and inside any batch;
+* the server returns, for every request, the result of that query alone.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import prnu_oracle as O  # noqa: E402
+from rocket_amd import _lib, csi, serving  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def fingerprints(count, n, seed):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    f = torch.randn((count, n), generator=g, device="cuda")
+    return (f - f.mean(dim=1, keepdim=True)).contiguous()
+
+
+def queries_from(fp, owners, seed, gain=0.3):
+    """Patterns of images of the owners' cameras (owner -1: unknown camera)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    q = torch.randn((len(owners), fp.shape[1]), generator=g, device="cuda")
+    for i, o in enumerate(owners):
+        if o >= 0:
+            q[i] += gain * fp[o]
+    return (q - q.mean(dim=1, keepdim=True)).contiguous()
+
+
+@pytest.fixture(scope="module")
+def small_store():
+    h, w, cap = 128, 96, 300
+    fp = fingerprints(cap, h * w, seed=5)
+    store = serving.FingerprintStore(h, w, cap)
+    store.put(fp[:200])
+    store.put(fp[200:], first_slot=200)
+    yield h, w, fp, store
+    store.close()
+
+
+def test_store_info(small_store):
+    h, w, fp, store = small_store
+    info = store.info()
+    assert info["size"] == 300 and info["capacity"] == 300 and info["n"] == h * w
+
+
+def test_query_topk_and_exact_scores(small_store):
+    h, w, fp, store = small_store
+    owners = [3, 150, 299, -1, 42, -1, 7]
+    q = queries_from(fp, owners, seed=9)
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    idx, score, scan = store.query(ctx, q, k=10, scan=True)
+    idx, score, scan = idx.cpu().numpy(), score.cpu().numpy(), scan.cpu().numpy()
+    qh, fh = q.cpu().numpy(), fp.cpu().numpy()
+    for r, o in enumerate(owners):
+        if o >= 0:
+            assert idx[r, 0] == o
+        for c in range(10):
+            want = O.ncc_reference(qh[r], fh[idx[r, c]])
+            assert score[r, c] == pytest.approx(want, rel=1e-10, abs=1e-13)
+        assert np.all(np.diff(score[r]) <= 0)                     # sorted by exact score
+        # the top-k of the FP8 scan is its own exact ranking's prefix
+        assert set(idx[r]) == set(np.argsort(-scan[r], kind="stable")[:10])
+    # FP8 scan within the e4m3 bound of the exact NCC (6 sigma, see test_gpu_ncc)
+    exact = O.ncc_matrix(np.concatenate([qh, fh]))[: len(owners), len(owners):]
+    assert np.abs(scan - exact).max() <= 6 * 0.05 / np.sqrt(h * w) + 2e-3
+    ctx.close()
+
+
+def test_same_alone_and_in_mix(small_store):
+    h, w, fp, store = small_store
+    owners = list(range(0, 300, 7)) + [-1] * 5
+    q = queries_from(fp, owners, seed=11)
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    idx_all, sc_all = (t.cpu().numpy() for t in store.query(ctx, q, k=10))
+    for r in [0, 5, len(owners) - 1]:
+        idx1, sc1 = (t.cpu().numpy() for t in store.query(ctx, q[r:r + 1], k=10))
+        assert np.array_equal(idx1[0], idx_all[r]) and np.array_equal(sc1[0], sc_all[r])
+    ctx.close()
+
+
+def test_query_rejects_bad_k(small_store):
+    h, w, fp, store = small_store
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    with pytest.raises(ValueError):
+        store.query(ctx, fp[:1], k=17)
+    ctx.close()
+
+
+def test_server_continuous_batching_matches_alone():
+    """Bursts of requests from a client thread while batches are in flight;
+    every answer equals the answer for that image queried alone."""
+    from rocket_amd import synthetic
+    h, w, cams = 64, 80, 24
+    fp_cpu = np.stack([synthetic.camera_fingerprint(c, h, w) for c in range(cams)]).astype(np.float32)
+    fp = torch.from_numpy(fp_cpu).cuda()
+    store = serving.FingerprintStore(h, w, cams)
+    store.put(fp)
+    images = synthetic.make_images(40, h, w, cams + 8, workers=1)  # cameras >= 24 are unknown
+    srv = serving.Server(store, k=5, max_batch=16, slots=2).start()
+    reqs = []
+
+    def client():
+        rng = np.random.default_rng(3)
+        i = 0
+        while i < len(images):
+            burst = int(rng.integers(1, 9))
+            for img in images[i:i + burst]:
+                reqs.append(srv.submit(img))
+            i += burst
+            time.sleep(float(rng.exponential(0.003)))
+
+    t = threading.Thread(target=client)
+    t.start()
+    t.join()
+    for r in reqs:
+        assert r.wait(60), "request not answered"
+    srv.stop()
+    assert len(srv.batches) >= 2 and max(srv.batches) > 1          # really batched
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    pat = torch.empty(h * w, device="cuda")
+    for i, r in enumerate(reqs):
+        prep = csi.PRNUFilter(h, w, _ctx=ctx)
+        prep.apply_gpu(torch.from_numpy(images[i]).cuda(), pat, _lib.INPUT_GRAY8)
+        idx1, sc1 = (t.cpu().numpy() for t in store.query(ctx, pat.view(1, -1).clone(), k=5))
+        assert np.array_equal(idx1[0], r.idx) and np.array_equal(sc1[0], r.score)
+        assert r.latency > 0
+    ctx.close()
+    store.close()
