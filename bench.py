@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the query-serving leg")
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
+    ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
     return ap.parse_args()
 
 
@@ -202,6 +203,83 @@ def serving_bench(torch, duration: float):
     return res
 
 
+# uneven groups: (images, height, width); 40 cameras in total, all pairs within each group
+GROUPS_C3 = [(2000, 1024, 1024), (1500, 1536, 2048), (596, 3000, 4000)]
+
+
+def groups_bench(torch, steps=5):
+    """All-pairs NCC of three image-size groups (2000 @ 1024x1024, 1500 @
+    2048x1536, 596 @ 4000x3000; 3,300,560 pairs) from split-BF16 stores: one
+    grouped launch (autotuned slab, replayed as a HIP graph) against one
+    store Gram launch per group; FP64 check on 24 patterns of each group."""
+    from rocket_amd import csi
+    g = torch.Generator(device="cuda").manual_seed(31337)
+    stores, checks, spec = [], [], []
+    cam0 = 0
+    t0 = time.time()
+    for count, h, w in GROUPS_C3:
+        n = h * w
+        cams = {1024: 16, 1536: 16, 3000: 8}[h]
+        k = torch.randn((cams, n), generator=g, device="cuda")
+        st = csi.PatternStore(n, count, "split_bf16")
+        chunk = max(8, int((1 << 30) // (4 * n)))
+        for b in range(0, count, chunk):
+            e = min(count, b + chunk)
+            blk = torch.randn((e - b, n), generator=g, device="cuda")
+            blk += 0.02 * k[torch.arange(b, e, device="cuda") % cams]
+            blk -= blk.mean(dim=1, keepdim=True)
+            st.put(blk, first_slot=b)
+            if b == 0:
+                checks.append(blk[:24].cpu().numpy().astype(np.float64))
+            del blk
+        del k
+        cam0 += cams
+        stores.append(st)
+        spec.append((st, 0, count))
+    torch.cuda.synchronize()
+    log(f"groups: stores ready ({sum(c for c, _, _ in GROUPS_C3)} patterns, {cam0} cameras) in {time.time() - t0:.1f} s")
+    gram = csi.GramNCC(8, 8)
+    plan = csi.GramPlan(gram, spec)
+    slab, tune_ms = plan.autotune()
+    info = plan.info()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1000.0 * (time.perf_counter() - t1) / reps
+
+    ms_graph = timed(lambda: plan.run(use_graph=True, sync=False), steps)
+    ms_eager = timed(lambda: plan.run(use_graph=False, sync=False), steps)
+    outs = [o.cpu().numpy() for o in plan.outputs]
+    sep_out = [torch.empty((c, c), dtype=torch.float64, device="cuda") for c, _, _ in GROUPS_C3]
+
+    def separate():
+        for (st, first, c), o in zip(spec, sep_out):
+            st.gram(gram, c, first, out=o, sync=False)
+    ms_sep = timed(separate, steps)
+    dncc = 0.0
+    for host, got in zip(checks, outs):
+        gm = host @ host.T
+        sm = host.sum(axis=1)
+        d = np.sqrt(np.einsum("ij,ij->i", host, host) - sm * sm)
+        dncc = max(dncc, float(np.abs(got[:24, :24] - gm / np.outer(d, d)).max()))
+    pairs = sum(c * (c - 1) // 2 for c, _, _ in GROUPS_C3)
+    flops = 3 * info["flop"]
+    plan.close()
+    for st in stores:
+        st.close()
+    gram.close()
+    return {"workload": "groups 2000@1024x1024 + 1500@2048x1536 + 596@4000x3000", "pairs": pairs,
+            "pairs_per_s": pairs / (ms_graph / 1000.0), "ms_per_step": ms_graph, "eager_ms_per_step": ms_eager,
+            "separate_launches_ms_per_step": ms_sep, "slab_values": slab, "work_items": info["work_items"],
+            "mfma_tflops": flops / (ms_graph / 1000.0) / 1e12, "max_abs_dncc_vs_fp64_sample": dncc,
+            "autotune_best_ms": tune_ms}
+
+
 def extraction_bench(torch, batch=32, pce_spectra=32):
     """PRNU extraction of RGB8 frames of the workload size (one batched call per
     `batch` frames) in images/s, and the PCE of all pairs of `pce_spectra`
@@ -350,6 +428,14 @@ def main():
 
     del p                                               # free 12.9 GB before the other legs
     torch.cuda.empty_cache()
+    groups = None
+    if not args.no_groups:
+        log("uneven groups ...")
+        groups = groups_bench(torch)
+        log(f"groups: {groups['pairs_per_s']:.4g} pairs/s ({groups['ms_per_step']:.2f} ms graph, "
+            f"{groups['eager_ms_per_step']:.2f} ms eager, {groups['separate_launches_ms_per_step']:.2f} ms as "
+            f"separate launches), |dNCC| {groups['max_abs_dncc_vs_fp64_sample']:.2e}")
+
     extract = None
     if not args.no_extract:
         log("extraction / PCE ...")
@@ -386,6 +472,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serve,
         "extraction": extract,
+        "groups": groups,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

@@ -221,6 +221,44 @@ PRNU_EXPORT int prnu_store_stats(const prnu_store *store, const double **d_sum, 
 PRNU_EXPORT int prnu_store_gram_ncc(prnu_ctx *ctx, prnu_store *store, int first_slot, int count, double *d_ncc,
                                     prnu_stream stream);
 
+/* ------------------------------------------------- grouped all-pairs plan
+ * Image sets of several sizes (cameras of different resolutions): the
+ * reference compares only patterns of one size per context
+ * (CSI/CommonSourceIdentificationContext.java:97-99) and enumerates the pairs
+ * of one set with an index space (R/indexspace/HilbertCurveIndexSpace.java);
+ * a job over G sets is G such runs.  A plan computes the all-pairs NCC of
+ * every group in ONE launch over the concatenated triangular tile
+ * lists of all groups (equal-cost work items: every group is cut into K slabs
+ * of the same length), then one FP64 reduction per group; each group's output
+ * is what prnu_store_gram_ncc writes for it.  Groups may share a store (slot
+ * ranges) or use different stores; all stores must have the same format. */
+typedef struct prnu_gram_group {
+    prnu_store *store;
+    int32_t first_slot;
+    int32_t count;
+    double *d_ncc;          /* count x count FP64, row-major, device */
+} prnu_gram_group;
+typedef struct prnu_gram_plan prnu_gram_plan;
+
+/* slab_values: K values per work item (a multiple of 32), 0 = automatic
+ * (about 8 work items per CU).  The plan owns its partial-tile
+ * buffer (at most 8 GiB) and keeps `ctx` for timing; destroy it before ctx. */
+PRNU_EXPORT int prnu_gram_plan_create(prnu_ctx *ctx, const prnu_gram_group *groups, int ngroups,
+                                      int64_t slab_values, prnu_gram_plan **out);
+PRNU_EXPORT int prnu_gram_plan_destroy(prnu_gram_plan *plan);
+/* use_graph != 0: the first run captures the launches into a HIP graph,
+ * later runs replay it (one hipGraphLaunch); 0: plain launches. */
+PRNU_EXPORT int prnu_gram_plan_run(prnu_gram_plan *plan, int use_graph, prnu_stream stream);
+/* Times candidate slab lengths on the device (writes the outputs) and keeps
+ * the fastest; drops a captured graph. */
+PRNU_EXPORT int prnu_gram_plan_autotune(prnu_gram_plan *plan, prnu_stream stream, int64_t *best_slab_values,
+                                        float *best_ms);
+/* Work items, workgroups launched, slab length, partial bytes and the
+ * algorithmic FLOP of one product over every group's triangle (diagonal
+ * included). */
+PRNU_EXPORT int prnu_gram_plan_info(const prnu_gram_plan *plan, int64_t *work_items, int *workgroups,
+                                    int64_t *slab_values, size_t *partial_bytes, double *flop);
+
 /* ------------------------------------------------------ fingerprint serving
  * No reference counterpart (the reference has no serving mode).  A device-
  * resident store of camera fingerprints: an FP8-e4m3 copy with a per-slot

@@ -83,6 +83,12 @@ _SIGNATURES = {
     "prnu_store_stats": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p), ctypes.POINTER(_c_void_p),
                                   ctypes.POINTER(_c_void_p)]),
     "prnu_store_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p]),
+    "prnu_gram_plan_create": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_i64, ctypes.POINTER(_c_void_p)]),
+    "prnu_gram_plan_destroy": (_c_int, [_c_void_p]),
+    "prnu_gram_plan_run": (_c_int, [_c_void_p, _c_int, _c_void_p]),
+    "prnu_gram_plan_autotune": (_c_int, [_c_void_p, _c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(ctypes.c_float)]),
+    "prnu_gram_plan_info": (_c_int, [_c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int), ctypes.POINTER(_c_i64),
+                                     ctypes.POINTER(_c_size), ctypes.POINTER(ctypes.c_double)]),
     "prnu_fpstore_create": (_c_int, [_c_i64, _c_int, ctypes.POINTER(_c_void_p)]),
     "prnu_fpstore_destroy": (_c_int, [_c_void_p]),
     "prnu_fpstore_info": (_c_int, [_c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int),

@@ -460,3 +460,58 @@ class PatternStore:
             self.close()
         except Exception:
             pass
+
+
+class _GramGroup(ctypes.Structure):
+    _fields_ = [("store", ctypes.c_void_p), ("first_slot", ctypes.c_int32), ("count", ctypes.c_int32),
+                ("d_ncc", ctypes.c_void_p)]
+
+
+class GramPlan:
+    """All-pairs NCC of several pattern groups (e.g. one per image size) in one
+    launch (``prnu_gram_plan_*``).  ``groups`` is a list of
+    ``(PatternStore, first_slot, count)``; outputs are (count, count) FP64
+    CUDA tensors, one per group, in :attr:`outputs`."""
+
+    def __init__(self, gram_ctx: "GramNCC", groups, slab_values: int = 0):
+        lib = _lib.load()
+        self._keep = [g[0] for g in groups]          # stores must outlive the plan
+        self.outputs = [torch.empty((int(c), int(c)), dtype=torch.float64, device="cuda") for _, _, c in groups]
+        arr = (_GramGroup * len(groups))()
+        for i, ((store, first, count), out) in enumerate(zip(groups, self.outputs)):
+            arr[i] = _GramGroup(store.handle.value, int(first), int(count), out.data_ptr())
+        h = ctypes.c_void_p()
+        _lib.check(lib.prnu_gram_plan_create(gram_ctx._ctx.handle, arr, len(groups), int(slab_values),
+                                             ctypes.byref(h)), "prnu_gram_plan_create")
+        self.handle = h
+        self._ctx = gram_ctx
+
+    def run(self, use_graph: bool = True, sync: bool = True):
+        _lib.call("prnu_gram_plan_run", self.handle, int(bool(use_graph)), _stream())
+        _sync(sync)
+        return self.outputs
+
+    def autotune(self) -> tuple[int, float]:
+        v, ms = ctypes.c_int64(), ctypes.c_float()
+        _lib.call("prnu_gram_plan_autotune", self.handle, _stream(), ctypes.byref(v), ctypes.byref(ms))
+        _sync(True)
+        return v.value, ms.value
+
+    def info(self) -> dict:
+        items, wg, slab, part, flop = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t(), \
+            ctypes.c_double()
+        _lib.call("prnu_gram_plan_info", self.handle, ctypes.byref(items), ctypes.byref(wg), ctypes.byref(slab),
+                  ctypes.byref(part), ctypes.byref(flop))
+        return {"work_items": items.value, "workgroups": wg.value, "slab_values": slab.value,
+                "partial_bytes": part.value, "flop": flop.value}
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().prnu_gram_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <new>
+#include <vector>
 
 namespace prnu {
 
@@ -40,6 +41,7 @@ constexpr int kBlkBytes = 128;  // bytes per block
 constexpr int kTileS = 256;
 constexpr int kRowsS = 2 * kTileS;
 constexpr int kThrS = 512;
+constexpr int kMaxPlanGroups = 64;
 
 __device__ inline uint16_t bf16_bits_s(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 __device__ inline float bf16_val_s(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
@@ -162,32 +164,22 @@ __device__ inline int2 tri_tile_s(int t, int nt) {
     return make_int2(I, I + t);
 }
 
-// 16-byte chunk c of LDS row r, XOR-swizzled by row.
+// 16-byte chunk c of LDS row r (128-byte rows), XOR-swizzled by row.
 __device__ __forceinline__ int s_off(int row, int chunk) { return row * kBlkBytes + ((chunk ^ (row & 7)) << 4); }
 
-template <int FMT>
-__global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
-    constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
-    constexpr int kBuf = kRowsS * kBlkBytes;  // 64 KiB per stage
-    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBuf];
+constexpr int kBufS = kRowsS * kBlkBytes;  // 64 KiB per LDS stage
 
-    const int bid = blockIdx.x;
-    int t, s;
-    if (a.xcd_remap) {
-        const int local = bid >> 3;
-        t = local % a.T;
-        s = (local / a.T) * 8 + (bid & 7);
-    } else {
-        t = bid % a.T;
-        s = bid / a.T;
-    }
-    if (s >= a.S) return;
-    const int2 tile = tri_tile_s(t, a.nt);
+// One work item: the 256 x 256 tile `tile` of the slots [first, first+count)
+// over the K blocks [b_begin, b_end), into the FP32 partial tile `out`.
+// Ends with a barrier, so a caller may start the next item on the same LDS.
+template <int FMT>
+__device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, int64_t cap_bytes, int first, int count,
+                                          int2 tile, int64_t b_begin, int64_t b_end, float *__restrict__ out,
+                                          uint8_t *lds) {
+    constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
     const bool diag = tile.x == tile.y;
     const int nrows = diag ? kTileS : kRowsS;
     const int bbase = diag ? 0 : kTileS;
-    const int64_t b_begin = (int64_t)s * a.kcb;
-    const int64_t b_end = min(a.nkb, b_begin + a.kcb);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wrow0 = (wave >> 2) * 128, wcol0 = (wave & 3) * 64;
@@ -195,22 +187,18 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
     // thread -> (row = (tid >> 3) + 64*it, chunk = tid & 7); rows past count
     // read a clamped slot and are zeroed on the LDS store
     const int r0 = tid >> 3, chunk = tid & 7;
-    const int la = min(tile.x * kTileS + r0, a.count - 1);
-    const int lb = min(tile.y * kTileS + r0, a.count - 1);
     uint32_t valid = 0;
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
         const int r = r0 + 64 * it;
         const int loc = (r < kTileS) ? tile.x * kTileS + r : tile.y * kTileS + (r - kTileS);
-        valid |= (uint32_t)(r < nrows && loc < a.count) << it;
+        valid |= (uint32_t)(r < nrows && loc < count) << it;
     }
     // per-row byte offsets inside one block column (rows clamped into the set)
     auto row_off = [&](int it) -> int64_t {
-        if (it < 4) return (int64_t)(a.first + min(tile.x * kTileS + r0 + 64 * it, a.count - 1)) * kBlkBytes;
-        return (int64_t)(a.first + min(tile.y * kTileS + r0 + 64 * (it - 4), a.count - 1)) * kBlkBytes;
+        if (it < 4) return (int64_t)(first + min(tile.x * kTileS + r0 + 64 * it, count - 1)) * kBlkBytes;
+        return (int64_t)(first + min(tile.y * kTileS + r0 + 64 * (it - 4), count - 1)) * kBlkBytes;
     };
-    (void)la;
-    (void)lb;
 
     floatx4 acc[8][4];
 #pragma unroll
@@ -220,7 +208,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
 
     uint4 st[8];
     auto load = [&](int64_t b) {
-        const uint8_t *col = a.blocks + b * a.cap_bytes + (chunk << 4);
+        const uint8_t *col = blocks + b * cap_bytes + (chunk << 4);
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             if (it < 4 || !diag) {
@@ -294,12 +282,11 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
     for (int64_t q = 0; q < nst; ++q) {
         const bool next = q + 1 < nst;
         if (next) load(b_begin + q + 1);
-        compute(lds + (q & 1) * kBuf);
-        if (next) store(lds + ((q + 1) & 1) * kBuf);
+        compute(lds + (q & 1) * kBufS);
+        if (next) store(lds + ((q + 1) & 1) * kBufS);
         __syncthreads();
     }
 
-    float *out = a.partials + ((int64_t)t * a.S + s) * (kTileS * kTileS);
     const int ocol = lane & 15, orow = (lane >> 4) * 4;
 #pragma unroll
     for (int rb = 0; rb < 8; ++rb)
@@ -308,6 +295,65 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 out[(wrow0 + rb * 16 + orow + r) * kTileS + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
+}
+
+// One launch = one work item per workgroup.  With xcd_remap, the T tiles of
+// a slab run on one XCD (workgroup ids are dealt to the 8 XCDs round-robin).
+template <int FMT>
+__global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
+    const int bid = blockIdx.x;
+    int t, s;
+    if (a.xcd_remap) {
+        const int local = bid >> 3;
+        t = local % a.T;
+        s = (local / a.T) * 8 + (bid & 7);
+    } else {
+        t = bid % a.T;
+        s = bid / a.T;
+    }
+    if (s >= a.S) return;
+    const int64_t b_begin = (int64_t)s * a.kcb;
+    const int64_t b_end = min(a.nkb, b_begin + a.kcb);
+    gram_item<FMT>(a.blocks, a.cap_bytes, a.first, a.count, tri_tile_s(t, a.nt), b_begin, b_end,
+                   a.partials + ((int64_t)t * a.S + s) * (kTileS * kTileS), lds);
+}
+
+// ------------------------------------------------------------ grouped plan
+// Several pattern groups (each all-pairs within itself) in one launch over
+// their concatenated triangular tile lists.  Group g owns the virtual item
+// ids [v_begin, v_begin + T*S8), with S8 = S rounded up to 8 and v_begin a
+// multiple of 8; id v maps to tile ((v - v_begin) >> 3) % T and slab
+// ((v - v_begin) >> 3) / T * 8 + (v & 7).  Every slab has the same number of
+// K blocks in every group, so all items cost the same; the dispatcher hands
+// workgroups to CUs in id order as they free up, which is the work queue.
+// v & 7 is the XCD, so the T tiles of a slab share that XCD's L2.
+// (A persistent form that loops over items per workgroup needs ~836 B/lane
+// of scratch at this kernel's 256-VGPR budget, so each workgroup runs one.)
+struct GroupDev {
+    const uint8_t *blocks;
+    float *partials;
+    int64_t cap_bytes, nkb, kcb, v_begin;
+    int first, count, nt, T, S, pad;
+};
+
+template <int FMT>
+__global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const GroupDev *__restrict__ groups,
+                                                                    int ngroups) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
+    const int64_t v = blockIdx.x;
+    int gi = 0;
+    while (gi + 1 < ngroups && v >= groups[gi + 1].v_begin) ++gi;
+    const GroupDev &g = groups[gi];
+    const int64_t vl = v - g.v_begin;
+    const int local = (int)(vl >> 3);
+    const int t = local % g.T;
+    const int s = (local / g.T) * 8 + (int)(vl & 7);
+    if (s >= g.S) return;  // padding item
+    const int64_t b_begin = (int64_t)s * g.kcb;
+    const int64_t b_end = min(g.nkb, b_begin + g.kcb);
+    gram_item<FMT>(g.blocks, g.cap_bytes, g.first, g.count, tri_tile_s(t, g.nt), b_begin, b_end,
+                   g.partials + ((int64_t)t * g.S + s) * (kTileS * kTileS), lds);
 }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
@@ -443,6 +489,245 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     return PRNU_OK;
 }
 
+// ------------------------------------------------------------ grouped plan
+struct GramPlan {
+    Context *c = nullptr;
+    int format = PRNU_PREC_SPLIT_BF16;
+    std::vector<prnu_gram_group> groups;
+    std::vector<GroupDev> gd;        // host copy of the device descriptors
+    GroupDev *d_groups = nullptr;
+    float *partials = nullptr;
+    size_t partial_bytes = 0;
+    int64_t kcb = 0;                 // K blocks per slab
+    int64_t v_total = 0, items = 0;  // workgroups launched, work items (padding excluded)
+    int cus = 0;
+    double flop = 0.0;               // one product over every group's triangle (incl. diagonal)
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+constexpr size_t kPlanPartialCap = size_t(8) << 30;
+
+void plan_drop_graph(GramPlan *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+// Work list for slabs of `kcb` blocks: descriptors, partial-tile buffer.
+int plan_layout(GramPlan *p, int64_t kcb) {
+    plan_drop_graph(p);
+    const int64_t tile_elems = (int64_t)kTileS * kTileS;
+    std::vector<GroupDev> gd(p->groups.size());
+    int64_t v = 0, items = 0, part = 0;
+    for (size_t i = 0; i < p->groups.size(); ++i) {
+        const prnu_gram_group &g = p->groups[i];
+        const Store *s = reinterpret_cast<const Store *>(g.store);
+        GroupDev &d = gd[i];
+        d.blocks = s->blocks;
+        d.cap_bytes = (int64_t)s->cap_pad * kBlkBytes;
+        d.nkb = s->nkb;
+        d.kcb = std::min<int64_t>(kcb, s->nkb);
+        d.first = g.first_slot;
+        d.count = g.count;
+        d.nt = (g.count + kTileS - 1) / kTileS;
+        d.T = d.nt * (d.nt + 1) / 2;
+        d.S = (int)((s->nkb + d.kcb - 1) / d.kcb);
+        d.pad = 0;
+        d.v_begin = v;
+        d.partials = reinterpret_cast<float *>((uintptr_t)part);  // offset for now
+        v += (int64_t)d.T * ((d.S + 7) / 8 * 8);
+        items += (int64_t)d.T * d.S;
+        part += tile_elems * sizeof(float) * d.T * d.S;
+    }
+    PRNU_REQUIRE(v < (int64_t(1) << 31), "gram_plan: %lld work items; use longer slabs", (long long)v);
+    PRNU_REQUIRE((size_t)part <= kPlanPartialCap,
+                 "gram_plan: %lld bytes of partial tiles exceed the %zu-byte limit; use longer slabs",
+                 (long long)part, kPlanPartialCap);
+    if ((size_t)part > p->partial_bytes) {
+        if (p->partials) PRNU_CHECK_HIP(hipFree(p->partials));
+        p->partials = nullptr;
+        p->partial_bytes = 0;
+        hipError_t e = hipMalloc(&p->partials, (size_t)part);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("gram_plan: %lld bytes of partial tiles: %s", (long long)part, hipGetErrorString(e));
+            return PRNU_ENOMEM;
+        }
+        p->partial_bytes = (size_t)part;
+    }
+    for (GroupDev &d : gd)
+        d.partials = reinterpret_cast<float *>(reinterpret_cast<uint8_t *>(p->partials) + (uintptr_t)d.partials);
+    if (!p->d_groups) PRNU_CHECK_HIP(hipMalloc(&p->d_groups, sizeof(GroupDev) * kMaxPlanGroups));
+    PRNU_CHECK_HIP(hipMemcpy(p->d_groups, gd.data(), sizeof(GroupDev) * gd.size(), hipMemcpyHostToDevice));
+    p->gd = gd;
+    p->kcb = kcb;
+    p->v_total = v;
+    p->items = items;
+    return PRNU_OK;
+}
+
+// Default slab: about 8 workgroups per CU over all items.
+int64_t plan_default_kcb(const GramPlan *p) {
+    double work = 0.0;
+    for (const prnu_gram_group &g : p->groups) {
+        const Store *s = reinterpret_cast<const Store *>(g.store);
+        const int nt = (g.count + kTileS - 1) / kTileS;
+        work += (double)nt * (nt + 1) / 2 * (double)s->nkb;
+    }
+    int64_t kcb = (int64_t)(work / (8.0 * p->cus));
+    return std::max<int64_t>(16, kcb);
+}
+
+int plan_launch(GramPlan *p, hipStream_t st) {
+    const int ng = (int)p->gd.size();
+    if (p->format == PRNU_PREC_SPLIT_BF16)
+        hipLaunchKernelGGL(store_gram_groups_kernel<PRNU_PREC_SPLIT_BF16>, dim3((unsigned)p->v_total), dim3(kThrS),
+                           0, st, p->d_groups, ng);
+    else
+        hipLaunchKernelGGL(store_gram_groups_kernel<PRNU_PREC_FP32>, dim3((unsigned)p->v_total), dim3(kThrS), 0, st,
+                           p->d_groups, ng);
+    PRNU_CHECK_LAUNCH();
+    for (int i = 0; i < ng; ++i) {
+        const GroupDev &d = p->gd[i];
+        const Store *s = reinterpret_cast<const Store *>(p->groups[i].store);
+        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, d.T), dim3(256), 0, st, d.partials, d.S,
+                           d.nt, d.count, s->sum + d.first, s->sumsq + d.first, p->groups[i].d_ncc);
+        PRNU_CHECK_LAUNCH();
+    }
+    return PRNU_OK;
+}
+
+int plan_capture(GramPlan *p) {
+    if (!p->cap_stream) PRNU_CHECK_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    PRNU_CHECK_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = plan_launch(p, p->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+    if (rc != PRNU_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    PRNU_CHECK_HIP(e);
+    p->graph = g;
+    PRNU_CHECK_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return PRNU_OK;
+}
+
+int plan_run(GramPlan *p, int use_graph, hipStream_t st) {
+    Context *c = p->c;
+    if (use_graph && !p->exec) PRNU_TRY(plan_capture(p));
+    time_begin(c, st);
+    if (use_graph) {
+        PRNU_CHECK_HIP(hipGraphLaunch(p->exec, st));
+    } else {
+        PRNU_TRY(plan_launch(p, st));
+    }
+    time_end(c, st);
+    return PRNU_OK;
+}
+
+void plan_free(GramPlan *p) {
+    if (!p) return;
+    (void)hipDeviceSynchronize();
+    plan_drop_graph(p);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->partials) (void)hipFree(p->partials);
+    if (p->d_groups) (void)hipFree(p->d_groups);
+    delete p;
+}
+
+}  // namespace
+
+int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int64_t slab_values, GramPlan **out) {
+    *out = nullptr;
+    PRNU_REQUIRE(groups && ngroups >= 1 && ngroups <= kMaxPlanGroups, "gram_plan: 1..%d groups, got %d",
+                 kMaxPlanGroups, ngroups);
+    PRNU_REQUIRE(slab_values >= 0 && slab_values % kBlk == 0,
+                 "gram_plan: slab_values must be a non-negative multiple of %d", kBlk);
+    int format = -1;
+    for (int i = 0; i < ngroups; ++i) {
+        const prnu_gram_group &g = groups[i];
+        PRNU_REQUIRE(g.store && g.d_ncc, "gram_plan: group %d has a null store or output", i);
+        const Store *s = reinterpret_cast<const Store *>(g.store);
+        PRNU_REQUIRE(g.count >= 1 && g.first_slot >= 0 && g.first_slot + g.count <= s->capacity,
+                     "gram_plan: group %d slots [%d, %d) outside the capacity %d", i, g.first_slot,
+                     g.first_slot + g.count, s->capacity);
+        if (format < 0) format = s->format;
+        PRNU_REQUIRE(s->format == format, "gram_plan: every group's store must have the same format");
+    }
+    GramPlan *p = new (std::nothrow) GramPlan();
+    if (!p) {
+        set_error("gram_plan: out of host memory");
+        return PRNU_ENOMEM;
+    }
+    p->c = c;
+    p->format = format;
+    p->groups.assign(groups, groups + ngroups);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
+    p->cus = cus;  // one 512-thread, 128 KiB-LDS workgroup per CU at a time
+    for (const prnu_gram_group &g : p->groups) {
+        const Store *s = reinterpret_cast<const Store *>(g.store);
+        const int nt = (g.count + kTileS - 1) / kTileS;
+        p->flop += 2.0 * (double)s->n * ((double)g.count * (g.count + 1) / 2.0);
+        (void)nt;
+    }
+    const int64_t kcb = slab_values ? slab_values / kBlk : plan_default_kcb(p);
+    const int rc = plan_layout(p, kcb);
+    if (rc != PRNU_OK) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return PRNU_OK;
+}
+
+// Times every candidate slab length (eager launches, best of `reps`) and keeps
+// the fastest.
+int gram_plan_autotune(GramPlan *p, hipStream_t st, int64_t *best_values, float *best_ms) {
+    const int64_t base = plan_default_kcb(p);
+    const int64_t cands[] = {base / 4, base / 2, base, base * 2, base * 4, 256, 512, 768, 1024};
+    hipEvent_t e0, e1;
+    PRNU_CHECK_HIP(hipEventCreate(&e0));
+    PRNU_CHECK_HIP(hipEventCreate(&e1));
+    float best = 3.4e38f;
+    int64_t best_kcb = p->kcb;
+    int rc = PRNU_OK;
+    for (int64_t kcb : cands) {
+        if (kcb < 16) continue;
+        if (plan_layout(p, kcb) != PRNU_OK) continue;  // e.g. partials over the limit
+        rc = plan_launch(p, st);  // warm-up
+        if (rc != PRNU_OK) break;
+        float ms_min = 3.4e38f;
+        for (int r = 0; r < 3 && rc == PRNU_OK; ++r) {
+            (void)hipEventRecord(e0, st);
+            rc = plan_launch(p, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = PRNU_EDEVICE;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms_min = std::min(ms_min, ms);
+        }
+        if (rc != PRNU_OK) break;
+        if (ms_min < best) {
+            best = ms_min;
+            best_kcb = kcb;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != PRNU_OK) return rc;
+    PRNU_TRY(plan_layout(p, best_kcb));
+    if (best_values) *best_values = best_kcb * kBlk;
+    if (best_ms) *best_ms = best;
+    return PRNU_OK;
+}
+
 }  // namespace prnu
 
 using namespace prnu;
@@ -498,6 +783,48 @@ int prnu_store_gram_ncc(prnu_ctx *ctx, prnu_store *store, int first_slot, int co
     hipStream_t st = pick_stream(c, stream);
     PRNU_TRY(store_gram_ncc(c, s, first_slot, count, d_ncc, st));
     return finish(c, stream);
+}
+
+int prnu_gram_plan_create(prnu_ctx *ctx, const prnu_gram_group *groups, int ngroups, int64_t slab_values,
+                          prnu_gram_plan **out) {
+    PRNU_REQUIRE(ctx && out, "gram_plan_create: null pointer");
+    GramPlan *p = nullptr;
+    PRNU_TRY(gram_plan_create(reinterpret_cast<Context *>(ctx), groups, ngroups, slab_values, &p));
+    *out = reinterpret_cast<prnu_gram_plan *>(p);
+    return PRNU_OK;
+}
+
+int prnu_gram_plan_destroy(prnu_gram_plan *plan) {
+    plan_free(reinterpret_cast<GramPlan *>(plan));
+    return PRNU_OK;
+}
+
+int prnu_gram_plan_run(prnu_gram_plan *plan, int use_graph, prnu_stream stream) {
+    PRNU_REQUIRE(plan, "gram_plan_run: null plan");
+    GramPlan *p = reinterpret_cast<GramPlan *>(plan);
+    hipStream_t st = pick_stream(p->c, stream);
+    PRNU_TRY(plan_run(p, use_graph, st));
+    return finish(p->c, stream);
+}
+
+int prnu_gram_plan_autotune(prnu_gram_plan *plan, prnu_stream stream, int64_t *best_slab_values, float *best_ms) {
+    PRNU_REQUIRE(plan, "gram_plan_autotune: null plan");
+    GramPlan *p = reinterpret_cast<GramPlan *>(plan);
+    hipStream_t st = pick_stream(p->c, stream);
+    PRNU_TRY(gram_plan_autotune(p, st, best_slab_values, best_ms));
+    return finish(p->c, stream);
+}
+
+int prnu_gram_plan_info(const prnu_gram_plan *plan, int64_t *work_items, int *workgroups, int64_t *slab_values,
+                        size_t *partial_bytes, double *flop) {
+    PRNU_REQUIRE(plan, "gram_plan_info: null plan");
+    const GramPlan *p = reinterpret_cast<const GramPlan *>(plan);
+    if (work_items) *work_items = p->items;
+    if (workgroups) *workgroups = (int)p->v_total;
+    if (slab_values) *slab_values = p->kcb * kBlk;
+    if (partial_bytes) *partial_bytes = p->partial_bytes;
+    if (flop) *flop = p->flop;
+    return PRNU_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,116 @@
+"""GPU tests of the grouped all-pairs plan (several image sizes in one launch).
+
+* every group's output matches the FP64 oracle NCC matrix of that group
+  (|dNCC| <= 1e-5, the bar of the batched path) and the single-group store
+  Gram of the same slots;
+* groups with ragged counts (not multiples of the 256 tile), groups sharing a
+  store through slot ranges, FP32 and split-BF16 stores;
+* HIP-graph replays are bitwise equal to plain launches, replay after replay,
+  with unrelated eager work and synchronisations between the replays;
+* the autotuner keeps a slab length whose results stay within the bar;
+* invalid plans are rejected with ValueError.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import prnu_oracle as O  # noqa: E402
+from rocket_amd import csi  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+NCC_TOL = 1e-5
+
+
+def patterns(count, n, cams, seed, gain=0.2):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    k = torch.randn((cams, n), generator=g, device="cuda")
+    p = torch.randn((count, n), generator=g, device="cuda")
+    p += gain * k[torch.arange(count, device="cuda") % cams]
+    p -= p.mean(dim=1, keepdim=True)
+    return p.float().contiguous()
+
+
+# (n, count, store capacity, first slot): ragged counts, different K
+GROUPS = [(96 * 128, 300, 310, 4), (64 * 80, 37, 37, 0), (150 * 200, 513, 520, 7)]
+
+
+@pytest.fixture(scope="module", params=["split_bf16", "fp32"])
+def groups(request):
+    fmt = request.param
+    out = []
+    for i, (n, count, cap, first) in enumerate(GROUPS):
+        p = patterns(count, n, 4, seed=300 + i)
+        st = csi.PatternStore(n, cap, fmt)
+        st.put(p, first_slot=first)
+        out.append((st, first, count, p))
+    gram = csi.GramNCC(8, 8)   # workspace/timing context; sizes come from the stores
+    yield fmt, out, gram
+    for st, *_ in out:
+        st.close()
+    gram.close()
+
+
+def test_plan_matches_oracle_and_single_group(groups):
+    fmt, gs, gram = groups
+    # a fourth group: a slot range inside the first store
+    st0, first0, _, p0 = gs[0]
+    spec = [(st, first, count) for st, first, count, _ in gs] + [(st0, first0 + 100, 190)]
+    plan = csi.GramPlan(gram, spec)
+    info = plan.info()
+    assert info["work_items"] > 0 and info["slab_values"] % 32 == 0
+    outs = [o.cpu().numpy() for o in plan.run(use_graph=False)]
+    hosts = [p.cpu().numpy() for *_, p in gs] + [p0[100:290].cpu().numpy()]
+    for g, (got, host) in enumerate(zip(outs, hosts)):
+        want = O.ncc_matrix(host)
+        err = np.abs(got - want).max()
+        assert err <= NCC_TOL, f"{fmt} group {g}: max |dNCC| = {err:.3e}"
+        assert np.array_equal(got, got.T)
+    for g, (st, first, count, _) in enumerate(gs):
+        single = st.gram(gram, count, first).cpu().numpy()
+        assert np.abs(single - outs[g]).max() <= NCC_TOL       # only the slab summation order differs
+    plan.close()
+
+
+def test_plan_graph_replays_bitwise(groups):
+    fmt, gs, gram = groups
+    plan = csi.GramPlan(gram, [(st, first, count) for st, first, count, _ in gs])
+    eager = [o.clone() for o in plan.run(use_graph=False)]
+    scratch = torch.zeros(1 << 20, device="cuda")
+    for rep in range(4):
+        for o in plan.outputs:
+            o.fill_(float("nan"))
+        got = plan.run(use_graph=True)
+        for g, (a, b) in enumerate(zip(got, eager)):
+            assert torch.equal(a, b), f"replay {rep} group {g}"
+        scratch.add_(1.0)                                   # unrelated eager work between replays
+        torch.cuda.synchronize()
+    plan.close()
+
+
+def test_plan_autotune(groups):
+    fmt, gs, gram = groups
+    plan = csi.GramPlan(gram, [(st, first, count) for st, first, count, _ in gs])
+    slab, ms = plan.autotune()
+    assert slab > 0 and slab % 32 == 0 and ms > 0
+    assert plan.info()["slab_values"] == slab
+    outs = [o.cpu().numpy() for o in plan.run(use_graph=True)]
+    for g, (st, first, count, p) in enumerate(gs):
+        assert np.abs(outs[g] - O.ncc_matrix(p.cpu().numpy())).max() <= NCC_TOL
+    plan.close()
+
+
+def test_plan_rejects_invalid(groups):
+    fmt, gs, gram = groups
+    st, first, count, _ = gs[0]
+    with pytest.raises(ValueError):                          # beyond the store's capacity
+        csi.GramPlan(gram, [(st, first, st.capacity)])
+    with pytest.raises(ValueError):                          # slab not a multiple of 32
+        csi.GramPlan(gram, [(st, first, count)], slab_values=100)
+    other = csi.PatternStore(64, 8, "fp32" if fmt == "split_bf16" else "split_bf16")
+    with pytest.raises(ValueError):                          # mixed store formats
+        csi.GramPlan(gram, [(st, first, count), (other, 0, 8)])
+    other.close()
+    with pytest.raises(ValueError):
+        csi.GramPlan(gram, [])
