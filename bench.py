@@ -222,7 +222,7 @@ def groups_bench(torch, steps=5):
         cams = {1024: 16, 1536: 16, 3000: 8}[h]
         k = torch.randn((cams, n), generator=g, device="cuda")
         st = csi.PatternStore(n, count, "split_bf16")
-        chunk = max(8, int((1 << 30) // (4 * n)))
+        chunk = max(24, int((1 << 30) // (4 * n)))
         for b in range(0, count, chunk):
             e = min(count, b + chunk)
             blk = torch.randn((e - b, n), generator=g, device="cuda")

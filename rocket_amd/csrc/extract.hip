@@ -580,14 +580,18 @@ int run_preprocess_batch(Context *c, int fmt, const void *in, size_t in_bytes, i
     const int bpp = input_bytes_per_pixel(fmt);
     PRNU_REQUIRE(bpp > 0, "preprocess: unknown input format %d", fmt);
     PRNU_REQUIRE(count >= 1, "preprocess: count must be >= 1");
+    PRNU_REQUIRE(fmt != PRNU_INPUT_FLOAT32 || reinterpret_cast<uintptr_t>(in) % 4 == 0,
+                 "preprocess: FP32 input must be 4-byte aligned");
     const size_t img_bytes = (size_t)bpp * c->pixels();
     const size_t need = c->mode == PRNU_MODE_PCE ? 8 * (size_t)c->spectrum_elems() : 4 * (size_t)c->pixels();
     if (count > 1) {
         PRNU_REQUIRE(in_stride >= (int64_t)img_bytes && out_stride >= (int64_t)need,
                      "preprocess: strides (%lld, %lld) smaller than an image (%zu) or a pattern (%zu)",
                      (long long)in_stride, (long long)out_stride, img_bytes, need);
-        PRNU_REQUIRE(in_stride % bpp == 0 && out_stride % (c->mode == PRNU_MODE_PCE ? 8 : 4) == 0,
-                     "preprocess: strides must be whole pixels");
+        // 8-bit images may start at any byte; FP32 images and patterns need their alignment
+        PRNU_REQUIRE((fmt != PRNU_INPUT_FLOAT32 || in_stride % 4 == 0) &&
+                         out_stride % (c->mode == PRNU_MODE_PCE ? 8 : 4) == 0,
+                     "preprocess: FP32 strides must be multiples of the element size");
     } else {
         in_stride = (int64_t)img_bytes;
         out_stride = (int64_t)need;
