@@ -148,21 +148,32 @@ def synth_query_images(torch, fingerprints, owners, seed):
 def serving_bench(torch, duration: float):
     """Poisson bursts (uniform 1..64 queries, mean rate 200 q/s) against a
     20,000-fingerprint FP8 store at 1024x1024; p50/p99 latency from submit
-    to result, queries/s over the run."""
-    from rocket_amd import serving
+    to result, queries/s over the run.  Slots 0..63 hold enrolled cameras
+    (fingerprint = mean extracted pattern of 4 images of the camera), the
+    rest random fingerprints; half of the queries come from enrolled cameras."""
+    from rocket_amd import _lib, csi, serving
     n = SERVE_H * SERVE_W
     t0 = time.time()
     store = serving.FingerprintStore(SERVE_H, SERVE_W, SERVE_STORE)
     g = torch.Generator(device="cuda").manual_seed(4242)
-    known = None
+    sensors = torch.randn((64, n), generator=g, device="cuda")       # PRNU K of the enrolled cameras
+    ref = csi.PRNUFilter(SERVE_H, SERVE_W)
+    pats = torch.empty((256, n), device="cuda")
+    imgs = synth_query_images(torch, sensors, [c for c in range(64) for _ in range(4)], seed=99)
+    ref.apply_gpu_batch(imgs.view(256, n), pats, _lib.INPUT_GRAY8)
+    enrolled = pats.view(64, 4, n).mean(dim=1)
+    enrolled -= enrolled.mean(dim=1, keepdim=True)
+    store.put(enrolled.contiguous(), first_slot=0)
+    del pats, imgs, enrolled
+    ref.cleanup()
     chunk = 500
-    for b in range(0, SERVE_STORE, chunk):
-        f = torch.randn((chunk, n), generator=g, device="cuda")
+    for b in range(64, SERVE_STORE, chunk):
+        e = min(SERVE_STORE, b + chunk)
+        f = torch.randn((e - b, n), generator=g, device="cuda")
         f -= f.mean(dim=1, keepdim=True)
         store.put(f, first_slot=b)
-        if known is None:
-            known = f[:64].clone()          # cameras the queries may come from
         del f
+    known = sensors
     torch.cuda.synchronize()
     log(f"serving: store of {SERVE_STORE} fingerprints ({store.info()['bytes'] / 1e9:.0f} GB) in {time.time() - t0:.1f} s")
     rng = np.random.default_rng(20261202)

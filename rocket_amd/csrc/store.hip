@@ -126,21 +126,39 @@ __global__ __launch_bounds__(256) void store_put_kernel(const float *const *ptrs
     }
 }
 
-__global__ void store_stats_final_kernel(const StatPartS *part, int chunks, int count, int first, double *sum,
-                                         double *sumsq, float *absmax) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
+// One 256-thread workgroup per pattern: strided partial sums, then a fixed
+// tree over the block (deterministic).
+__global__ __launch_bounds__(256) void store_stats_final_kernel(const StatPartS *part, int chunks, int count,
+                                                                int first, double *sum, double *sumsq,
+                                                                float *absmax) {
+    __shared__ double sh_s[256], sh_ss[256];
+    __shared__ float sh_am[256];
+    const int p = blockIdx.x, tid = threadIdx.x;
     double s = 0.0, ss = 0.0;
     float am = 0.0f;
-    for (int c = 0; c < chunks; ++c) {
+    for (int c = tid; c < chunks; c += 256) {
         const StatPartS q = part[(int64_t)p * chunks + c];
         s += q.s;
         ss += q.ss;
         am = fmaxf(am, q.am);
     }
-    sum[first + p] = s;
-    sumsq[first + p] = ss;
-    absmax[first + p] = am;
+    sh_s[tid] = s;
+    sh_ss[tid] = ss;
+    sh_am[tid] = am;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sh_s[tid] += sh_s[tid + w];
+            sh_ss[tid] += sh_ss[tid + w];
+            sh_am[tid] = fmaxf(sh_am[tid], sh_am[tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        sum[first + p] = sh_s[0];
+        sumsq[first + p] = sh_ss[0];
+        absmax[first + p] = sh_am[0];
+    }
 }
 
 // -------------------------------------------------------------------- Gram
@@ -151,7 +169,7 @@ struct StoreGramArgs {
     int64_t nkb;
     int64_t kcb;         // blocks per slab
     int first, count;
-    int nt, T, S;
+    int nt, T, S;        // T: work items per slab
     int xcd_remap;
 };
 
@@ -169,20 +187,86 @@ __device__ __forceinline__ int s_off(int row, int chunk) { return row * kBlkByte
 
 constexpr int kBufS = kRowsS * kBlkBytes;  // 64 KiB per LDS stage
 
-// One work item: the 256 x 256 tile `tile` of the slots [first, first+count)
-// over the K blocks [b_begin, b_end), into the FP32 partial tile `out`.
-// Ends with a barrier, so a caller may start the next item on the same LDS.
+// Work items.  Per K slab a group has nt(nt-1)/2 off-diagonal 256 x 256
+// tiles and ceil(6 nt / 8) diagonal packs.  A diagonal tile only needs the
+// 128 x 64 wave blocks that meet its upper triangle: 6 of 8, namely
+// (wr, wc) = (0,0) (0,1) (0,2) (0,3) (1,2) (1,3).  The 6*nt such blocks of a
+// group are numbered e = 6*d + l; pack p gives block 8p + w to wave w.  Eight
+// consecutive blocks span at most two diagonal tiles d0 = 8p/6 and d0 + 1,
+// whose rows fill the two LDS halves, so a pack costs one tile and the
+// diagonal costs 3/4 of the tiles it would otherwise take.
+struct ItemWaves {
+    int rows_x, rows_y;   // tiles whose 256 rows go to LDS rows [0,256) / [256,512); rows_y < 0: none
+    int a_off, b_off;     // this wave's first A row / first B row in LDS
+    int out_tile;         // triangle index of the tile this wave writes, -1: idle wave
+    int out_off;          // offset of the wave's 128 x 64 block inside that tile
+};
+
+__device__ __forceinline__ int tri_index(int I, int J, int nt) { return I * nt - I * (I - 1) / 2 + (J - I); }
+
+// Item u of a slab (u < nt(nt-1)/2: off-diagonal tile, else a diagonal pack).
+__device__ __forceinline__ ItemWaves item_waves(int u, int nt, int wave) {
+    ItemWaves r;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int toff = nt * (nt - 1) / 2;
+    if (u < toff) {
+        int I = 0, t = u;
+        while (t >= nt - 1 - I) {
+            t -= nt - 1 - I;
+            ++I;
+        }
+        const int J = I + 1 + t;
+        r.rows_x = I;
+        r.rows_y = J;
+        r.a_off = wr * 128;
+        r.b_off = kTileS + wc * 64;
+        r.out_tile = tri_index(I, J, nt);
+        r.out_off = wr * 128 * kTileS + wc * 64;
+        return r;
+    }
+    const int p = u - toff;
+    const int d0 = (8 * p) / 6;
+    const int dl = min(nt - 1, (8 * p + 7) / 6);
+    r.rows_x = d0;
+    r.rows_y = dl > d0 ? d0 + 1 : -1;
+    const int e = 8 * p + wave;
+    if (e >= 6 * nt) {
+        r.a_off = r.b_off = 0;
+        r.out_tile = -1;
+        r.out_off = 0;
+        return r;
+    }
+    const int d = e / 6, l = e - 6 * (e / 6);
+    const int bwr = l < 4 ? 0 : 1;
+    const int bwc = l < 4 ? l : l - 2;
+    const int base = (d - d0) * kTileS;
+    r.a_off = base + bwr * 128;
+    r.b_off = base + bwc * 64;
+    r.out_tile = tri_index(d, d, nt);
+    r.out_off = bwr * 128 * kTileS + bwc * 64;
+    return r;
+}
+
+__device__ __forceinline__ int items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
+
+// One work item over the K blocks [b_begin, b_end) of the slots
+// [first, first+count): LDS rows [0,256) hold the rows of tile w.rows_x,
+// [256,512) those of w.rows_y; each wave multiplies its 128 A rows by its 64
+// B rows and writes its FP32 128 x 64 partial block to `out` (row stride
+// 256; null for an idle wave).  Ends with a barrier, so a caller may start
+// the next item on the same LDS.
 template <int FMT>
 __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, int64_t cap_bytes, int first, int count,
-                                          int2 tile, int64_t b_begin, int64_t b_end, float *__restrict__ out,
+                                          const ItemWaves &w, int64_t b_begin, int64_t b_end, float *__restrict__ out,
                                           uint8_t *lds) {
     constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
-    const bool diag = tile.x == tile.y;
-    const int nrows = diag ? kTileS : kRowsS;
-    const int bbase = diag ? 0 : kTileS;
+    const bool half = w.rows_y < 0;
+    const int nrows = half ? kTileS : kRowsS;
+    const bool active = out != nullptr;
+    const int2 tile = make_int2(w.rows_x, half ? w.rows_x : w.rows_y);
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wrow0 = (wave >> 2) * 128, wcol0 = (wave & 3) * 64;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wrow0 = w.a_off, wcol0 = w.b_off;
 
     // thread -> (row = (tid >> 3) + 64*it, chunk = tid & 7); rows past count
     // read a clamped slot and are zeroed on the LDS store
@@ -211,7 +295,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
         const uint8_t *col = blocks + b * cap_bytes + (chunk << 4);
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
-            if (it < 4 || !diag) {
+            if (it < 4 || !half) {
                 typedef const __attribute__((address_space(1))) unsigned int __attribute__((ext_vector_type(4))) *gu4;
                 const auto v = *(gu4)(col + row_off(it));
                 st[it] = make_uint4(v[0], v[1], v[2], v[3]);
@@ -236,7 +320,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
             bf16x8 bh[4], bl[4];
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
-                const int r = bbase + wcol0 + cb * 16 + frow;
+                const int r = wcol0 + cb * 16 + frow;
                 bh[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
                 bl[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
             }
@@ -259,7 +343,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
                 float4 bf[4];
 #pragma unroll
                 for (int cb = 0; cb < 4; ++cb)
-                    bf[cb] = *reinterpret_cast<const float4 *>(buf + s_off(bbase + wcol0 + cb * 16 + frow, ch));
+                    bf[cb] = *reinterpret_cast<const float4 *>(buf + s_off(wcol0 + cb * 16 + frow, ch));
 #pragma unroll
                 for (int rb = 0; rb < 8; ++rb) {
                     const float4 af = *reinterpret_cast<const float4 *>(buf + s_off(wrow0 + rb * 16 + frow, ch));
@@ -282,11 +366,12 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     for (int64_t q = 0; q < nst; ++q) {
         const bool next = q + 1 < nst;
         if (next) load(b_begin + q + 1);
-        compute(lds + (q & 1) * kBufS);
+        if (active) compute(lds + (q & 1) * kBufS);
         if (next) store(lds + ((q + 1) & 1) * kBufS);
         __syncthreads();
     }
 
+    if (!active) return;
     const int ocol = lane & 15, orow = (lane >> 4) * 4;
 #pragma unroll
     for (int rb = 0; rb < 8; ++rb)
@@ -294,40 +379,57 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                out[(wrow0 + rb * 16 + orow + r) * kTileS + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
+                out[(rb * 16 + orow + r) * kTileS + cb * 16 + ocol] = acc[rb][cb][r];
 }
 
-// One launch = one work item per workgroup.  With xcd_remap, the T tiles of
+// Run item u of slab s: partial tile t of slab s lives at partials + (t*S + s) * 256*256.
+template <int FMT>
+__device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int nt,
+                                         int S, int64_t nkb, int64_t kcb, float *partials, int u, int s,
+                                         uint8_t *lds) {
+    // wave-uniform: keep it in scalar registers
+    ItemWaves w = item_waves(u, nt, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+    w.rows_x = __builtin_amdgcn_readfirstlane(w.rows_x);
+    w.rows_y = __builtin_amdgcn_readfirstlane(w.rows_y);
+    w.a_off = __builtin_amdgcn_readfirstlane(w.a_off >> 6) << 6;  // multiples of 64 rows: the swizzle
+    w.b_off = __builtin_amdgcn_readfirstlane(w.b_off >> 6) << 6;  // term (row & 7) stays per lane
+    w.out_tile = __builtin_amdgcn_readfirstlane(w.out_tile);
+    w.out_off = __builtin_amdgcn_readfirstlane(w.out_off);
+    const int64_t b_begin = (int64_t)s * kcb;
+    const int64_t b_end = min(nkb, b_begin + kcb);
+    float *out = w.out_tile < 0 ? nullptr
+                                : partials + ((int64_t)w.out_tile * S + s) * (kTileS * kTileS) + w.out_off;
+    gram_item<FMT>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
+}
+
+// One launch = one work item per workgroup.  With xcd_remap, the items of
 // a slab run on one XCD (workgroup ids are dealt to the 8 XCDs round-robin).
 template <int FMT>
 __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
     const int bid = blockIdx.x;
-    int t, s;
+    int u, s;
     if (a.xcd_remap) {
         const int local = bid >> 3;
-        t = local % a.T;
+        u = local % a.T;
         s = (local / a.T) * 8 + (bid & 7);
     } else {
-        t = bid % a.T;
+        u = bid % a.T;
         s = bid / a.T;
     }
     if (s >= a.S) return;
-    const int64_t b_begin = (int64_t)s * a.kcb;
-    const int64_t b_end = min(a.nkb, b_begin + a.kcb);
-    gram_item<FMT>(a.blocks, a.cap_bytes, a.first, a.count, tri_tile_s(t, a.nt), b_begin, b_end,
-                   a.partials + ((int64_t)t * a.S + s) * (kTileS * kTileS), lds);
+    run_item<FMT>(a.blocks, a.cap_bytes, a.first, a.count, a.nt, a.S, a.nkb, a.kcb, a.partials, u, s, lds);
 }
 
 // ------------------------------------------------------------ grouped plan
 // Several pattern groups (each all-pairs within itself) in one launch over
-// their concatenated triangular tile lists.  Group g owns the virtual item
-// ids [v_begin, v_begin + T*S8), with S8 = S rounded up to 8 and v_begin a
-// multiple of 8; id v maps to tile ((v - v_begin) >> 3) % T and slab
+// their concatenated work-item lists.  Group g (T items per slab) owns the
+// virtual ids [v_begin, v_begin + T*S8), with S8 = S rounded up to 8 and
+// v_begin a multiple of 8; id v maps to item ((v - v_begin) >> 3) % T and slab
 // ((v - v_begin) >> 3) / T * 8 + (v & 7).  Every slab has the same number of
 // K blocks in every group, so all items cost the same; the dispatcher hands
 // workgroups to CUs in id order as they free up, which is the work queue.
-// v & 7 is the XCD, so the T tiles of a slab share that XCD's L2.
+// v & 7 is the XCD, so the T items of a slab share that XCD's L2.
 // (A persistent form that loops over items per workgroup needs ~836 B/lane
 // of scratch at this kernel's 256-VGPR budget, so each workgroup runs one.)
 struct GroupDev {
@@ -347,13 +449,10 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
     const GroupDev &g = groups[gi];
     const int64_t vl = v - g.v_begin;
     const int local = (int)(vl >> 3);
-    const int t = local % g.T;
+    const int u = local % g.T;
     const int s = (local / g.T) * 8 + (int)(vl & 7);
     if (s >= g.S) return;  // padding item
-    const int64_t b_begin = (int64_t)s * g.kcb;
-    const int64_t b_end = min(g.nkb, b_begin + g.kcb);
-    gram_item<FMT>(g.blocks, g.cap_bytes, g.first, g.count, tri_tile_s(t, g.nt), b_begin, b_end,
-                   g.partials + ((int64_t)t * g.S + s) * (kTileS * kTileS), lds);
+    run_item<FMT>(g.blocks, g.cap_bytes, g.first, g.count, g.nt, g.S, g.nkb, g.kcb, g.partials, u, s, lds);
 }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
@@ -379,6 +478,9 @@ __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restri
 }
 
 }  // namespace
+
+// Work items per K slab: off-diagonal tiles plus diagonal packs (item_waves).
+int host_items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
 
 int store_create(int64_t n, int capacity, int format, Store **out) {
     *out = nullptr;
@@ -443,8 +545,8 @@ int store_put(Store *s, const float *const *d_ptrs, int count, int first, hipStr
         hipLaunchKernelGGL(store_put_kernel<PRNU_PREC_FP32>, grid, dim3(256), 0, st, d_ptrs, count, s->n, s->nkb,
                            s->blocks, s->cap_pad, first, part);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(store_stats_final_kernel, dim3((count + 255) / 256), dim3(256), 0, st, part, (int)gx, count,
-                       first, s->sum, s->sumsq, s->absmax);
+    hipLaunchKernelGGL(store_stats_final_kernel, dim3(count), dim3(256), 0, st, part, (int)gx, count, first, s->sum,
+                       s->sumsq, s->absmax);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -459,7 +561,8 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     a.first = first;
     a.count = count;
     a.nt = (count + kTileS - 1) / kTileS;
-    a.T = a.nt * (a.nt + 1) / 2;
+    a.T = host_items_per_slab(a.nt);
+    const int tiles = a.nt * (a.nt + 1) / 2;  // partial tiles per slab (triangle incl. diagonal)
     const prnu_gram_config cfg = c->gram_cfg[s->format];
     int64_t S = cfg.slabs;
     if (S <= 0) {
@@ -468,11 +571,11 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     }
     S = std::min<int64_t>(S, s->nkb);
     const int64_t tile_bytes = (int64_t)kTileS * kTileS * sizeof(float);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * a.T)));
+    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * tiles)));
     a.kcb = (s->nkb + S - 1) / S;
     a.S = (int)((s->nkb + a.kcb - 1) / a.kcb);
     a.xcd_remap = cfg.xcd_remap;
-    PRNU_TRY(c->ws_gram.ensure((size_t)tile_bytes * a.T * a.S));
+    PRNU_TRY(c->ws_gram.ensure((size_t)tile_bytes * tiles * a.S));
     a.partials = static_cast<float *>(c->ws_gram.ptr);
     const int64_t S8 = a.xcd_remap ? (a.S + 7) / 8 * 8 : a.S;
     const dim3 grid((unsigned)(a.T * S8));
@@ -483,8 +586,8 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
         hipLaunchKernelGGL(store_gram_kernel<PRNU_PREC_FP32>, grid, dim3(kThrS), 0, st, a);
     PRNU_CHECK_LAUNCH();
     time_end(c, st);
-    hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, a.T), dim3(256), 0, st, a.partials, a.S, a.nt,
-                       count, s->sum + first, s->sumsq + first, d_ncc);
+    hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials, a.S,
+                       a.nt, count, s->sum + first, s->sumsq + first, d_ncc);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -535,14 +638,14 @@ int plan_layout(GramPlan *p, int64_t kcb) {
         d.first = g.first_slot;
         d.count = g.count;
         d.nt = (g.count + kTileS - 1) / kTileS;
-        d.T = d.nt * (d.nt + 1) / 2;
+        d.T = host_items_per_slab(d.nt);
         d.S = (int)((s->nkb + d.kcb - 1) / d.kcb);
         d.pad = 0;
         d.v_begin = v;
         d.partials = reinterpret_cast<float *>((uintptr_t)part);  // offset for now
         v += (int64_t)d.T * ((d.S + 7) / 8 * 8);
         items += (int64_t)d.T * d.S;
-        part += tile_elems * sizeof(float) * d.T * d.S;
+        part += tile_elems * sizeof(float) * (d.nt * (d.nt + 1) / 2) * d.S;
     }
     PRNU_REQUIRE(v < (int64_t(1) << 31), "gram_plan: %lld work items; use longer slabs", (long long)v);
     PRNU_REQUIRE((size_t)part <= kPlanPartialCap,
@@ -577,7 +680,7 @@ int64_t plan_default_kcb(const GramPlan *p) {
     for (const prnu_gram_group &g : p->groups) {
         const Store *s = reinterpret_cast<const Store *>(g.store);
         const int nt = (g.count + kTileS - 1) / kTileS;
-        work += (double)nt * (nt + 1) / 2 * (double)s->nkb;
+        work += (double)host_items_per_slab(nt) * (double)s->nkb;
     }
     int64_t kcb = (int64_t)(work / (8.0 * p->cus));
     return std::max<int64_t>(16, kcb);
@@ -595,8 +698,8 @@ int plan_launch(GramPlan *p, hipStream_t st) {
     for (int i = 0; i < ng; ++i) {
         const GroupDev &d = p->gd[i];
         const Store *s = reinterpret_cast<const Store *>(p->groups[i].store);
-        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, d.T), dim3(256), 0, st, d.partials, d.S,
-                           d.nt, d.count, s->sum + d.first, s->sumsq + d.first, p->groups[i].d_ncc);
+        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, d.nt * (d.nt + 1) / 2), dim3(256), 0, st,
+                           d.partials, d.S, d.nt, d.count, s->sum + d.first, s->sumsq + d.first, p->groups[i].d_ncc);
         PRNU_CHECK_LAUNCH();
     }
     return PRNU_OK;

@@ -99,23 +99,32 @@ def test_query_rejects_bad_k(small_store):
 
 def test_server_continuous_batching_matches_alone():
     """Bursts of requests from a client thread while batches are in flight;
-    every answer equals the answer for that image queried alone."""
+    every answer equals the answer for that image queried alone, and queries
+    from enrolled cameras find their camera first.  A camera's fingerprint is
+    the mean extracted pattern of 4 reference images of it (a raw sensor
+    pattern K correlates negatively with extracted patterns: FastNoise is a
+    second-derivative filter)."""
     from rocket_amd import synthetic
-    h, w, cams = 64, 80, 24
-    fp_cpu = np.stack([synthetic.camera_fingerprint(c, h, w) for c in range(cams)]).astype(np.float32)
-    fp = torch.from_numpy(fp_cpu).cuda()
-    store = serving.FingerprintStore(h, w, cams)
+    h, w, cams, enrolled = 64, 80, 32, 24
+    images = synthetic.make_images(40 + 4 * cams, h, w, cams, workers=1)   # camera of image i: i % 32
+    ref = csi.PRNUFilter(h, w)
+    pats = torch.empty((len(images), h * w), device="cuda")
+    ref.apply_gpu_batch(torch.from_numpy(images.reshape(len(images), -1)).cuda(), pats, _lib.INPUT_GRAY8)
+    fp = torch.stack([pats[[i for i in range(40, len(images)) if i % cams == c]].mean(dim=0)
+                      for c in range(enrolled)])
+    fp = (fp - fp.mean(dim=1, keepdim=True)).contiguous()
+    store = serving.FingerprintStore(h, w, enrolled)
     store.put(fp)
-    images = synthetic.make_images(40, h, w, cams + 8, workers=1)  # cameras >= 24 are unknown
+    queries = images[:40]                          # cameras 24..31 are not enrolled
     srv = serving.Server(store, k=5, max_batch=16, slots=2).start()
     reqs = []
 
     def client():
         rng = np.random.default_rng(3)
         i = 0
-        while i < len(images):
+        while i < len(queries):
             burst = int(rng.integers(1, 9))
-            for img in images[i:i + burst]:
+            for img in queries[i:i + burst]:
                 reqs.append(srv.submit(img))
             i += burst
             time.sleep(float(rng.exponential(0.003)))
@@ -131,9 +140,14 @@ def test_server_continuous_batching_matches_alone():
     pat = torch.empty(h * w, device="cuda")
     for i, r in enumerate(reqs):
         prep = csi.PRNUFilter(h, w, _ctx=ctx)
-        prep.apply_gpu(torch.from_numpy(images[i]).cuda(), pat, _lib.INPUT_GRAY8)
+        prep.apply_gpu(torch.from_numpy(queries[i]).cuda(), pat, _lib.INPUT_GRAY8)
         idx1, sc1 = (t.cpu().numpy() for t in store.query(ctx, pat.view(1, -1).clone(), k=5))
         assert np.array_equal(idx1[0], r.idx) and np.array_equal(sc1[0], r.score)
         assert r.latency > 0
+        if i % cams < enrolled:
+            assert r.idx[0] == i % cams and r.score[0] > 0.3, f"query {i}"
+        else:
+            assert r.score[0] < 0.15, f"query {i} of an unenrolled camera"
     ctx.close()
+    ref.cleanup()
     store.close()
