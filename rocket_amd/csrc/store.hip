@@ -15,6 +15,7 @@
 #include "prnu_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 #include <vector>
 
@@ -482,6 +483,40 @@ __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restri
 // Work items per K slab: off-diagonal tiles plus diagonal packs (item_waves).
 int host_items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
 
+int device_cus(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    return cus;
+}
+
+// Blocks per slab for groups of (items per slab, K blocks): all work items
+// cost about kcb block-stages and one CU runs one item at a time, so the
+// launch takes about ceil(items / cus) * kcb stages, plus a per-slab cost
+// (the partial tile each item writes and the reduction reads, ~2 stages).
+// Searches kcb and returns the cheapest whose partial tiles fit `max_bytes`.
+int64_t choose_kcb(const int *items, const int *tiles, const int64_t *nkb, int ngroups, int cus, size_t max_bytes) {
+    int64_t max_nkb = 1;
+    for (int g = 0; g < ngroups; ++g) max_nkb = std::max(max_nkb, nkb[g]);
+    double best = 1e300;
+    int64_t best_kcb = max_nkb;
+    for (int64_t kcb = 16; kcb <= max_nkb; kcb += (kcb < 256 ? 8 : kcb < 2048 ? 32 : 256)) {
+        double total = 0.0, part = 0.0;
+        for (int g = 0; g < ngroups; ++g) {
+            const int64_t S = (nkb[g] + kcb - 1) / kcb;
+            total += (double)items[g] * S;
+            part += (double)tiles[g] * S * kTileS * kTileS * sizeof(float);
+        }
+        if (part > (double)max_bytes) continue;
+        const double rounds = std::ceil(total / cus);
+        const double cost = rounds * (double)(kcb + 2);
+        if (cost < best * (1.0 - 1e-9)) {
+            best = cost;
+            best_kcb = kcb;
+        }
+    }
+    return best_kcb;
+}
+
 int store_create(int64_t n, int capacity, int format, Store **out) {
     *out = nullptr;
     PRNU_REQUIRE(n >= 1 && capacity >= 1, "store_create: n and capacity must be positive");
@@ -564,15 +599,15 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     a.T = host_items_per_slab(a.nt);
     const int tiles = a.nt * (a.nt + 1) / 2;  // partial tiles per slab (triangle incl. diagonal)
     const prnu_gram_config cfg = c->gram_cfg[s->format];
-    int64_t S = cfg.slabs;
-    if (S <= 0) {
-        S = std::max<int64_t>(1, (256 * 5 + a.T - 1) / a.T);
-        S = std::min<int64_t>(S, std::max<int64_t>(1, s->nkb / 16));
-    }
-    S = std::min<int64_t>(S, s->nkb);
     const int64_t tile_bytes = (int64_t)kTileS * kTileS * sizeof(float);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * tiles)));
-    a.kcb = (s->nkb + S - 1) / S;
+    if (cfg.slabs > 0) {
+        int64_t S = std::min<int64_t>(cfg.slabs, s->nkb);
+        S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * tiles)));
+        a.kcb = (s->nkb + S - 1) / S;
+    } else {
+        const int64_t nkb = s->nkb;
+        a.kcb = choose_kcb(&a.T, &tiles, &nkb, 1, device_cus(c->device), size_t(2) << 30);
+    }
     a.S = (int)((s->nkb + a.kcb - 1) / a.kcb);
     a.xcd_remap = cfg.xcd_remap;
     PRNU_TRY(c->ws_gram.ensure((size_t)tile_bytes * tiles * a.S));
@@ -674,16 +709,18 @@ int plan_layout(GramPlan *p, int64_t kcb) {
     return PRNU_OK;
 }
 
-// Default slab: about 8 workgroups per CU over all items.
+// Default slab: the cheapest by the rounds model of choose_kcb.
 int64_t plan_default_kcb(const GramPlan *p) {
-    double work = 0.0;
+    std::vector<int> items, tiles;
+    std::vector<int64_t> nkb;
     for (const prnu_gram_group &g : p->groups) {
         const Store *s = reinterpret_cast<const Store *>(g.store);
         const int nt = (g.count + kTileS - 1) / kTileS;
-        work += (double)host_items_per_slab(nt) * (double)s->nkb;
+        items.push_back(host_items_per_slab(nt));
+        tiles.push_back(nt * (nt + 1) / 2);
+        nkb.push_back(s->nkb);
     }
-    int64_t kcb = (int64_t)(work / (8.0 * p->cus));
-    return std::max<int64_t>(16, kcb);
+    return choose_kcb(items.data(), tiles.data(), nkb.data(), (int)items.size(), p->cus, kPlanPartialCap);
 }
 
 int plan_launch(GramPlan *p, hipStream_t st) {
@@ -771,9 +808,7 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
     p->c = c;
     p->format = format;
     p->groups.assign(groups, groups + ngroups);
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
-    p->cus = cus;  // one 512-thread, 128 KiB-LDS workgroup per CU at a time
+    p->cus = device_cus(c->device);  // one 512-thread, 128 KiB-LDS workgroup per CU at a time
     for (const prnu_gram_group &g : p->groups) {
         const Store *s = reinterpret_cast<const Store *>(g.store);
         const int nt = (g.count + kTileS - 1) / kTileS;
