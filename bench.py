@@ -123,12 +123,13 @@ def cpu_baseline(torch, p):
 SERVE_H, SERVE_W, SERVE_STORE, SERVE_RATE = 1024, 1024, 20000, 200.0
 
 
-def synth_query_images(torch, fingerprints, owners, seed):
+def synth_images(torch, fingerprints, owners, seed, h=None, w=None):
     """u8 (len(owners), h, w) images on the GPU: I = clamp(round(S*(1+0.02*K) + N(0, 2^2)), 0, 255)
     with S a sum of 8 random-phase cosines (frequencies <= 4 per frame) rescaled
     to [40, 215] and K the owner's fingerprint (a fresh N(0,1) field for -1)."""
     g = torch.Generator(device="cuda").manual_seed(seed)
-    h, w = SERVE_H, SERVE_W
+    h = SERVE_H if h is None else h
+    w = SERVE_W if w is None else w
     y = torch.arange(h, device="cuda", dtype=torch.float32)[:, None]
     x = torch.arange(w, device="cuda", dtype=torch.float32)[None, :]
     out = torch.empty((len(owners), h, w), dtype=torch.uint8, device="cuda")
@@ -159,7 +160,7 @@ def serving_bench(torch, duration: float):
     sensors = torch.randn((64, n), generator=g, device="cuda")       # PRNU K of the enrolled cameras
     ref = csi.PRNUFilter(SERVE_H, SERVE_W)
     pats = torch.empty((256, n), device="cuda")
-    imgs = synth_query_images(torch, sensors, [c for c in range(64) for _ in range(4)], seed=99)
+    imgs = synth_images(torch, sensors, [c for c in range(64) for _ in range(4)], seed=99)
     ref.apply_gpu_batch(imgs.view(256, n), pats, _lib.INPUT_GRAY8)
     enrolled = pats.view(64, 4, n).mean(dim=1)
     enrolled -= enrolled.mean(dim=1, keepdim=True)
@@ -186,7 +187,7 @@ def serving_bench(torch, duration: float):
     owners = [int(rng.integers(0, 64)) if rng.random() < 0.5 else -1 for _ in range(total)]
     imgs = []
     for b in range(0, total, 128):
-        imgs.append(synth_query_images(torch, known, owners[b:b + 128], seed=b).cpu().numpy())
+        imgs.append(synth_images(torch, known, owners[b:b + 128], seed=b).cpu().numpy())
     imgs = np.concatenate(imgs)
     srv = serving.Server(store, k=10, max_batch=64, slots=2).start()
     reqs = []
@@ -291,45 +292,85 @@ def groups_bench(torch, steps=5):
             "autotune_best_ms": tune_ms}
 
 
-def extraction_bench(torch, batch=32, pce_spectra=32):
-    """PRNU extraction of RGB8 frames of the workload size (one batched call per
-    `batch` frames) in images/s, and the PCE of all pairs of `pce_spectra`
-    extracted spectra (reference variant) in pairs/s."""
+def extraction_bench(torch, batch=32):
+    """PRNU extraction of RGB8 frames (one batched call per `batch` frames) in
+    images/s at 2048x1536 and 512x512, NCC patterns and PCE spectra."""
     from rocket_amd import _lib, csi
-    lib = _lib.load()
-    n = HEIGHT * WIDTH
-    g = torch.Generator(device="cuda").manual_seed(77)
-    imgs = torch.randint(0, 256, (batch, 3 * n), generator=g, device="cuda", dtype=torch.uint8)
     res = {}
-    for mode, name in ((_lib.MODE_NCC, "ncc"), (_lib.MODE_PCE, "pce")):
-        f = csi.PRNUFilter(HEIGHT, WIDTH, apply_spectral_filter=(mode == _lib.MODE_PCE))
-        out = torch.empty((batch, f.get_output_size() // 4), device="cuda")
-        f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8)            # warm-up, sizes the workspace
-        reps = 3
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(reps):
-            f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8, sync=False)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t1) / reps
-        res[f"{name}_images_per_s"] = batch / dt
-        res[f"{name}_ms_per_image"] = 1000.0 * dt / batch
-        if mode == _lib.MODE_PCE:
-            m = min(pce_spectra, batch)
-            cmp = csi.PeakToCorrelationEnergy(HEIGHT, WIDTH, _lib.PCE_REFERENCE, _ctx=f._ctx)
-            pairs = np.array([(i, j) for i in range(m) for j in range(i + 1, m)], dtype=np.int32)
-            cmp.pairs(out[:m], pairs)                             # warm-up
+    for h, w, tag in ((HEIGHT, WIDTH, ""), (512, 512, "_512")):
+        n = h * w
+        g = torch.Generator(device="cuda").manual_seed(77)
+        imgs = torch.randint(0, 256, (batch, 3 * n), generator=g, device="cuda", dtype=torch.uint8)
+        for mode, name in ((_lib.MODE_NCC, "ncc"), (_lib.MODE_PCE, "pce")):
+            f = csi.PRNUFilter(h, w, apply_spectral_filter=(mode == _lib.MODE_PCE))
+            out = torch.empty((batch, f.get_output_size() // 4), device="cuda")
+            f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8)            # warm-up, sizes the workspace
+            reps = 3
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            cmp.pairs(out[:m], pairs, sync=False)
+            for _ in range(reps):
+                f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8, sync=False)
             torch.cuda.synchronize()
-            dt = time.perf_counter() - t1
-            res["pce_pairs_per_s"] = len(pairs) / dt
-            res["pce_pairs"] = int(len(pairs))
-        f.cleanup()
-        del out
-    res.update({"batch": batch, "input": f"RGB8 {WIDTH}x{HEIGHT}"})
+            dt = (time.perf_counter() - t1) / reps
+            res[f"{name}_images_per_s{tag}"] = batch / dt
+            res[f"{name}_ms_per_image{tag}"] = 1000.0 * dt / batch
+            f.cleanup()
+            del out
+    res.update({"batch": batch, "input": f"RGB8 {WIDTH}x{HEIGHT} and 512x512"})
     return res
+
+
+PCE_H, PCE_W, PCE_COUNT, PCE_CAMERAS = 1024, 1024, 256, 8
+
+
+def pce_bench(torch):
+    """PCE of all 32,640 pairs of 256 extracted 1024x1024 spectra (8 cameras),
+    reference and textbook variants, from device-resident spectra."""
+    from rocket_amd import _lib, csi
+    n = PCE_H * PCE_W
+    g = torch.Generator(device="cuda").manual_seed(2020)
+    sensors = torch.randn((PCE_CAMERAS, n), generator=g, device="cuda")
+    f = csi.PRNUFilter(PCE_H, PCE_W, apply_spectral_filter=True)
+    spec = torch.empty((PCE_COUNT, f.get_output_size() // 4), device="cuda")
+    owners = [i % PCE_CAMERAS for i in range(PCE_COUNT)]
+    for b in range(0, PCE_COUNT, 64):
+        imgs = synth_images(torch, sensors, owners[b:b + 64], seed=b, h=PCE_H, w=PCE_W)
+        f.apply_gpu_batch(imgs.view(len(imgs), n), spec[b:b + 64], _lib.INPUT_GRAY8)
+    pairs = np.array([(i, j) for i in range(PCE_COUNT) for j in range(i + 1, PCE_COUNT)], dtype=np.int32)
+    same = np.array([owners[i] == owners[j] for i, j in pairs])
+    res = {"workload": f"pce{PCE_COUNT}_{PCE_W}x{PCE_H}", "pairs": int(len(pairs))}
+    for variant, name in ((_lib.PCE_REFERENCE, "reference"), (_lib.PCE_TEXTBOOK, "textbook")):
+        cmp = csi.PeakToCorrelationEnergy(PCE_H, PCE_W, variant, _ctx=f._ctx)
+        out = cmp.pairs(spec, pairs[:256])                         # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out = cmp.pairs(spec, pairs, sync=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        v = out.cpu().numpy()
+        res[f"{name}_pairs_per_s"] = len(pairs) / dt
+        res[f"{name}_same_camera_min"] = float(v[same].min())
+        res[f"{name}_other_camera_max"] = float(v[~same].max())
+    f.cleanup()
+    return res
+
+
+def fp8_bench(torch, p, gram, reps=3):
+    """All-pairs NCC of the bench patterns with FP8-e4m3 MFMA (per-pattern
+    scale), statistics and quantisation included; max |dNCC| on the sample."""
+    from rocket_amd import csi
+    ptrs = csi.pattern_pointer_table(p)
+    out = torch.empty((COUNT, COUNT), dtype=torch.float64, device="cuda")
+    stats = gram.stats(ptrs, COUNT)
+    gram.ncc(precision="fp8", stats=stats, out=out, ptrs=ptrs)     # warm-up
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        gram.ncc(precision="fp8", stats=stats, out=out, sync=False, ptrs=ptrs)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t1) / reps
+    return {"fp8_pairs_per_s": COUNT * (COUNT - 1) / 2 / (ms / 1000.0), "fp8_ms_per_step": ms,
+            "fp8_max_abs_dncc": fp64_check(torch, p, out.cpu().numpy())}
 
 
 def main():
@@ -416,6 +457,9 @@ def main():
         mp, kp = timed(ptr_step(prec), 3, 1)
         extra.update({"pointer_table_ms_per_step": mp, "pointer_table_tile_kernel_ms": kp})
         log(f"{prec} pointer table: {mp:.3f} ms/step")
+        f8 = fp8_bench(torch, p, gram)
+        extra.update(f8)
+        log(f"fp8: {f8['fp8_ms_per_step']:.3f} ms/step, max |dNCC| {f8['fp8_max_abs_dncc']:.2e}")
     for st in stores.values():
         st.close()
 
@@ -447,12 +491,16 @@ def main():
             f"{groups['eager_ms_per_step']:.2f} ms eager, {groups['separate_launches_ms_per_step']:.2f} ms as "
             f"separate launches), |dNCC| {groups['max_abs_dncc_vs_fp64_sample']:.2e}")
 
-    extract = None
+    extract = pce = None
     if not args.no_extract:
-        log("extraction / PCE ...")
+        log("extraction ...")
         extract = extraction_bench(torch)
         log(f"extraction: {extract['ncc_images_per_s']:.1f} images/s (NCC patterns), "
-            f"{extract['pce_images_per_s']:.1f} images/s (PCE spectra); PCE {extract['pce_pairs_per_s']:.1f} pairs/s")
+            f"{extract['pce_images_per_s']:.1f} images/s (PCE spectra) at {WIDTH}x{HEIGHT}")
+        log("PCE ...")
+        pce = pce_bench(torch)
+        log(f"PCE: {pce['reference_pairs_per_s']:.1f} pairs/s (reference), "
+            f"{pce['textbook_pairs_per_s']:.1f} pairs/s (textbook)")
 
     serve = None
     if not args.no_serving:
@@ -483,6 +531,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serve,
         "extraction": extract,
+        "pce": pce,
         "groups": groups,
     }
     line.update(extra)
