@@ -256,7 +256,7 @@ __device__ __forceinline__ int items_per_slab(int nt) { return nt * (nt - 1) / 2
 // B rows and writes its FP32 128 x 64 partial block to `out` (row stride
 // 256; null for an idle wave).  Ends with a barrier, so a caller may start
 // the next item on the same LDS.
-template <int FMT>
+template <int FMT, bool DMA>
 __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, int64_t cap_bytes, int first, int count,
                                           const ItemWaves &w, int64_t b_begin, int64_t b_end, float *__restrict__ out,
                                           uint8_t *lds) {
@@ -291,26 +291,55 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    uint4 st[8];
-    auto load = [&](int64_t b) {
-        const uint8_t *col = blocks + b * cap_bytes + (chunk << 4);
+    // Register-staged loads (DMA = false): global -> VGPR st[] -> LDS after
+    // the MFMAs, rows past count zeroed.  Direct-to-LDS loads (DMA = true,
+    // global_load_lds_dwordx4): each wave instruction fills 1 KiB of LDS, lane
+    // i at byte 16*i, i.e. rows r0 + 64*it of this thread's wave, LDS slot
+    // (tid & 7); the lane therefore fetches global chunk (tid & 7) ^ (r & 7),
+    // which is the XOR swizzle s_off reads.  Rows past count hold a clamped
+    // duplicate: they only feed output rows/columns >= count, which the
+    // reduction never reads.
+    uint4 st[DMA ? 1 : 8];
+    const int dchunk = chunk ^ (r0 & 7);
+    auto load_to = [&](int64_t b, uint8_t *buf) {
+        if constexpr (DMA) {
+            const uint8_t *col = blocks + b * cap_bytes + (dchunk << 4);
+            const int wbase = (tid >> 6) * 8;  // first row of this wave within a 64-row group
 #pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            if (it < 4 || !half) {
-                typedef const __attribute__((address_space(1))) unsigned int __attribute__((ext_vector_type(4))) *gu4;
-                const auto v = *(gu4)(col + row_off(it));
-                st[it] = make_uint4(v[0], v[1], v[2], v[3]);
+            for (int it = 0; it < 8; ++it) {
+                if (it < 4 || !half) {
+                    typedef const __attribute__((address_space(1))) void *gptr;
+                    typedef __attribute__((address_space(3))) void *lptr;
+                    __builtin_amdgcn_global_load_lds((gptr)(col + row_off(it)),
+                                                     (lptr)(buf + (64 * it + wbase) * kBlkBytes), 16, 0, 0);
+                }
+            }
+        } else {
+            (void)buf;
+            const uint8_t *col = blocks + b * cap_bytes + (chunk << 4);
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                if (it < 4 || !half) {
+                    typedef const __attribute__((address_space(1))) unsigned int __attribute__((ext_vector_type(4))) *gu4;
+                    const auto v = *(gu4)(col + row_off(it));
+                    st[it] = make_uint4(v[0], v[1], v[2], v[3]);
+                }
             }
         }
     };
     auto store = [&](uint8_t *buf) {
+        if constexpr (DMA) {
+            (void)buf;
+            __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this thread's LDS fills have landed
+        } else {
 #pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const int r = r0 + 64 * it;
-            if (r < nrows) {
-                uint4 v = st[it];
-                if (!((valid >> it) & 1)) v = make_uint4(0u, 0u, 0u, 0u);
-                *reinterpret_cast<uint4 *>(buf + s_off(r, chunk)) = v;
+            for (int it = 0; it < 8; ++it) {
+                const int r = r0 + 64 * it;
+                if (r < nrows) {
+                    uint4 v = st[it];
+                    if (!((valid >> it) & 1)) v = make_uint4(0u, 0u, 0u, 0u);
+                    *reinterpret_cast<uint4 *>(buf + s_off(r, chunk)) = v;
+                }
             }
         }
     };
@@ -361,12 +390,12 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     };
 
     const int64_t nst = b_end - b_begin;
-    load(b_begin);
+    load_to(b_begin, lds);
     store(lds);
     __syncthreads();
     for (int64_t q = 0; q < nst; ++q) {
         const bool next = q + 1 < nst;
-        if (next) load(b_begin + q + 1);
+        if (next) load_to(b_begin + q + 1, lds + ((q + 1) & 1) * kBufS);
         if (active) compute(lds + (q & 1) * kBufS);
         if (next) store(lds + ((q + 1) & 1) * kBufS);
         __syncthreads();
@@ -384,7 +413,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 }
 
 // Run item u of slab s: partial tile t of slab s lives at partials + (t*S + s) * 256*256.
-template <int FMT>
+template <int FMT, bool DMA>
 __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int nt,
                                          int S, int64_t nkb, int64_t kcb, float *partials, int u, int s,
                                          uint8_t *lds) {
@@ -400,12 +429,12 @@ __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_byte
     const int64_t b_end = min(nkb, b_begin + kcb);
     float *out = w.out_tile < 0 ? nullptr
                                 : partials + ((int64_t)w.out_tile * S + s) * (kTileS * kTileS) + w.out_off;
-    gram_item<FMT>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
+    gram_item<FMT, DMA>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
 }
 
 // One launch = one work item per workgroup.  With xcd_remap, the items of
 // a slab run on one XCD (workgroup ids are dealt to the 8 XCDs round-robin).
-template <int FMT>
+template <int FMT, bool DMA>
 __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
     const int bid = blockIdx.x;
@@ -419,7 +448,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
         s = bid / a.T;
     }
     if (s >= a.S) return;
-    run_item<FMT>(a.blocks, a.cap_bytes, a.first, a.count, a.nt, a.S, a.nkb, a.kcb, a.partials, u, s, lds);
+    run_item<FMT, DMA>(a.blocks, a.cap_bytes, a.first, a.count, a.nt, a.S, a.nkb, a.kcb, a.partials, u, s, lds);
 }
 
 // ------------------------------------------------------------ grouped plan
@@ -440,7 +469,7 @@ struct GroupDev {
     int first, count, nt, T, S, pad;
 };
 
-template <int FMT>
+template <int FMT, bool DMA>
 __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const GroupDev *__restrict__ groups,
                                                                     int ngroups) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
@@ -453,7 +482,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
     const int u = local % g.T;
     const int s = (local / g.T) * 8 + (int)(vl & 7);
     if (s >= g.S) return;  // padding item
-    run_item<FMT>(g.blocks, g.cap_bytes, g.first, g.count, g.nt, g.S, g.nkb, g.kcb, g.partials, u, s, lds);
+    run_item<FMT, DMA>(g.blocks, g.cap_bytes, g.first, g.count, g.nt, g.S, g.nkb, g.kcb, g.partials, u, s, lds);
 }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
@@ -615,10 +644,18 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const int64_t S8 = a.xcd_remap ? (a.S + 7) / 8 * 8 : a.S;
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
-    if (s->format == PRNU_PREC_SPLIT_BF16)
-        hipLaunchKernelGGL(store_gram_kernel<PRNU_PREC_SPLIT_BF16>, grid, dim3(kThrS), 0, st, a);
-    else
-        hipLaunchKernelGGL(store_gram_kernel<PRNU_PREC_FP32>, grid, dim3(kThrS), 0, st, a);
+    const bool dma = cfg.variant == 1;  // direct-to-LDS loads
+    if (s->format == PRNU_PREC_SPLIT_BF16) {
+        if (dma)
+            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_SPLIT_BF16, true>), grid, dim3(kThrS), 0, st, a);
+        else
+            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_SPLIT_BF16, false>), grid, dim3(kThrS), 0, st, a);
+    } else {
+        if (dma)
+            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_FP32, true>), grid, dim3(kThrS), 0, st, a);
+        else
+            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_FP32, false>), grid, dim3(kThrS), 0, st, a);
+    }
     PRNU_CHECK_LAUNCH();
     time_end(c, st);
     hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials, a.S,
@@ -639,6 +676,7 @@ struct GramPlan {
     int64_t kcb = 0;                 // K blocks per slab
     int64_t v_total = 0, items = 0;  // workgroups launched, work items (padding excluded)
     int cus = 0;
+    bool dma = false;                // direct-to-LDS loads (the context's store-format config, variant 1)
     double flop = 0.0;               // one product over every group's triangle (incl. diagonal)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -725,12 +763,22 @@ int64_t plan_default_kcb(const GramPlan *p) {
 
 int plan_launch(GramPlan *p, hipStream_t st) {
     const int ng = (int)p->gd.size();
-    if (p->format == PRNU_PREC_SPLIT_BF16)
-        hipLaunchKernelGGL(store_gram_groups_kernel<PRNU_PREC_SPLIT_BF16>, dim3((unsigned)p->v_total), dim3(kThrS),
-                           0, st, p->d_groups, ng);
-    else
-        hipLaunchKernelGGL(store_gram_groups_kernel<PRNU_PREC_FP32>, dim3((unsigned)p->v_total), dim3(kThrS), 0, st,
-                           p->d_groups, ng);
+    const dim3 grid((unsigned)p->v_total);
+    if (p->format == PRNU_PREC_SPLIT_BF16) {
+        if (p->dma)
+            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_SPLIT_BF16, true>), grid, dim3(kThrS), 0, st,
+                               p->d_groups, ng);
+        else
+            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_SPLIT_BF16, false>), grid, dim3(kThrS), 0, st,
+                               p->d_groups, ng);
+    } else {
+        if (p->dma)
+            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_FP32, true>), grid, dim3(kThrS), 0, st,
+                               p->d_groups, ng);
+        else
+            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_FP32, false>), grid, dim3(kThrS), 0, st,
+                               p->d_groups, ng);
+    }
     PRNU_CHECK_LAUNCH();
     for (int i = 0; i < ng; ++i) {
         const GroupDev &d = p->gd[i];
@@ -807,6 +855,7 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
     }
     p->c = c;
     p->format = format;
+    p->dma = c->gram_cfg[format].variant == 1;
     p->groups.assign(groups, groups + ngroups);
     p->cus = device_cus(c->device);  // one 512-thread, 128 KiB-LDS workgroup per CU at a time
     for (const prnu_gram_group &g : p->groups) {

@@ -114,3 +114,31 @@ def test_plan_rejects_invalid(groups):
     other.close()
     with pytest.raises(ValueError):
         csi.GramPlan(gram, [])
+
+
+@pytest.mark.parametrize("fmt", ["split_bf16", "fp32"])
+def test_store_gram_direct_to_lds_variant(fmt):
+    """variant 1 of the store Gram (direct global-to-LDS loads) against the
+    oracle and against variant 0, ragged count and an offset slot range."""
+    import ctypes
+    from rocket_amd import _lib
+    n, count, cap, first = 120 * 97, 300, 310, 5
+    p = patterns(count, n, 4, seed=911)
+    st = csi.PatternStore(n, cap, fmt)
+    st.put(p, first_slot=first)
+    gram = csi.GramNCC(8, 8)
+    base = st.gram(gram, count, first).cpu().numpy()
+    prec = _lib.PRECISIONS[fmt]
+    cfg = (ctypes.c_int32 * 3)(0, 1, 1)
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, prec, cfg))
+    got = st.gram(gram, count, first).cpu().numpy()
+    plan = csi.GramPlan(gram, [(st, first, count)])
+    via_plan = plan.run(use_graph=True)[0].cpu().numpy()
+    plan.close()
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, prec, None))
+    want = O.ncc_matrix(p.cpu().numpy())
+    assert np.abs(got - want).max() <= NCC_TOL
+    assert np.abs(via_plan - want).max() <= NCC_TOL
+    assert np.array_equal(got, base)          # same slabs and order: bitwise equal to variant 0
+    st.close()
+    gram.close()
