@@ -355,22 +355,33 @@ def pce_bench(torch):
     return res
 
 
-def fp8_bench(torch, p, gram, reps=3):
-    """All-pairs NCC of the bench patterns with FP8-e4m3 MFMA (per-pattern
-    scale), statistics and quantisation included; max |dNCC| on the sample."""
-    from rocket_amd import csi
-    ptrs = csi.pattern_pointer_table(p)
+def fp8_bench(torch, p, gram, reps=5):
+    """All-pairs NCC of the bench patterns in FP8-e4m3 (per-pattern scale) from
+    a K-blocked FP8 store; max |dNCC| on the FP64 sample."""
+    from rocket_amd import _lib, csi
+    lib = _lib.load()
     out = torch.empty((COUNT, COUNT), dtype=torch.float64, device="cuda")
-    stats = gram.stats(ptrs, COUNT)
-    gram.ncc(precision="fp8", stats=stats, out=out, ptrs=ptrs)     # warm-up
+    st = csi.PatternStore(HEIGHT * WIDTH, COUNT, "fp8")
+    st.put(p)
+    ms = ctypes.c_float()
+    st.gram(gram, COUNT, out=out)                                   # warm-up
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    kms = []
     for _ in range(reps):
-        gram.ncc(precision="fp8", stats=stats, out=out, sync=False, ptrs=ptrs)
+        st.gram(gram, COUNT, out=out, sync=False)
+        _lib.check(lib.prnu_ctx_last_kernel_ms(gram._ctx.handle, ctypes.byref(ms)))
+        kms.append(ms.value)
     torch.cuda.synchronize()
-    ms = 1000.0 * (time.perf_counter() - t1) / reps
-    return {"fp8_pairs_per_s": COUNT * (COUNT - 1) / 2 / (ms / 1000.0), "fp8_ms_per_step": ms,
-            "fp8_max_abs_dncc": fp64_check(torch, p, out.cpu().numpy())}
+    step = 1000.0 * (time.perf_counter() - t1) / reps
+    kernel = float(np.mean(kms))
+    flops = 2.0 * HEIGHT * WIDTH * COUNT * (COUNT + 1) / 2.0
+    res = {"fp8_pairs_per_s": COUNT * (COUNT - 1) / 2 / (step / 1000.0), "fp8_ms_per_step": step,
+           "fp8_tile_kernel_ms": kernel, "fp8_tflops": flops / (kernel / 1000.0) / 1e12,
+           "fp8_max_abs_dncc": fp64_check(torch, p, out.cpu().numpy()),
+           "fp8_store": "K-blocked e4m3, 128 values per block, scale max|x|/448 per pattern"}
+    st.close()
+    return res
 
 
 def main():

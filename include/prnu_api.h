@@ -198,9 +198,12 @@ PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int
  * Device-resident pattern set for the batched path, the counterpart of the
  * device cache slots Rocket keeps preprocessed patterns in
  * (R/cache/DeviceCache.java:19-44), laid out for the Gram: each pattern is cut
- * into blocks of 32 values and block b of all slots is contiguous, stored as
- * split-BF16 (hi, lo) or FP32 (`format` = PRNU_PREC_SPLIT_BF16 or
- * PRNU_PREC_FP32).  Inserting computes the per-pattern FP64 statistics. */
+ * into 128-byte blocks and block b of all slots is contiguous, stored as
+ * split-BF16 (hi, lo; 32 values per block), FP32 (32 values) or FP8 e4m3
+ * (128 values of x / scale, scale = max|x| / 448 per pattern) for `format` =
+ * PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32 or PRNU_PREC_FP8.  Inserting computes
+ * the per-pattern FP64 statistics (FP64 sums of FP32 products, as the
+ * reference's NCC), so the NCC epilogue is exact whatever the format. */
 typedef struct prnu_store prnu_store;
 
 PRNU_EXPORT int prnu_store_create(int64_t n, int capacity, int format, prnu_store **out);

@@ -4,7 +4,9 @@
 // Layout.  A pattern of n values is cut into nkb = ceil(n/32) blocks of 32.
 // Block b of every slot is contiguous: byte offset (b * cap_pad + slot) * 128.
 // A block is 128 bytes: split-BF16 stores hi = rne(x) (32 bf16) then
-// lo = rne(x - hi) (32 bf16); FP32 stores the 32 floats.  Values past n are 0.
+// lo = rne(x - hi) (32 bf16); FP32 stores the 32 floats.  The FP8 format
+// instead cuts patterns into blocks of 128 values and stores e4m3(x / scale),
+// scale = max|x| / 448 per pattern.  Values past n are 0.
 //
 // Why.  One K stage of a 256 x 256 Gram tile reads 32 values of 512 patterns.
 // With row-major patterns those are 512 reads 4*n bytes apart, one page each;
@@ -29,6 +31,7 @@ struct Store {
     uint8_t *blocks = nullptr;
     double *sum = nullptr, *sumsq = nullptr;
     float *absmax = nullptr;
+    float *scale = nullptr;  // FP8: max|x| / 448 per slot
     Workspace ws_part;  // statistics partials
 };
 
@@ -36,8 +39,11 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 
-constexpr int kBlk = 32;        // values per block
+constexpr int kBlk = 32;        // values per block (split-BF16, FP32)
+constexpr int kBlk8 = 128;      // values per block (FP8)
+constexpr float kE4M3Max = 448.0f;
 constexpr int kBlkBytes = 128;  // bytes per block
 constexpr int kTileS = 256;
 constexpr int kRowsS = 2 * kTileS;
@@ -125,6 +131,47 @@ __global__ __launch_bounds__(256) void store_put_kernel(const float *const *ptrs
         o.pad = 0.f;
         part[(int64_t)p * gridDim.x + blockIdx.x] = o;
     }
+}
+
+// FP8 insert: thread -> 32 values (a quarter of a 128-value block) of one
+// pattern, quantised with the pattern's scale; grid (ceil(4*nkb/256), count).
+__global__ __launch_bounds__(256) void store_put_fp8_kernel(const float *const *ptrs, int count, int64_t n,
+                                                            int64_t nkb, uint8_t *blocks, int cap_pad, int first,
+                                                            const float *absmax, float *scale) {
+    const int p = blockIdx.y;
+    const int64_t sub = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float am = absmax[first + p];
+    const float inv = am > 0.f ? kE4M3Max / am : 1.0f;
+    if (sub == 0) scale[first + p] = am > 0.f ? am / kE4M3Max : 1.0f;
+    if (sub >= 4 * nkb) return;
+    const float *x = ptrs[p];
+    const int64_t k0 = sub * 32;
+    float v[32];
+    if (k0 + 32 <= n && ((reinterpret_cast<uintptr_t>(x + k0) & 15) == 0)) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 f = reinterpret_cast<const float4 *>(x + k0)[q];
+            v[4 * q] = f.x;
+            v[4 * q + 1] = f.y;
+            v[4 * q + 2] = f.z;
+            v[4 * q + 3] = f.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 32; ++e) v[e] = (k0 + e < n) ? x[k0 + e] : 0.f;
+    }
+    int w[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        int t = 0;
+        t = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * g] * inv, v[4 * g + 1] * inv, t, false);
+        t = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * g + 2] * inv, v[4 * g + 3] * inv, t, true);
+        w[g] = t;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(blocks + ((size_t)(sub >> 2) * cap_pad + first + p) * kBlkBytes +
+                                           (sub & 3) * 32);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
 // One 256-thread workgroup per pattern: strided partial sums, then a fixed
@@ -261,6 +308,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
                                           const ItemWaves &w, int64_t b_begin, int64_t b_end, float *__restrict__ out,
                                           uint8_t *lds) {
     constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
+    constexpr bool kFp8 = FMT == PRNU_PREC_FP8;
     const bool half = w.rows_y < 0;
     const int nrows = half ? kTileS : kRowsS;
     const bool active = out != nullptr;
@@ -345,7 +393,30 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     };
     auto compute = [&](const uint8_t *buf) {
         const int frow = lane & 15;
-        if (kSplit) {
+        if constexpr (kFp8) {
+            // one 128-value block per stage: lane (row frow, k-group lane>>4)
+            // holds bytes [32*(lane>>4), +32) = 16-byte chunks ch, ch+1
+            const int ch = (lane >> 4) * 2;
+            intx8 bq[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const int r = wcol0 + cb * 16 + frow;
+                const int4 lo = *reinterpret_cast<const int4 *>(buf + s_off(r, ch));
+                const int4 hi = *reinterpret_cast<const int4 *>(buf + s_off(r, ch + 1));
+                bq[cb] = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            }
+#pragma unroll
+            for (int rb = 0; rb < 8; ++rb) {
+                const int r = wrow0 + rb * 16 + frow;
+                const int4 lo = *reinterpret_cast<const int4 *>(buf + s_off(r, ch));
+                const int4 hi = *reinterpret_cast<const int4 *>(buf + s_off(r, ch + 1));
+                const intx8 aq = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aq, bq[cb], acc[rb][cb], 0, 0, 0,
+                                                                                   127, 0, 127);
+            }
+        } else if (kSplit) {
             const int ch = lane >> 4;  // hi chunks 0..3, lo chunks 4..7
             bf16x8 bh[4], bl[4];
 #pragma unroll
@@ -489,6 +560,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
 __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restrict__ partials, int S, int nt,
                                                            int count, const double *__restrict__ sum,
                                                            const double *__restrict__ sumsq,
+                                                           const float *__restrict__ scale,
                                                            double *__restrict__ out) {
     const int t = blockIdx.y;
     const int e = blockIdx.x * 256 + threadIdx.x;
@@ -500,6 +572,7 @@ __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restri
     const float *p = partials + (int64_t)t * S * (kTileS * kTileS) + e;
     double g = 0.0;
     for (int q = 0; q < S; ++q) g += (double)p[(int64_t)q * (kTileS * kTileS)];
+    if (scale) g *= (double)scale[i] * (double)scale[j];  // FP8: undo the per-pattern scales
     const double di = sqrt(sumsq[i] - sum[i] * sum[i]);
     const double dj = sqrt(sumsq[j] - sum[j] * sum[j]);
     const double v = g / (di * dj);
@@ -508,6 +581,9 @@ __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restri
 }
 
 }  // namespace
+
+// Values per 128-byte block of a store format.
+int block_values(int format) { return format == PRNU_PREC_FP8 ? kBlk8 : kBlk; }
 
 // Work items per K slab: off-diagonal tiles plus diagonal packs (item_waves).
 int host_items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
@@ -549,15 +625,16 @@ int64_t choose_kcb(const int *items, const int *tiles, const int64_t *nkb, int n
 int store_create(int64_t n, int capacity, int format, Store **out) {
     *out = nullptr;
     PRNU_REQUIRE(n >= 1 && capacity >= 1, "store_create: n and capacity must be positive");
-    PRNU_REQUIRE(format == PRNU_PREC_SPLIT_BF16 || format == PRNU_PREC_FP32,
-                 "store_create: format must be PRNU_PREC_SPLIT_BF16 or PRNU_PREC_FP32");
+    PRNU_REQUIRE(format == PRNU_PREC_SPLIT_BF16 || format == PRNU_PREC_FP32 || format == PRNU_PREC_FP8,
+                 "store_create: format must be PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32 or PRNU_PREC_FP8");
     Store *s = new (std::nothrow) Store();
     if (!s) {
         set_error("store_create: out of host memory");
         return PRNU_ENOMEM;
     }
     s->n = n;
-    s->nkb = (n + kBlk - 1) / kBlk;
+    const int vals = block_values(format);
+    s->nkb = (n + vals - 1) / vals;
     s->capacity = capacity;
     s->cap_pad = (capacity + kTileS - 1) / kTileS * kTileS;
     s->format = format;
@@ -567,6 +644,7 @@ int store_create(int64_t n, int capacity, int format, Store **out) {
     if (e == hipSuccess) e = hipMalloc(&s->sum, sizeof(double) * s->cap_pad);
     if (e == hipSuccess) e = hipMalloc(&s->sumsq, sizeof(double) * s->cap_pad);
     if (e == hipSuccess) e = hipMalloc(&s->absmax, sizeof(float) * s->cap_pad);
+    if (e == hipSuccess && format == PRNU_PREC_FP8) e = hipMalloc(&s->scale, sizeof(float) * s->cap_pad);
     // empty slots read as zeros (the Gram masks them anyway)
     if (e == hipSuccess) e = hipMemset(s->blocks, 0, bytes);
     if (e != hipSuccess) {
@@ -576,6 +654,7 @@ int store_create(int64_t n, int capacity, int format, Store **out) {
         if (s->sum) (void)hipFree(s->sum);
         if (s->sumsq) (void)hipFree(s->sumsq);
         if (s->absmax) (void)hipFree(s->absmax);
+        if (s->scale) (void)hipFree(s->scale);
         delete s;
         return PRNU_ENOMEM;
     }
@@ -590,6 +669,7 @@ void store_destroy(Store *s) {
     if (s->sum) (void)hipFree(s->sum);
     if (s->sumsq) (void)hipFree(s->sumsq);
     if (s->absmax) (void)hipFree(s->absmax);
+    if (s->scale) (void)hipFree(s->scale);
     s->ws_part.release();
     delete s;
 }
@@ -597,6 +677,17 @@ void store_destroy(Store *s) {
 int store_put(Store *s, const float *const *d_ptrs, int count, int first, hipStream_t st) {
     PRNU_REQUIRE(count >= 1 && first >= 0 && first + count <= s->capacity,
                  "store_put: slots [%d, %d) outside the capacity %d", first, first + count, s->capacity);
+    if (s->format == PRNU_PREC_FP8) {
+        // statistics (and max|x|, which sets the scale) first, then quantise
+        PRNU_REQUIRE(count <= 65535, "store_put: at most 65535 FP8 patterns per call");
+        PRNU_TRY(launch_pattern_stats(d_ptrs, count, s->n, s->sum + first, s->sumsq + first, s->absmax + first,
+                                      s->ws_part, st));
+        const dim3 grid8((unsigned)((4 * s->nkb + 255) / 256), (unsigned)count);
+        hipLaunchKernelGGL(store_put_fp8_kernel, grid8, dim3(256), 0, st, d_ptrs, count, s->n, s->nkb, s->blocks,
+                           s->cap_pad, first, s->absmax, s->scale);
+        PRNU_CHECK_LAUNCH();
+        return PRNU_OK;
+    }
     const unsigned gx = (unsigned)((s->nkb + 31) / 32);
     PRNU_REQUIRE((count + 7) / 8 <= 65535, "store_put: at most 524280 patterns per call");
     PRNU_TRY(s->ws_part.ensure(sizeof(StatPartS) * (size_t)gx * count));
@@ -645,21 +736,24 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
     const bool dma = cfg.variant == 1;  // direct-to-LDS loads
-    if (s->format == PRNU_PREC_SPLIT_BF16) {
-        if (dma)
-            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_SPLIT_BF16, true>), grid, dim3(kThrS), 0, st, a);
-        else
-            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_SPLIT_BF16, false>), grid, dim3(kThrS), 0, st, a);
-    } else {
-        if (dma)
-            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_FP32, true>), grid, dim3(kThrS), 0, st, a);
-        else
-            hipLaunchKernelGGL((store_gram_kernel<PRNU_PREC_FP32, false>), grid, dim3(kThrS), 0, st, a);
-    }
+#define PRNU_STORE_GRAM(F)                                                                        \
+    do {                                                                                          \
+        if (dma)                                                                                  \
+            hipLaunchKernelGGL((store_gram_kernel<F, true>), grid, dim3(kThrS), 0, st, a);        \
+        else                                                                                      \
+            hipLaunchKernelGGL((store_gram_kernel<F, false>), grid, dim3(kThrS), 0, st, a);       \
+    } while (0)
+    if (s->format == PRNU_PREC_SPLIT_BF16)
+        PRNU_STORE_GRAM(PRNU_PREC_SPLIT_BF16);
+    else if (s->format == PRNU_PREC_FP8)
+        PRNU_STORE_GRAM(PRNU_PREC_FP8);
+    else
+        PRNU_STORE_GRAM(PRNU_PREC_FP32);
+#undef PRNU_STORE_GRAM
     PRNU_CHECK_LAUNCH();
     time_end(c, st);
     hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials, a.S,
-                       a.nt, count, s->sum + first, s->sumsq + first, d_ncc);
+                       a.nt, count, s->sum + first, s->sumsq + first, s->scale ? s->scale + first : nullptr, d_ncc);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -764,27 +858,27 @@ int64_t plan_default_kcb(const GramPlan *p) {
 int plan_launch(GramPlan *p, hipStream_t st) {
     const int ng = (int)p->gd.size();
     const dim3 grid((unsigned)p->v_total);
-    if (p->format == PRNU_PREC_SPLIT_BF16) {
-        if (p->dma)
-            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_SPLIT_BF16, true>), grid, dim3(kThrS), 0, st,
-                               p->d_groups, ng);
-        else
-            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_SPLIT_BF16, false>), grid, dim3(kThrS), 0, st,
-                               p->d_groups, ng);
-    } else {
-        if (p->dma)
-            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_FP32, true>), grid, dim3(kThrS), 0, st,
-                               p->d_groups, ng);
-        else
-            hipLaunchKernelGGL((store_gram_groups_kernel<PRNU_PREC_FP32, false>), grid, dim3(kThrS), 0, st,
-                               p->d_groups, ng);
-    }
+#define PRNU_GROUPS_GRAM(F)                                                                                 \
+    do {                                                                                                    \
+        if (p->dma)                                                                                         \
+            hipLaunchKernelGGL((store_gram_groups_kernel<F, true>), grid, dim3(kThrS), 0, st, p->d_groups, ng);  \
+        else                                                                                                \
+            hipLaunchKernelGGL((store_gram_groups_kernel<F, false>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
+    } while (0)
+    if (p->format == PRNU_PREC_SPLIT_BF16)
+        PRNU_GROUPS_GRAM(PRNU_PREC_SPLIT_BF16);
+    else if (p->format == PRNU_PREC_FP8)
+        PRNU_GROUPS_GRAM(PRNU_PREC_FP8);
+    else
+        PRNU_GROUPS_GRAM(PRNU_PREC_FP32);
+#undef PRNU_GROUPS_GRAM
     PRNU_CHECK_LAUNCH();
     for (int i = 0; i < ng; ++i) {
         const GroupDev &d = p->gd[i];
         const Store *s = reinterpret_cast<const Store *>(p->groups[i].store);
         hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, d.nt * (d.nt + 1) / 2), dim3(256), 0, st,
-                           d.partials, d.S, d.nt, d.count, s->sum + d.first, s->sumsq + d.first, p->groups[i].d_ncc);
+                           d.partials, d.S, d.nt, d.count, s->sum + d.first, s->sumsq + d.first,
+                           s->scale ? s->scale + d.first : nullptr, p->groups[i].d_ncc);
         PRNU_CHECK_LAUNCH();
     }
     return PRNU_OK;
@@ -835,8 +929,7 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
     *out = nullptr;
     PRNU_REQUIRE(groups && ngroups >= 1 && ngroups <= kMaxPlanGroups, "gram_plan: 1..%d groups, got %d",
                  kMaxPlanGroups, ngroups);
-    PRNU_REQUIRE(slab_values >= 0 && slab_values % kBlk == 0,
-                 "gram_plan: slab_values must be a non-negative multiple of %d", kBlk);
+    PRNU_REQUIRE(slab_values >= 0, "gram_plan: slab_values must be non-negative");
     int format = -1;
     for (int i = 0; i < ngroups; ++i) {
         const prnu_gram_group &g = groups[i];
@@ -864,7 +957,13 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
         p->flop += 2.0 * (double)s->n * ((double)g.count * (g.count + 1) / 2.0);
         (void)nt;
     }
-    const int64_t kcb = slab_values ? slab_values / kBlk : plan_default_kcb(p);
+    const int vals = block_values(format);
+    if (slab_values % vals != 0) {
+        set_error("gram_plan: slab_values must be a multiple of %d for this store format", vals);
+        plan_free(p);
+        return PRNU_EINVAL;
+    }
+    const int64_t kcb = slab_values ? slab_values / vals : plan_default_kcb(p);
     const int rc = plan_layout(p, kcb);
     if (rc != PRNU_OK) {
         plan_free(p);
@@ -910,7 +1009,7 @@ int gram_plan_autotune(GramPlan *p, hipStream_t st, int64_t *best_values, float 
     (void)hipEventDestroy(e1);
     if (rc != PRNU_OK) return rc;
     PRNU_TRY(plan_layout(p, best_kcb));
-    if (best_values) *best_values = best_kcb * kBlk;
+    if (best_values) *best_values = best_kcb * block_values(p->format);
     if (best_ms) *best_ms = best;
     return PRNU_OK;
 }
@@ -1008,7 +1107,7 @@ int prnu_gram_plan_info(const prnu_gram_plan *plan, int64_t *work_items, int *wo
     const GramPlan *p = reinterpret_cast<const GramPlan *>(plan);
     if (work_items) *work_items = p->items;
     if (workgroups) *workgroups = (int)p->v_total;
-    if (slab_values) *slab_values = p->kcb * kBlk;
+    if (slab_values) *slab_values = p->kcb * block_values(p->format);
     if (partial_bytes) *partial_bytes = p->partial_bytes;
     if (flop) *flop = p->flop;
     return PRNU_OK;

@@ -142,3 +142,68 @@ def test_store_gram_direct_to_lds_variant(fmt):
     assert np.array_equal(got, base)          # same slabs and order: bitwise equal to variant 0
     st.close()
     gram.close()
+
+
+def fp8_bounds(n):
+    """e4m3 error bounds of test_gpu_ncc.test_gram_fp8_within_e4m3_bound:
+    off-diagonal 6 sigma of 0.05/sqrt(n); diagonal bias + 6 sigma."""
+    return 6 * 0.05 / np.sqrt(n), 1.5e-3 + 6 * 0.036 * np.sqrt(3.0 / n)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fp8_store_gram_within_e4m3_bound(variant):
+    """FP8 store (blocks of 128 values, per-pattern scale): n not a multiple
+    of 128, ragged count, offset slots; both load variants; close to the
+    pointer-table FP8 Gram, which quantises the same way."""
+    import ctypes
+    from rocket_amd import _lib
+    n, count, cap, first = 200 * 321, 300, 310, 3
+    p = patterns(count, n, 4, seed=777)
+    st = csi.PatternStore(n, cap, "fp8")
+    st.put(p, first_slot=first)
+    gram = csi.GramNCC(200, 321)
+    cfg = (ctypes.c_int32 * 3)(0, 1, variant)
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, _lib.PREC_FP8, cfg))
+    got = st.gram(gram, count, first).cpu().numpy()
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, _lib.PREC_FP8, None))
+    want = O.ncc_matrix(p.cpu().numpy())
+    off = ~np.eye(count, dtype=bool)
+    b_off, b_diag = fp8_bounds(n)
+    assert np.abs(got - want)[off].max() <= b_off
+    assert np.abs(np.diag(got) - np.diag(want)).max() <= b_diag
+    assert np.array_equal(got, got.T)
+    ptr = gram.ncc(p, "fp8").cpu().numpy()
+    assert np.abs(got - ptr).max() <= NCC_TOL        # same quantisation, different K slabs
+    s, ss, am = st.stats(count, first)
+    assert np.array_equal(am, np.abs(p.cpu().numpy()).max(axis=1))
+    st.close()
+    gram.close()
+
+
+def test_fp8_plan_two_sizes():
+    """Grouped plan over two FP8 stores of different sizes equals each store's
+    own Gram within the summation-order tolerance."""
+    specs = [(64 * 64, 130), (100 * 90, 257)]
+    stores, ps = [], []
+    for i, (n, count) in enumerate(specs):
+        p = patterns(count, n, 3, seed=880 + i)
+        st = csi.PatternStore(n, count, "fp8")
+        st.put(p)
+        stores.append(st)
+        ps.append(p)
+    gram = csi.GramNCC(8, 8)
+    plan = csi.GramPlan(gram, [(st, 0, c) for st, (_, c) in zip(stores, specs)])
+    assert plan.info()["slab_values"] % 128 == 0
+    outs = [o.cpu().numpy() for o in plan.run(use_graph=True)]
+    with pytest.raises(ValueError):                  # FP8 slabs are whole 128-value blocks
+        csi.GramPlan(gram, [(stores[0], 0, 130)], slab_values=96)
+    plan.close()
+    for st, (n, count), p, got in zip(stores, specs, ps, outs):
+        single = st.gram(gram, count).cpu().numpy()
+        assert np.abs(single - got).max() <= NCC_TOL
+        b_off, _ = fp8_bounds(n)
+        want = O.ncc_matrix(p.cpu().numpy())
+        off = ~np.eye(count, dtype=bool)
+        assert np.abs(got - want)[off].max() <= b_off
+        st.close()
+    gram.close()
