@@ -123,6 +123,33 @@ def cpu_baseline(torch, p):
 SERVE_H, SERVE_W, SERVE_STORE, SERVE_RATE = 1024, 1024, 20000, 200.0
 
 
+def cpu_baseline_aux(seconds=6.0):
+    """The FP64 numpy restatement of the extraction chain (RGB8 512x512) and of
+    reference-variant PCE (1024x1024 spectra) timed on one host core over a
+    bounded sample (part of the CPU baseline leg)."""
+    sys.path.insert(0, ROOT)
+    from oracle import prnu_oracle as O
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, size=(512, 512, 3), dtype=np.uint8)
+    t0 = time.perf_counter()
+    n_img = 0
+    while time.perf_counter() - t0 < seconds / 2 or n_img < 1:
+        O.prnu_pattern(img)
+        n_img += 1
+    ext = n_img / (time.perf_counter() - t0)
+    a = rng.standard_normal((1024, 1024))
+    b = rng.standard_normal((1024, 1024))
+    t0 = time.perf_counter()
+    n_pce = 0
+    while time.perf_counter() - t0 < seconds / 2 or n_pce < 1:
+        O.pce_reference(a, b)
+        n_pce += 1
+    pce = n_pce / (time.perf_counter() - t0)
+    return {"extraction_images_per_s_512": ext, "pce_pairs_per_s_1024": pce, "cores": 1, "kind": "port",
+            "sample": f"{n_img} RGB8 512x512 extractions and {n_pce} reference-variant PCE pairs at 1024x1024, "
+                      "FP64 numpy restatement, one thread"}
+
+
 def synth_images(torch, fingerprints, owners, seed, h=None, w=None):
     """u8 (len(owners), h, w) images on the GPU: I = clamp(round(S*(1+0.02*K) + N(0, 2^2)), 0, 255)
     with S a sum of 8 random-phase cosines (frequencies <= 4 per frame) rescaled
@@ -485,12 +512,15 @@ def main():
                 "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic, "kernel": kernel_name,
                 "kernel_ms": kms, "algorithmic_flop": flops}
 
-    cpu = None
+    cpu = cpu_aux = None
     if not args.no_cpu:
         log("CPU baseline ...")
         cpu = cpu_baseline(torch, p)
         if cpu:
             log(f"CPU: {cpu['value']:.4g} pairs/s on {cpu['cores']} threads")
+        cpu_aux = cpu_baseline_aux()
+        log(f"CPU: extraction {cpu_aux['extraction_images_per_s_512']:.2f} images/s (512x512), "
+            f"PCE {cpu_aux['pce_pairs_per_s_1024']:.2f} pairs/s (1024x1024), one thread")
 
     del p                                               # free 12.9 GB before the other legs
     torch.cuda.empty_cache()
@@ -540,6 +570,7 @@ def main():
         "max_abs_dncc_vs_fp64": dncc,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_aux": cpu_aux,
         "serving": serve,
         "extraction": extract,
         "pce": pce,
