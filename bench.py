@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving"],
+                    help="run one secondary leg alone and print its result (for profiling)")
     return ap.parse_args()
 
 
@@ -420,6 +422,12 @@ def main():
     from rocket_amd import _lib, csi
 
     lib = _lib.load()
+    if args.only:
+        leg = {"pce": lambda: pce_bench(torch), "extract": lambda: extraction_bench(torch),
+               "groups": lambda: groups_bench(torch),
+               "serving": lambda: serving_bench(torch, args.serving_seconds)}[args.only]
+        print(json.dumps({args.only: leg()}), flush=True)
+        return
     prec = args.precision
     t0 = time.time()
     p = make_patterns(torch)
@@ -506,7 +514,7 @@ def main():
     flops = one_product * PRODUCTS[prec]
     kms = float(np.mean(kernel_ms))
     achieved = flops / (kms / 1000.0) / 1e12
-    kernel_name = f"store_gram_kernel<{1 if prec == 'split_bf16' else 0}>"
+    kernel_name = f"store_gram_kernel<{1 if prec == 'split_bf16' else 0}, 0>"
     traffic = load_traffic(kernel_name)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic, "kernel": kernel_name,

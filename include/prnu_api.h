@@ -290,10 +290,11 @@ PRNU_EXPORT int prnu_fpstore_query(prnu_ctx *ctx, prnu_fpstore *store, const flo
 /* Launch shape of the Gram tile kernel (no reference counterpart).
  * slabs: K slabs per tile (0 = automatic); xcd_remap: 1 keeps the tiles of one
  * K slab on one XCD so they share it through that XCD's L2; variant: for the
- * split-BF16 Gram over pointer tables, 0 = 8 waves, one K stage in flight,
- * 1 = 4 waves with the full register file each, two K stages in flight; for
- * the store Gram and grouped plans (split-BF16 and FP32 stores), 0 = loads
- * staged through registers, 1 = direct global-to-LDS loads. */
+ * split-BF16 Gram over pointer tables, 1 = 4 waves with the full register
+ * file each, two K stages in flight, otherwise 8 waves, one K stage in
+ * flight; for the store Gram and grouped plans, 0 = direct global-to-LDS
+ * loads, 1 = loads staged through registers, 2 = direct loads with the wave
+ * priority raised over the MFMA block. */
 typedef struct {
     int32_t slabs;
     int32_t xcd_remap;

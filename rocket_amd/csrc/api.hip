@@ -260,9 +260,9 @@ int prnu_gram_set_config(prnu_ctx *ctx, int precision, const prnu_gram_config *c
         c->gram_cfg[precision] = prnu_gram_config{0, 1, 0};
         return PRNU_OK;
     }
-    PRNU_REQUIRE(cfg->slabs >= 0 && (cfg->xcd_remap == 0 || cfg->xcd_remap == 1) &&
-                     (cfg->variant == 0 || cfg->variant == 1),
-                 "gram_set_config: slabs must be >= 0, xcd_remap and variant 0 or 1");
+    PRNU_REQUIRE(cfg->slabs >= 0 && (cfg->xcd_remap == 0 || cfg->xcd_remap == 1) && cfg->variant >= 0 &&
+                     cfg->variant <= 2,
+                 "gram_set_config: slabs must be >= 0, xcd_remap 0 or 1, variant 0, 1 or 2");
     c->gram_cfg[precision] = *cfg;
     return PRNU_OK;
 }

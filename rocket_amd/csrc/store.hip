@@ -303,12 +303,16 @@ __device__ __forceinline__ int items_per_slab(int nt) { return nt * (nt - 1) / 2
 // B rows and writes its FP32 128 x 64 partial block to `out` (row stride
 // 256; null for an idle wave).  Ends with a barrier, so a caller may start
 // the next item on the same LDS.
-template <int FMT, bool DMA>
+// VAR: 0 = direct-to-LDS loads, 1 = loads staged through registers,
+// 2 = direct-to-LDS loads and raised wave priority over the MFMA block.
+template <int FMT, int VAR>
 __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, int64_t cap_bytes, int first, int count,
                                           const ItemWaves &w, int64_t b_begin, int64_t b_end, float *__restrict__ out,
                                           uint8_t *lds) {
     constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
     constexpr bool kFp8 = FMT == PRNU_PREC_FP8;
+    constexpr bool DMA = VAR != 1;
+    constexpr bool kPrio = VAR == 2;
     const bool half = w.rows_y < 0;
     const int nrows = half ? kTileS : kRowsS;
     const bool active = out != nullptr;
@@ -467,7 +471,11 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     for (int64_t q = 0; q < nst; ++q) {
         const bool next = q + 1 < nst;
         if (next) load_to(b_begin + q + 1, lds + ((q + 1) & 1) * kBufS);
-        if (active) compute(lds + (q & 1) * kBufS);
+        if (active) {
+            if constexpr (kPrio) __builtin_amdgcn_s_setprio(1);
+            compute(lds + (q & 1) * kBufS);
+            if constexpr (kPrio) __builtin_amdgcn_s_setprio(0);
+        }
         if (next) store(lds + ((q + 1) & 1) * kBufS);
         __syncthreads();
     }
@@ -484,7 +492,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 }
 
 // Run item u of slab s: partial tile t of slab s lives at partials + (t*S + s) * 256*256.
-template <int FMT, bool DMA>
+template <int FMT, int VAR>
 __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int nt,
                                          int S, int64_t nkb, int64_t kcb, float *partials, int u, int s,
                                          uint8_t *lds) {
@@ -500,12 +508,12 @@ __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_byte
     const int64_t b_end = min(nkb, b_begin + kcb);
     float *out = w.out_tile < 0 ? nullptr
                                 : partials + ((int64_t)w.out_tile * S + s) * (kTileS * kTileS) + w.out_off;
-    gram_item<FMT, DMA>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
+    gram_item<FMT, VAR>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
 }
 
 // One launch = one work item per workgroup.  With xcd_remap, the items of
 // a slab run on one XCD (workgroup ids are dealt to the 8 XCDs round-robin).
-template <int FMT, bool DMA>
+template <int FMT, int VAR>
 __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
     const int bid = blockIdx.x;
@@ -519,7 +527,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
         s = bid / a.T;
     }
     if (s >= a.S) return;
-    run_item<FMT, DMA>(a.blocks, a.cap_bytes, a.first, a.count, a.nt, a.S, a.nkb, a.kcb, a.partials, u, s, lds);
+    run_item<FMT, VAR>(a.blocks, a.cap_bytes, a.first, a.count, a.nt, a.S, a.nkb, a.kcb, a.partials, u, s, lds);
 }
 
 // ------------------------------------------------------------ grouped plan
@@ -540,7 +548,7 @@ struct GroupDev {
     int first, count, nt, T, S, pad;
 };
 
-template <int FMT, bool DMA>
+template <int FMT, int VAR>
 __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const GroupDev *__restrict__ groups,
                                                                     int ngroups) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
@@ -553,7 +561,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
     const int u = local % g.T;
     const int s = (local / g.T) * 8 + (int)(vl & 7);
     if (s >= g.S) return;  // padding item
-    run_item<FMT, DMA>(g.blocks, g.cap_bytes, g.first, g.count, g.nt, g.S, g.nkb, g.kcb, g.partials, u, s, lds);
+    run_item<FMT, VAR>(g.blocks, g.cap_bytes, g.first, g.count, g.nt, g.S, g.nkb, g.kcb, g.partials, u, s, lds);
 }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
@@ -735,13 +743,15 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const int64_t S8 = a.xcd_remap ? (a.S + 7) / 8 * 8 : a.S;
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
-    const bool dma = cfg.variant == 1;  // direct-to-LDS loads
+    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 direct-to-LDS + priority
 #define PRNU_STORE_GRAM(F)                                                                        \
     do {                                                                                          \
-        if (dma)                                                                                  \
-            hipLaunchKernelGGL((store_gram_kernel<F, true>), grid, dim3(kThrS), 0, st, a);        \
+        if (var == 1)                                                                             \
+            hipLaunchKernelGGL((store_gram_kernel<F, 1>), grid, dim3(kThrS), 0, st, a);           \
+        else if (var == 2)                                                                        \
+            hipLaunchKernelGGL((store_gram_kernel<F, 2>), grid, dim3(kThrS), 0, st, a);           \
         else                                                                                      \
-            hipLaunchKernelGGL((store_gram_kernel<F, false>), grid, dim3(kThrS), 0, st, a);       \
+            hipLaunchKernelGGL((store_gram_kernel<F, 0>), grid, dim3(kThrS), 0, st, a);           \
     } while (0)
     if (s->format == PRNU_PREC_SPLIT_BF16)
         PRNU_STORE_GRAM(PRNU_PREC_SPLIT_BF16);
@@ -770,7 +780,7 @@ struct GramPlan {
     int64_t kcb = 0;                 // K blocks per slab
     int64_t v_total = 0, items = 0;  // workgroups launched, work items (padding excluded)
     int cus = 0;
-    bool dma = false;                // direct-to-LDS loads (the context's store-format config, variant 1)
+    int var = 0;                     // load variant (the context's config of the format)
     double flop = 0.0;               // one product over every group's triangle (incl. diagonal)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -858,12 +868,14 @@ int64_t plan_default_kcb(const GramPlan *p) {
 int plan_launch(GramPlan *p, hipStream_t st) {
     const int ng = (int)p->gd.size();
     const dim3 grid((unsigned)p->v_total);
-#define PRNU_GROUPS_GRAM(F)                                                                                 \
-    do {                                                                                                    \
-        if (p->dma)                                                                                         \
-            hipLaunchKernelGGL((store_gram_groups_kernel<F, true>), grid, dim3(kThrS), 0, st, p->d_groups, ng);  \
-        else                                                                                                \
-            hipLaunchKernelGGL((store_gram_groups_kernel<F, false>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
+#define PRNU_GROUPS_GRAM(F)                                                                              \
+    do {                                                                                                 \
+        if (p->var == 1)                                                                                 \
+            hipLaunchKernelGGL((store_gram_groups_kernel<F, 1>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
+        else if (p->var == 2)                                                                            \
+            hipLaunchKernelGGL((store_gram_groups_kernel<F, 2>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
+        else                                                                                             \
+            hipLaunchKernelGGL((store_gram_groups_kernel<F, 0>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
     } while (0)
     if (p->format == PRNU_PREC_SPLIT_BF16)
         PRNU_GROUPS_GRAM(PRNU_PREC_SPLIT_BF16);
@@ -948,7 +960,7 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
     }
     p->c = c;
     p->format = format;
-    p->dma = c->gram_cfg[format].variant == 1;
+    p->var = c->gram_cfg[format].variant;
     p->groups.assign(groups, groups + ngroups);
     p->cus = device_cus(c->device);  // one 512-thread, 128 KiB-LDS workgroup per CU at a time
     for (const prnu_gram_group &g : p->groups) {

@@ -116,10 +116,12 @@ def test_plan_rejects_invalid(groups):
         csi.GramPlan(gram, [])
 
 
+@pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("fmt", ["split_bf16", "fp32"])
-def test_store_gram_direct_to_lds_variant(fmt):
-    """variant 1 of the store Gram (direct global-to-LDS loads) against the
-    oracle and against variant 0, ragged count and an offset slot range."""
+def test_store_gram_load_variants(fmt, variant):
+    """Store Gram load variants (1: register-staged loads, 2: direct loads
+    with raised priority) against the oracle and bitwise against the default
+    (direct global-to-LDS loads); ragged count, offset slot range."""
     import ctypes
     from rocket_amd import _lib
     n, count, cap, first = 120 * 97, 300, 310, 5
@@ -129,7 +131,7 @@ def test_store_gram_direct_to_lds_variant(fmt):
     gram = csi.GramNCC(8, 8)
     base = st.gram(gram, count, first).cpu().numpy()
     prec = _lib.PRECISIONS[fmt]
-    cfg = (ctypes.c_int32 * 3)(0, 1, 1)
+    cfg = (ctypes.c_int32 * 3)(0, 1, variant)
     _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, prec, cfg))
     got = st.gram(gram, count, first).cpu().numpy()
     plan = csi.GramPlan(gram, [(st, first, count)])
@@ -139,7 +141,7 @@ def test_store_gram_direct_to_lds_variant(fmt):
     want = O.ncc_matrix(p.cpu().numpy())
     assert np.abs(got - want).max() <= NCC_TOL
     assert np.abs(via_plan - want).max() <= NCC_TOL
-    assert np.array_equal(got, base)          # same slabs and order: bitwise equal to variant 0
+    assert np.array_equal(got, base)          # same slabs and order: bitwise equal to the default
     st.close()
     gram.close()
 
@@ -150,7 +152,7 @@ def fp8_bounds(n):
     return 6 * 0.05 / np.sqrt(n), 1.5e-3 + 6 * 0.036 * np.sqrt(3.0 / n)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_fp8_store_gram_within_e4m3_bound(variant):
     """FP8 store (blocks of 128 values, per-pattern scale): n not a multiple
     of 128, ragged count, offset slots; both load variants; close to the
