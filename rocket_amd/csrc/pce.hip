@@ -1,15 +1,21 @@
 // Peak-to-correlation energy on gfx950 (CSI/kernels/compare/
 // peaktocorrelationenergy.cu and PeakToCorrelationEnergy.java:202-232).
 //
-// Pipeline per pair, inputs are two R2C half spectra h x (w/2+1):
-//   1. pce_rows_kernel   builds row i of the full cross spectrum C and runs
-//                        the inverse row FFT (length w) in LDS;
-//   2. transpose         (h x w -> w x h);
-//   3. pce_cols_kernel   inverse FFT of each column (length h) in LDS; keeps
-//                        Re c, and per column the max |Re c| with its index
-//                        and sum (Re c)^2 in FP64;
-//   4. pce_final_kernel  argmax over columns, energy = total - window (FP64),
-//                        peak value, PCE.
+// Only Re c = Re IFFT2(C) enters the PCE, and Re IFFT2(C) = IFFT2(H) with
+// H(k) = (C(k) + conj(C(-k)))/2 Hermitian, so IFFT2 of Z = H_a + i H_b gives
+// Re c_a + i Re c_b: two pairs share one complex transform.  Pipeline per
+// couple of pairs, inputs are R2C half spectra h x (w/2+1):
+//   1. pce_rows2_kernel  builds row i of Z and runs the inverse row FFT
+//                        (length w) in LDS;
+//   2. pce_cols2_kernel  loads a strip of columns (coalesced rows of the
+//                        strip, no transpose), inverse FFT of each column
+//                        (length h) in LDS; writes Re c of both pairs and per
+//                        column the max |Re c| with its index and sum (Re c)^2
+//                        in FP64;
+//   3. pce_final_kernel  per pair: argmax over columns, energy = total -
+//                        window (FP64), peak value, PCE.
+// A pair's result depends on its couple partner only through FP32 rounding
+// (a fixed pair list always forms the same couples).
 // Reference variant (quirk list in the header of prnu_api.h's correlate):
 // right half of C uses row i instead of (h-i) mod h; window rows/cols strictly
 // inside (p-5, p+5), clipped, divisor N-121; peak value = Re c[h-1][w-1],
@@ -92,65 +98,109 @@ __device__ __forceinline__ void pce_operands(const PceBatch &b, int p, const flo
     }
 }
 
+// Cross spectrum of a pair at (i, j) of the full h x w grid from the half
+// spectra (wh = w/2 + 1 columns).  Reference variant
+// (peaktocorrelationenergy.cu:102-134): conj(X)Y on the left half, X conj(Y)
+// mirrored from column w - j of the SAME row i on the right.  Textbook: the
+// true cross spectrum (right half mirrored from row (h - i) mod h).
 template <int VARIANT>
-__global__ __launch_bounds__(kFftThreads) void pce_rows_kernel(FftDesc d, int h, PceBatch bt, float2 *__restrict__ Aall,
-                                                               int64_t pstride) {
-    extern __shared__ float2 buf[];
-    const int i = blockIdx.x;
-    const int w = d.n, wh = w / 2 + 1;
-    const float2 *X, *Y;
-    pce_operands(bt, blockIdx.y, X, Y);
-    float2 *A = Aall + (int64_t)blockIdx.y * pstride;
-    // rows of the half spectra feeding the right half of C
+__device__ __forceinline__ float2 cross_at(const float2 *X, const float2 *Y, int h, int w, int i, int j) {
+    const int wh = w / 2 + 1;
+    if (j < wh) {
+        const float2 x = X[(int64_t)i * wh + j], y = Y[(int64_t)i * wh + j];
+        return cmul(cconj(x), y);
+    }
     const int mi = (VARIANT == PRNU_PCE_REFERENCE) ? i : (h - i) % h;
+    const float2 x = X[(int64_t)mi * wh + (w - j)], y = Y[(int64_t)mi * wh + (w - j)];
+    return cmul(x, cconj(y));
+}
+
+// Hermitian part H(i, j) = (C(i, j) + conj(C(-i, -j)))/2; the textbook cross
+// spectrum of real patterns is Hermitian already.
+template <int VARIANT>
+__device__ __forceinline__ float2 herm_at(const float2 *X, const float2 *Y, int h, int w, int i, int j) {
+    const float2 c = cross_at<VARIANT>(X, Y, h, w, i, j);
+    if (VARIANT != PRNU_PCE_REFERENCE) return c;
+    const float2 m = cross_at<VARIANT>(X, Y, h, w, (h - i) % h, (w - j) % w);
+    return make_float2(0.5f * (c.x + m.x), 0.5f * (c.y - m.y));
+}
+
+// Row i of Z = H_a + i H_b for couple blockIdx.y (pairs 2q, 2q+1 of the
+// batch; the second may be absent), inverse row FFT.  grid (h, couples).
+template <int VARIANT>
+__global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h, int nb, PceBatch bt,
+                                                                float2 *__restrict__ Aall, int64_t pstride) {
+    extern __shared__ float2 buf[];
+    const int i = blockIdx.x, q = blockIdx.y;
+    const int w = d.n;
+    const float2 *Xa, *Ya, *Xb = nullptr, *Yb = nullptr;
+    pce_operands(bt, 2 * q, Xa, Ya);
+    const bool two = 2 * q + 1 < nb;
+    if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
     for (int j = threadIdx.x; j < w; j += kFftThreads) {
-        float2 c;
-        if (j < wh) {
-            const float2 x = X[(int64_t)i * wh + j], y = Y[(int64_t)i * wh + j];
-            c = cmul(cconj(x), y);  // (xRe*yRe + xIm*yIm, xRe*yIm - xIm*yRe)
-        } else {
-            const float2 x = X[(int64_t)mi * wh + (w - j)], y = Y[(int64_t)mi * wh + (w - j)];
-            c = cmul(x, cconj(y));
+        const float2 ha = herm_at<VARIANT>(Xa, Ya, h, w, i, j);
+        float2 z = ha;
+        if (two) {
+            const float2 hb = herm_at<VARIANT>(Xb, Yb, h, w, i, j);
+            z = make_float2(ha.x - hb.y, ha.y + hb.x);
         }
-        buf[j] = c;
+        buf[j] = z;
     }
     __syncthreads();
     fft_lds<1>(buf, d);
+    float2 *A = Aall + (int64_t)q * pstride;
     for (int j = threadIdx.x; j < w; j += kFftThreads) A[(int64_t)i * w + j] = buf[j];
 }
 
-// Row j of B is column j of the plane; inverse FFT over h.  grid (w, batch).
-__global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w, const float2 *__restrict__ Ball,
-                                                               float *__restrict__ Rall, PeakPart *__restrict__ partall,
-                                                               int64_t pstride, int64_t rstride) {
+// Columns [j0, j0 + sw) of couple blockIdx.y: rows of the strip are read
+// contiguously into column-major LDS, each column is inverse-transformed
+// (length h), and Re / Im go to the two pairs' correlation planes (stored
+// column-major, R[j*h + i]) with per-column energy and |value| argmax.
+// grid (ceil(w/sw), couples), LDS sw*h complex.
+__global__ __launch_bounds__(kFftThreads) void pce_cols2_kernel(FftDesc d, int w, int sw, int nb,
+                                                                const float2 *__restrict__ Aall, int64_t pstride,
+                                                                float *__restrict__ Rall, int64_t rstride,
+                                                                PeakPart *__restrict__ partall) {
     extern __shared__ float2 buf[];
     __shared__ double shd[4];
     __shared__ float shv[4];
     __shared__ int shi[4];
-    const int j = blockIdx.x;
-    const int h = d.n;
-    const float2 *B = Ball + (int64_t)blockIdx.y * pstride;
-    float *R = Rall + (int64_t)blockIdx.y * rstride;
-    for (int i = threadIdx.x; i < h; i += kFftThreads) buf[i] = B[(int64_t)j * h + i];
-    __syncthreads();
-    fft_lds<1>(buf, d);
-    double e = 0.0;
-    float best = -1.0f;
-    int bi = 0x7fffffff;
-    for (int i = threadIdx.x; i < h; i += kFftThreads) {
-        const float v = buf[i].x;
-        R[(int64_t)j * h + i] = v;
-        e += (double)v * (double)v;
-        argmax_merge(best, bi, fabsf(v), i * w + j);
+    const int h = d.n, q = blockIdx.y;
+    const int j0 = blockIdx.x * sw;
+    const int ncol = min(sw, w - j0);
+    const float2 *A = Aall + (int64_t)q * pstride;
+    for (int idx = threadIdx.x; idx < h * ncol; idx += kFftThreads) {
+        const int i = idx / ncol, c = idx - (idx / ncol) * ncol;
+        buf[c * h + i] = A[(int64_t)i * w + j0 + c];
     }
-    e = block_sum_256(e, shd);
-    block_argmax_256(best, bi, shv, shi);
-    if (threadIdx.x == 0) {
-        PeakPart p;
-        p.energy = e;
-        p.absmax = best;
-        p.index = bi;
-        partall[(int64_t)blockIdx.y * w + j] = p;
+    __syncthreads();
+    for (int c = 0; c < ncol; ++c) fft_lds<1>(buf + c * h, d);
+    const bool two = 2 * q + 1 < nb;
+    for (int c = 0; c < ncol; ++c) {
+        const int j = j0 + c;
+        for (int half = 0; half < (two ? 2 : 1); ++half) {
+            const int pair = 2 * q + half;
+            float *R = Rall + (int64_t)pair * rstride;
+            double e = 0.0;
+            float best = -1.0f;
+            int bi = 0x7fffffff;
+            for (int i = threadIdx.x; i < h; i += kFftThreads) {
+                const float2 z = buf[c * h + i];
+                const float v = half ? z.y : z.x;
+                R[(int64_t)j * h + i] = v;
+                e += (double)v * (double)v;
+                argmax_merge(best, bi, fabsf(v), i * w + j);
+            }
+            e = block_sum_256(e, shd);
+            block_argmax_256(best, bi, shv, shi);
+            if (threadIdx.x == 0) {
+                PeakPart p;
+                p.energy = e;
+                p.absmax = best;
+                p.index = bi;
+                partall[(int64_t)pair * w + j] = p;
+            }
+        }
     }
 }
 
@@ -218,31 +268,45 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
     }
 }
 
+// Columns per strip of the column pass: a power of two <= 16 with the strip
+// (sw * h complex) in 64 KiB of LDS.
+int pce_strip(int h) {
+    int sw = 16;
+    while (sw > 1 && (size_t)sw * h * sizeof(float2) > 65536) sw >>= 1;
+    return sw;
+}
+
 int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, hipStream_t st) {
     const int h = c->h, w = c->w;
     const int64_t n = c->pixels();
+    const int couples = (nb + 1) / 2;
     const size_t plane = ((sizeof(float2) * n) + 255) & ~size_t(255);
     const size_t rbytes = ((sizeof(float) * n) + 255) & ~size_t(255);
     const size_t pbytes = ((sizeof(PeakPart) * w) + 255) & ~size_t(255);
     const int64_t pstride = (int64_t)(plane / sizeof(float2)), rstride = (int64_t)(rbytes / sizeof(float));
-    PRNU_TRY(c->ws_pce.ensure((2 * plane + rbytes + pbytes) * nb));
+    PRNU_TRY(c->ws_pce.ensure(plane * couples + (rbytes + pbytes) * nb));
     char *base = static_cast<char *>(c->ws_pce.ptr);
     float2 *A = reinterpret_cast<float2 *>(base);
-    float2 *B = reinterpret_cast<float2 *>(base + plane * nb);
-    float *R = reinterpret_cast<float *>(base + 2 * plane * nb);
-    PeakPart *part = reinterpret_cast<PeakPart *>(base + (2 * plane + rbytes) * nb);
+    float *R = reinterpret_cast<float *>(base + plane * couples);
+    PeakPart *part = reinterpret_cast<PeakPart *>(base + plane * couples + rbytes * nb);
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
     if (variant == PRNU_PCE_REFERENCE)
-        hipLaunchKernelGGL(pce_rows_kernel<PRNU_PCE_REFERENCE>, dim3(h, nb), dim3(kFftThreads), sizeof(float2) * w,
-                           st, dw, h, bt, A, pstride);
+        hipLaunchKernelGGL(pce_rows2_kernel<PRNU_PCE_REFERENCE>, dim3(h, couples), dim3(kFftThreads),
+                           sizeof(float2) * w, st, dw, h, nb, bt, A, pstride);
     else
-        hipLaunchKernelGGL(pce_rows_kernel<PRNU_PCE_TEXTBOOK>, dim3(h, nb), dim3(kFftThreads), sizeof(float2) * w,
-                           st, dw, h, bt, A, pstride);
+        hipLaunchKernelGGL(pce_rows2_kernel<PRNU_PCE_TEXTBOOK>, dim3(h, couples), dim3(kFftThreads),
+                           sizeof(float2) * w, st, dw, h, nb, bt, A, pstride);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(A, B, h, w, nb, pstride, st));
-    hipLaunchKernelGGL(pce_cols_kernel, dim3(w, nb), dim3(kFftThreads), sizeof(float2) * h, st, dh, w, B, R, part,
-                       pstride, rstride);
+    const int sw = pce_strip(h);
+    static bool lds_attr = false;  // the strip may use the full 64 KiB of dynamic LDS
+    if (!lds_attr) {
+        PRNU_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&pce_cols2_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
+        lds_attr = true;
+    }
+    hipLaunchKernelGGL(pce_cols2_kernel, dim3((w + sw - 1) / sw, couples), dim3(kFftThreads),
+                       sizeof(float2) * sw * h, st, dh, w, sw, nb, A, pstride, R, rstride, part);
     PRNU_CHECK_LAUNCH();
     if (variant == PRNU_PCE_REFERENCE)
         hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_REFERENCE>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,
@@ -280,8 +344,8 @@ int run_pce_pairs(Context *c, int variant, const void *const *d_tab, const int *
     PRNU_TRY(ensure_fft_plans(c));
     if (batch <= 0) {
         // about 1 GiB of planes
-        const int64_t per = 20 * c->pixels();
-        batch = (int)std::max<int64_t>(1, std::min<int64_t>(256, (int64_t(1) << 30) / per));
+        const int64_t per = 12 * c->pixels();  // half a complex plane + a real plane per pair
+        batch = (int)std::max<int64_t>(2, std::min<int64_t>(512, (int64_t(1) << 30) / per)) & ~1;
     }
     time_begin(c, st);
     for (int b0 = 0; b0 < npairs; b0 += batch) {

@@ -222,7 +222,9 @@ def test_pce_pairs_batched_equals_single(variant, h, w, rng):
     fn = O.pce_reference if variant == _lib.PCE_REFERENCE else O.pce_textbook
     for q, (i, j) in enumerate(pairs):
         cmp.apply_gpu(spec[i], spec[j], one)
-        assert got[q] == one.item()                    # same kernels, same order: bitwise
+        # pairs share complex transforms two by two: equal to the single-pair
+        # result up to FP32 rounding of the shared transform
+        assert got[q] == pytest.approx(one.item(), rel=1e-5)
         assert got[q] == pytest.approx(fn(pats[i].astype(np.float64), pats[j].astype(np.float64)), rel=1e-4)
     cmp.cleanup()
 
