@@ -170,4 +170,65 @@ __device__ inline void fft_lds(float2 *buf, const FftDesc &d) {
     }
 }
 
+// Values held in LDS by one multi-transform call (cols * n), bounded so the
+// per-thread butterfly registers stay small.
+constexpr int kFftMultiMax = 4096;
+
+// One Stockham pass of radix R over `cols` independent transforms of length
+// n stored back to back (buf[c*n .. c*n+n)), all butterflies of all
+// transforms spread over the block: cols * n <= kFftMultiMax.
+template <int R, int DIR>
+__device__ __forceinline__ void fft_pass_multi(float2 *buf, int n, int cols, int Ns, const float2 *tw) {
+    constexpr int kMaxB = (kFftMultiMax / R + kFftThreads - 1) / kFftThreads;
+    const int nb = n / R;
+    const int total = nb * cols;
+    const int twstep = n / (Ns * R);
+    const int tid = threadIdx.x;
+    float2 v[kMaxB][R];
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int t = tid + b * kFftThreads;
+        if (t < total) {
+            const int c = t / nb, j = t - c * nb;
+            const int k = j % Ns;
+            const float2 *col = buf + c * n;
+            v[b][0] = col[j];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(col[j + r * nb], twiddle<DIR>(tw, r * k * twstep));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int t = tid + b * kFftThreads;
+        if (t < total) {
+            const int c = t / nb, j = t - c * nb;
+            dft<R, DIR>(v[b]);
+            const int k = j % Ns;
+            const int base = c * n + (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[base + q * Ns] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// `cols` in-LDS transforms of length d.n at buf + c*d.n (cols*d.n <= kFftMultiMax).
+template <int DIR>
+__device__ inline void fft_lds_multi(float2 *buf, const FftDesc &d, int cols) {
+    int Ns = 1;
+    for (int p = 0; p < d.passes; ++p) {
+        const int R = d.radix[p];
+        if (R == 4)
+            fft_pass_multi<4, DIR>(buf, d.n, cols, Ns, d.tw);
+        else if (R == 2)
+            fft_pass_multi<2, DIR>(buf, d.n, cols, Ns, d.tw);
+        else if (R == 3)
+            fft_pass_multi<3, DIR>(buf, d.n, cols, Ns, d.tw);
+        else
+            fft_pass_multi<5, DIR>(buf, d.n, cols, Ns, d.tw);
+        Ns *= R;
+    }
+}
+
 }  // namespace prnu

@@ -125,46 +125,57 @@ __device__ __forceinline__ float2 herm_at(const float2 *X, const float2 *Y, int 
     return make_float2(0.5f * (c.x + m.x), 0.5f * (c.y - m.y));
 }
 
-// Row i of Z = H_a + i H_b for couple blockIdx.y (pairs 2q, 2q+1 of the
-// batch; the second may be absent), inverse row FFT.  grid (h, couples).
-template <int VARIANT>
-__global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h, int nb, PceBatch bt,
-                                                                float2 *__restrict__ Aall, int64_t pstride) {
+// Rows [i0, i0 + rw) of Z = H_a + i H_b for couple blockIdx.y (pairs 2q,
+// 2q+1 of the batch; the second may be absent), inverse row FFTs in LDS.
+// grid (ceil(h/rw), couples), LDS rw*w complex.
+template <int VARIANT, bool BIG>
+__global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_rows2_kernel(FftDesc d, int h, int rw, int nb,
+                                                                             PceBatch bt, float2 *__restrict__ Aall,
+                                                                             int64_t pstride) {
     extern __shared__ float2 buf[];
-    const int i = blockIdx.x, q = blockIdx.y;
+    const int q = blockIdx.y, i0 = blockIdx.x * rw;
     const int w = d.n;
+    const int nrow = min(rw, h - i0);
     const float2 *Xa, *Ya, *Xb = nullptr, *Yb = nullptr;
     pce_operands(bt, 2 * q, Xa, Ya);
     const bool two = 2 * q + 1 < nb;
     if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
-    for (int j = threadIdx.x; j < w; j += kFftThreads) {
-        const float2 ha = herm_at<VARIANT>(Xa, Ya, h, w, i, j);
+    for (int idx = threadIdx.x; idx < nrow * w; idx += kFftThreads) {
+        const int r = idx / w, j = idx - (idx / w) * w;
+        const float2 ha = herm_at<VARIANT>(Xa, Ya, h, w, i0 + r, j);
         float2 z = ha;
         if (two) {
-            const float2 hb = herm_at<VARIANT>(Xb, Yb, h, w, i, j);
+            const float2 hb = herm_at<VARIANT>(Xb, Yb, h, w, i0 + r, j);
             z = make_float2(ha.x - hb.y, ha.y + hb.x);
         }
-        buf[j] = z;
+        buf[idx] = z;
     }
     __syncthreads();
-    fft_lds<1>(buf, d);
-    float2 *A = Aall + (int64_t)q * pstride;
-    for (int j = threadIdx.x; j < w; j += kFftThreads) A[(int64_t)i * w + j] = buf[j];
+    if constexpr (BIG)
+        fft_lds<1>(buf, d);  // one row longer than kFftMultiMax (rw == 1)
+    else
+        fft_lds_multi<1>(buf, d, nrow);
+    float2 *A = Aall + (int64_t)q * pstride + (int64_t)i0 * w;
+    for (int idx = threadIdx.x; idx < nrow * w; idx += kFftThreads) A[idx] = buf[idx];
 }
 
+constexpr int kMaxStrip = 16;
+
 // Columns [j0, j0 + sw) of couple blockIdx.y: rows of the strip are read
-// contiguously into column-major LDS, each column is inverse-transformed
-// (length h), and Re / Im go to the two pairs' correlation planes (stored
-// column-major, R[j*h + i]) with per-column energy and |value| argmax.
-// grid (ceil(w/sw), couples), LDS sw*h complex.
-__global__ __launch_bounds__(kFftThreads) void pce_cols2_kernel(FftDesc d, int w, int sw, int nb,
-                                                                const float2 *__restrict__ Aall, int64_t pstride,
-                                                                float *__restrict__ Rall, int64_t rstride,
-                                                                PeakPart *__restrict__ partall) {
+// contiguously into column-major LDS, the columns are inverse-transformed
+// together (length h), and Re / Im go to the two pairs' correlation planes
+// (column-major, R[j*h + i]) with per-column energy and |value| argmax.
+// grid (ceil(w/sw), couples), LDS sw*h complex, sw <= kMaxStrip.
+template <bool BIG>
+__global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_cols2_kernel(FftDesc d, int w, int sw, int nb,
+                                                                             const float2 *__restrict__ Aall,
+                                                                             int64_t pstride, float *__restrict__ Rall,
+                                                                             int64_t rstride,
+                                                                             PeakPart *__restrict__ partall) {
     extern __shared__ float2 buf[];
-    __shared__ double shd[4];
-    __shared__ float shv[4];
-    __shared__ int shi[4];
+    __shared__ double sh_e[2 * kMaxStrip][4];
+    __shared__ float sh_v[2 * kMaxStrip][4];
+    __shared__ int sh_i[2 * kMaxStrip][4];
     const int h = d.n, q = blockIdx.y;
     const int j0 = blockIdx.x * sw;
     const int ncol = min(sw, w - j0);
@@ -174,33 +185,53 @@ __global__ __launch_bounds__(kFftThreads) void pce_cols2_kernel(FftDesc d, int w
         buf[c * h + i] = A[(int64_t)i * w + j0 + c];
     }
     __syncthreads();
-    for (int c = 0; c < ncol; ++c) fft_lds<1>(buf + c * h, d);
-    const bool two = 2 * q + 1 < nb;
+    if constexpr (BIG)
+        fft_lds<1>(buf, d);  // one column longer than kFftMultiMax (sw == 1)
+    else
+        fft_lds_multi<1>(buf, d, ncol);
+    const int halves = 2 * q + 1 < nb ? 2 : 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int c = 0; c < ncol; ++c) {
         const int j = j0 + c;
-        for (int half = 0; half < (two ? 2 : 1); ++half) {
-            const int pair = 2 * q + half;
-            float *R = Rall + (int64_t)pair * rstride;
+        for (int hf = 0; hf < halves; ++hf) {
+            float *R = Rall + (int64_t)(2 * q + hf) * rstride;
             double e = 0.0;
             float best = -1.0f;
             int bi = 0x7fffffff;
             for (int i = threadIdx.x; i < h; i += kFftThreads) {
                 const float2 z = buf[c * h + i];
-                const float v = half ? z.y : z.x;
+                const float v = hf ? z.y : z.x;
                 R[(int64_t)j * h + i] = v;
                 e += (double)v * (double)v;
                 argmax_merge(best, bi, fabsf(v), i * w + j);
             }
-            e = block_sum_256(e, shd);
-            block_argmax_256(best, bi, shv, shi);
-            if (threadIdx.x == 0) {
-                PeakPart p;
-                p.energy = e;
-                p.absmax = best;
-                p.index = bi;
-                partall[(int64_t)pair * w + j] = p;
+            e = wave_sum_d(e);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const float v2 = __shfl_xor(best, off, 64);
+                const int i2 = __shfl_xor(bi, off, 64);
+                argmax_merge(best, bi, v2, i2);
+            }
+            if (lane == 0) {
+                sh_e[2 * c + hf][wave] = e;
+                sh_v[2 * c + hf][wave] = best;
+                sh_i[2 * c + hf][wave] = bi;
             }
         }
+    }
+    __syncthreads();
+    // one thread per (column, pair): fixed-order merge of the 4 waves
+    const int t = threadIdx.x;
+    if (t < 2 * ncol && (t & 1) < halves) {
+        const int c = t >> 1, hf = t & 1;
+        PeakPart p;
+        p.energy = (sh_e[t][0] + sh_e[t][1]) + (sh_e[t][2] + sh_e[t][3]);
+        float best = sh_v[t][0];
+        int bi = sh_i[t][0];
+        for (int k = 1; k < 4; ++k) argmax_merge(best, bi, sh_v[t][k], sh_i[t][k]);
+        p.absmax = best;
+        p.index = bi;
+        partall[(int64_t)(2 * q + hf) * w + j0 + c] = p;
     }
 }
 
@@ -268,13 +299,20 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
     }
 }
 
-// Columns per strip of the column pass: a power of two <= 16 with the strip
-// (sw * h complex) in 64 KiB of LDS.
+// Columns per strip of the column pass and rows per workgroup of the row
+// pass: powers of two with the transforms of one workgroup (<= kFftMultiMax
+// values, 32 KiB) transformed together; a transform longer than that runs
+// alone through fft_lds (the BIG kernels, up to kFftMaxN).
 int pce_strip(int h) {
-    int sw = 16;
-    while (sw > 1 && (size_t)sw * h * sizeof(float2) > 65536) sw >>= 1;
+    int sw = kMaxStrip;
+    while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
     return sw;
 }
+// The row pass reads the half spectra (two rows each for the reference
+// variant's Hermitian part) and is bound by those reads; one row per
+// workgroup at ~107 VGPRs keeps 4 waves per SIMD in flight, which measured
+// faster than grouping rows (184 VGPRs, 2 waves per SIMD).
+int pce_rows_per_block(int) { return 1; }
 
 int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, hipStream_t st) {
     const int h = c->h, w = c->w;
@@ -291,22 +329,30 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     PeakPart *part = reinterpret_cast<PeakPart *>(base + plane * couples + rbytes * nb);
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    if (variant == PRNU_PCE_REFERENCE)
-        hipLaunchKernelGGL(pce_rows2_kernel<PRNU_PCE_REFERENCE>, dim3(h, couples), dim3(kFftThreads),
-                           sizeof(float2) * w, st, dw, h, nb, bt, A, pstride);
-    else
-        hipLaunchKernelGGL(pce_rows2_kernel<PRNU_PCE_TEXTBOOK>, dim3(h, couples), dim3(kFftThreads),
-                           sizeof(float2) * w, st, dw, h, nb, bt, A, pstride);
+    const int rw = pce_rows_per_block(w);
+    const bool big_w = true, big_h = h > kFftMultiMax;  // rows: one per workgroup through fft_lds
+    const dim3 rgrid((h + rw - 1) / rw, couples);
+    const size_t rsm = sizeof(float2) * rw * w;
+#define PRNU_PCE_ROWS(V, B) \
+    hipLaunchKernelGGL((pce_rows2_kernel<V, B>), rgrid, dim3(kFftThreads), rsm, st, dw, h, rw, nb, bt, A, pstride)
+    if (variant == PRNU_PCE_REFERENCE) {
+        if (big_w) PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, true);
+        else PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, false);
+    } else {
+        if (big_w) PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, true);
+        else PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, false);
+    }
+#undef PRNU_PCE_ROWS
     PRNU_CHECK_LAUNCH();
     const int sw = pce_strip(h);
-    static bool lds_attr = false;  // the strip may use the full 64 KiB of dynamic LDS
-    if (!lds_attr) {
-        PRNU_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&pce_cols2_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
-        lds_attr = true;
-    }
-    hipLaunchKernelGGL(pce_cols2_kernel, dim3((w + sw - 1) / sw, couples), dim3(kFftThreads),
-                       sizeof(float2) * sw * h, st, dh, w, sw, nb, A, pstride, R, rstride, part);
+    const dim3 cgrid((w + sw - 1) / sw, couples);
+    const size_t csm = sizeof(float2) * sw * h;  // <= 64 KiB (one column of up to kFftMaxN)
+    if (big_h)
+        hipLaunchKernelGGL((pce_cols2_kernel<true>), cgrid, dim3(kFftThreads), csm, st, dh, w, sw, nb, A, pstride, R,
+                           rstride, part);
+    else
+        hipLaunchKernelGGL((pce_cols2_kernel<false>), cgrid, dim3(kFftThreads), csm, st, dh, w, sw, nb, A, pstride, R,
+                           rstride, part);
     PRNU_CHECK_LAUNCH();
     if (variant == PRNU_PCE_REFERENCE)
         hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_REFERENCE>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,

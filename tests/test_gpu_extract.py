@@ -223,8 +223,11 @@ def test_pce_pairs_batched_equals_single(variant, h, w, rng):
     for q, (i, j) in enumerate(pairs):
         cmp.apply_gpu(spec[i], spec[j], one)
         # pairs share complex transforms two by two: equal to the single-pair
-        # result up to FP32 rounding of the shared transform
-        assert got[q] == pytest.approx(one.item(), rel=1e-5)
+        # result up to FP32 rounding of the shared transform.  That rounding
+        # moves the peak by delta ~1e-6 of the plane's rms, so the PCE
+        # (peak^2 / energy) moves by ~2 delta sqrt(PCE): a relative bound for
+        # large PCE, a sqrt bound for noise-level ones
+        assert got[q] == pytest.approx(one.item(), rel=1e-5, abs=2e-5 * np.sqrt(abs(one.item())))
         assert got[q] == pytest.approx(fn(pats[i].astype(np.float64), pats[j].astype(np.float64)), rel=1e-4)
     cmp.cleanup()
 
