@@ -261,8 +261,8 @@ int prnu_gram_set_config(prnu_ctx *ctx, int precision, const prnu_gram_config *c
         return PRNU_OK;
     }
     PRNU_REQUIRE(cfg->slabs >= 0 && (cfg->xcd_remap == 0 || cfg->xcd_remap == 1) && cfg->variant >= 0 &&
-                     cfg->variant <= 2,
-                 "gram_set_config: slabs must be >= 0, xcd_remap 0 or 1, variant 0, 1 or 2");
+                     cfg->variant <= 3,
+                 "gram_set_config: slabs must be >= 0, xcd_remap 0 or 1, variant 0 to 3");
     c->gram_cfg[precision] = *cfg;
     return PRNU_OK;
 }

@@ -511,6 +511,167 @@ __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_byte
     gram_item<FMT, VAR>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
 }
 
+// ------------------------------------------------ 4-wave split-BF16 item
+// Same tile and LDS stage as gram_item, run by 4 waves (256 threads) of
+// 128 x 128 each: a wave holds 8 x 8 accumulator tiles (256 registers, in
+// AGPRs) with the full 512-register file of one wave per SIMD, and reads
+// 32 fragments per stage for 192 MFMAs (the 8-wave layout reads 24 for 96).
+// Diagonal tiles need 3 of their 4 wave blocks, (0,0) (0,1) (1,1); pack p
+// gives blocks 4p .. 4p+3 of the 3*nt to its waves (at most two diagonal
+// tiles per pack).
+constexpr int kThr4 = 256;
+
+__device__ __forceinline__ ItemWaves item_waves4(int u, int nt, int wave) {
+    ItemWaves r;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int toff = nt * (nt - 1) / 2;
+    if (u < toff) {
+        int I = 0, t = u;
+        while (t >= nt - 1 - I) {
+            t -= nt - 1 - I;
+            ++I;
+        }
+        const int J = I + 1 + t;
+        r.rows_x = I;
+        r.rows_y = J;
+        r.a_off = wr * 128;
+        r.b_off = kTileS + wc * 128;
+        r.out_tile = tri_index(I, J, nt);
+        r.out_off = wr * 128 * kTileS + wc * 128;
+        return r;
+    }
+    const int p = u - toff;
+    const int d0 = (4 * p) / 3;
+    const int dl = min(nt - 1, (4 * p + 3) / 3);
+    r.rows_x = d0;
+    r.rows_y = dl > d0 ? d0 + 1 : -1;
+    const int e = 4 * p + wave;
+    if (e >= 3 * nt) {
+        r.a_off = r.b_off = 0;
+        r.out_tile = -1;
+        r.out_off = 0;
+        return r;
+    }
+    const int d = e / 3, l = e - 3 * (e / 3);
+    const int bwr = l == 2 ? 1 : 0, bwc = l == 0 ? 0 : 1;
+    const int base = (d - d0) * kTileS;
+    r.a_off = base + bwr * 128;
+    r.b_off = base + bwc * 128;
+    r.out_tile = tri_index(d, d, nt);
+    r.out_off = bwr * 128 * kTileS + bwc * 128;
+    return r;
+}
+
+__global__ __launch_bounds__(kThr4, 1) void store_gram4_kernel(StoreGramArgs a) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
+    const int bid = blockIdx.x;
+    int u, sl;
+    if (a.xcd_remap) {
+        const int local = bid >> 3;
+        u = local % a.T;
+        sl = (local / a.T) * 8 + (bid & 7);
+    } else {
+        u = bid % a.T;
+        sl = bid / a.T;
+    }
+    if (sl >= a.S) return;
+    ItemWaves w = item_waves4(u, a.nt, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+    const int rows_x = __builtin_amdgcn_readfirstlane(w.rows_x);
+    const int rows_y = __builtin_amdgcn_readfirstlane(w.rows_y);
+    const int a_off = __builtin_amdgcn_readfirstlane(w.a_off >> 6) << 6;
+    const int b_off = __builtin_amdgcn_readfirstlane(w.b_off >> 6) << 6;
+    const int out_tile = __builtin_amdgcn_readfirstlane(w.out_tile);
+    const int out_off = __builtin_amdgcn_readfirstlane(w.out_off);
+    const int64_t b_begin = (int64_t)sl * a.kcb;
+    const int64_t b_end = min(a.nkb, b_begin + a.kcb);
+    const bool half = rows_y < 0;
+    const int ty = half ? rows_x : rows_y;
+    const int tid = threadIdx.x, lane = tid & 63;
+
+    // direct-to-LDS loads: thread -> rows (tid >> 3) + 32*it, it < 16; one
+    // wave instruction fills 8 consecutive rows (1 KiB)
+    const int r0 = tid >> 3, chunk = tid & 7;
+    const int dchunk = chunk ^ (r0 & 7);
+    // byte offsets inside one block column fit 32 bits (capacity * 128 < 2^31)
+    auto row_off = [&](int it) -> uint32_t {
+        const int r = r0 + 32 * it;
+        const int loc = r < kTileS ? rows_x * kTileS + r : ty * kTileS + (r - kTileS);
+        return (uint32_t)(a.first + min(loc, a.count - 1)) * (uint32_t)kBlkBytes;
+    };
+    // buffer loads with 32-bit per-row offsets: the resource points at the
+    // block column of the stage (scalar), the offsets stay fixed
+    uint32_t voff[16];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) voff[it] = row_off(it) + (uint32_t)(dchunk << 4);
+    const int wbase = (tid >> 6) * 8;
+    auto load_to = [&](int64_t b, uint8_t *buf) {
+        __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(a.blocks + b * a.cap_bytes), (short)0,
+                                              (int)(a.cap_bytes), 0x00020000);
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            if (it < 8 || !half) {
+                typedef __attribute__((address_space(3))) void *lptr;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lptr)(buf + (32 * it + wbase) * kBlkBytes), 16,
+                                                         voff[it], 0, 0, 0);
+            }
+        }
+    };
+
+    floatx4 acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const bool active = out_tile >= 0;
+    const int frow = lane & 15, ch = lane >> 4;
+    auto compute = [&](const uint8_t *buf) {
+        bf16x8 bh[8], bl[8];
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const int r = b_off + cb * 16 + frow;
+            bh[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
+            bl[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+        }
+#pragma unroll
+        for (int rb = 0; rb < 8; ++rb) {
+            const int r = a_off + rb * 16 + frow;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+            }
+        }
+    };
+
+    const int64_t nst = b_end - b_begin;
+    load_to(b_begin, lds);
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+    __syncthreads();
+    for (int64_t q = 0; q < nst; ++q) {
+        const bool next = q + 1 < nst;
+        if (next) load_to(b_begin + q + 1, lds + ((q + 1) & 1) * kBufS);
+        // idle waves (a partial diagonal pack) compute too and write nothing:
+        // a branch around the MFMAs makes the compiler copy the AGPR
+        // accumulators through VGPRs every stage
+        compute(lds + (q & 1) * kBufS);
+        if (next) __builtin_amdgcn_s_waitcnt(0x0f70);
+        __syncthreads();
+    }
+    if (!active) return;
+    float *out = a.partials + ((int64_t)out_tile * a.S + sl) * (kTileS * kTileS) + out_off;
+    const int ocol = lane & 15, orow = (lane >> 4) * 4;
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[(rb * 16 + orow + r) * kTileS + cb * 16 + ocol] = acc[rb][cb][r];
+}
+
 // One launch = one work item per workgroup.  With xcd_remap, the items of
 // a slab run on one XCD (workgroup ids are dealt to the 8 XCDs round-robin).
 template <int FMT, int VAR>
@@ -743,7 +904,17 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const int64_t S8 = a.xcd_remap ? (a.S + 7) / 8 * 8 : a.S;
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
-    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 direct-to-LDS + priority
+    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 + priority, 3 four waves (split)
+    if (var == 3 && s->format == PRNU_PREC_SPLIT_BF16) {
+        // same items per slab as the 8-wave packing (ceil(3nt/4) = ceil(6nt/8))
+        hipLaunchKernelGGL(store_gram4_kernel, grid, dim3(kThr4), 0, st, a);
+        PRNU_CHECK_LAUNCH();
+        time_end(c, st);
+        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials,
+                           a.S, a.nt, count, s->sum + first, s->sumsq + first, nullptr, d_ncc);
+        PRNU_CHECK_LAUNCH();
+        return PRNU_OK;
+    }
 #define PRNU_STORE_GRAM(F)                                                                        \
     do {                                                                                          \
         if (var == 1)                                                                             \

@@ -116,12 +116,14 @@ def test_plan_rejects_invalid(groups):
         csi.GramPlan(gram, [])
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("fmt", ["split_bf16", "fp32"])
 def test_store_gram_load_variants(fmt, variant):
-    """Store Gram load variants (1: register-staged loads, 2: direct loads
-    with raised priority) against the oracle and bitwise against the default
-    (direct global-to-LDS loads); ragged count, offset slot range."""
+    """Store Gram variants (1: register-staged loads, 2: direct loads with
+    raised priority, 3: four waves of 128 x 128 for split-BF16) against the
+    oracle and bitwise against the default (eight waves, direct global-to-LDS
+    loads): every output element sees the same MFMA sequence; ragged count,
+    offset slot range."""
     import ctypes
     from rocket_amd import _lib
     n, count, cap, first = 120 * 97, 300, 310, 5
