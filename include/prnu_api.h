@@ -256,6 +256,25 @@ PRNU_EXPORT int prnu_gram_plan_run(prnu_gram_plan *plan, int use_graph, prnu_str
  * the fastest; drops a captured graph. */
 PRNU_EXPORT int prnu_gram_plan_autotune(prnu_gram_plan *plan, prnu_stream stream, int64_t *best_slab_values,
                                         float *best_ms);
+/* A block of a plan: NCC of slots [row_first, row_first + row_count) against
+ * [col_first, col_first + col_count) of one store into d_out (row-major,
+ * leading dimension ld_out).  Identical ranges make a triangle block: the
+ * symmetric count x count matrix, both triangles written, as
+ * prnu_store_gram_ncc writes it.  Other blocks are rectangles (no mirror
+ * written), e.g. the leaves of an index-space
+ * division (R/indexspace/HilbertCurveIndexSpace.java:92-123) of the pair
+ * triangle, so a host scheduler can hand a whole division to one launch. */
+typedef struct prnu_gram_block {
+    prnu_store *store;
+    int32_t row_first, row_count;
+    int32_t col_first, col_count;
+    double *d_out;
+    int64_t ld_out;
+} prnu_gram_block;
+/* Plan over up to 4096 blocks of stores of one format; otherwise as
+ * prnu_gram_plan_create. */
+PRNU_EXPORT int prnu_gram_plan_create_blocks(prnu_ctx *ctx, const prnu_gram_block *blocks, int nblocks,
+                                             int64_t slab_values, prnu_gram_plan **out);
 /* Work items, workgroups launched, slab length, partial bytes and the
  * algorithmic FLOP of one product over every group's triangle (diagonal
  * included). */

@@ -84,6 +84,7 @@ _SIGNATURES = {
                                   ctypes.POINTER(_c_void_p)]),
     "prnu_store_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p]),
     "prnu_gram_plan_create": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_i64, ctypes.POINTER(_c_void_p)]),
+    "prnu_gram_plan_create_blocks": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_i64, ctypes.POINTER(_c_void_p)]),
     "prnu_gram_plan_destroy": (_c_int, [_c_void_p]),
     "prnu_gram_plan_run": (_c_int, [_c_void_p, _c_int, _c_void_p]),
     "prnu_gram_plan_autotune": (_c_int, [_c_void_p, _c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(ctypes.c_float)]),

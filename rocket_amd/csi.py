@@ -467,22 +467,47 @@ class _GramGroup(ctypes.Structure):
                 ("d_ncc", ctypes.c_void_p)]
 
 
+class _GramBlock(ctypes.Structure):
+    _fields_ = [("store", ctypes.c_void_p), ("row_first", ctypes.c_int32), ("row_count", ctypes.c_int32),
+                ("col_first", ctypes.c_int32), ("col_count", ctypes.c_int32), ("d_out", ctypes.c_void_p),
+                ("ld_out", ctypes.c_int64)]
+
+
 class GramPlan:
     """All-pairs NCC of several pattern groups (e.g. one per image size) in one
     launch (``prnu_gram_plan_*``).  ``groups`` is a list of
     ``(PatternStore, first_slot, count)``; outputs are (count, count) FP64
-    CUDA tensors, one per group, in :attr:`outputs`."""
+    CUDA tensors, one per group, in :attr:`outputs`.
 
-    def __init__(self, gram_ctx: "GramNCC", groups, slab_values: int = 0):
+    With ``blocks`` instead, a list of ``(PatternStore, row_first, row_count,
+    col_first, col_count, out)`` where ``out`` is a 2-D FP64 CUDA tensor view
+    (row stride = leading dimension): each block's NCC goes into its view
+    (``prnu_gram_plan_create_blocks``)."""
+
+    def __init__(self, gram_ctx: "GramNCC", groups=None, slab_values: int = 0, blocks=None):
         lib = _lib.load()
-        self._keep = [g[0] for g in groups]          # stores must outlive the plan
-        self.outputs = [torch.empty((int(c), int(c)), dtype=torch.float64, device="cuda") for _, _, c in groups]
-        arr = (_GramGroup * len(groups))()
-        for i, ((store, first, count), out) in enumerate(zip(groups, self.outputs)):
-            arr[i] = _GramGroup(store.handle.value, int(first), int(count), out.data_ptr())
         h = ctypes.c_void_p()
-        _lib.check(lib.prnu_gram_plan_create(gram_ctx._ctx.handle, arr, len(groups), int(slab_values),
-                                             ctypes.byref(h)), "prnu_gram_plan_create")
+        if blocks is None:
+            self._keep = [g[0] for g in groups]          # stores must outlive the plan
+            self.outputs = [torch.empty((int(c), int(c)), dtype=torch.float64, device="cuda") for _, _, c in groups]
+            arr = (_GramGroup * len(groups))()
+            for i, ((store, first, count), out) in enumerate(zip(groups, self.outputs)):
+                arr[i] = _GramGroup(store.handle.value, int(first), int(count), out.data_ptr())
+            _lib.check(lib.prnu_gram_plan_create(gram_ctx._ctx.handle, arr, len(groups), int(slab_values),
+                                                 ctypes.byref(h)), "prnu_gram_plan_create")
+        else:
+            self._keep = [b[0] for b in blocks]
+            self.outputs = [b[5] for b in blocks]
+            arr = (_GramBlock * len(blocks))()
+            for i, (store, r0, rn, c0, cn, out) in enumerate(blocks):
+                if out.dtype != torch.float64 or not out.is_cuda or out.stride(1) != 1:
+                    raise ValueError("block outputs must be FP64 CUDA views with unit column stride")
+                if out.shape[0] < rn or out.shape[1] < cn:
+                    raise ValueError("block output view smaller than the block")
+                arr[i] = _GramBlock(store.handle.value, int(r0), int(rn), int(c0), int(cn), out.data_ptr(),
+                                    int(out.stride(0)))
+            _lib.check(lib.prnu_gram_plan_create_blocks(gram_ctx._ctx.handle, arr, len(blocks), int(slab_values),
+                                                        ctypes.byref(h)), "prnu_gram_plan_create_blocks")
         self.handle = h
         self._ctx = gram_ctx
 

@@ -48,7 +48,7 @@ constexpr int kBlkBytes = 128;  // bytes per block
 constexpr int kTileS = 256;
 constexpr int kRowsS = 2 * kTileS;
 constexpr int kThrS = 512;
-constexpr int kMaxPlanGroups = 64;
+constexpr int kMaxPlanGroups = 4096;
 
 __device__ inline uint16_t bf16_bits_s(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 __device__ inline float bf16_val_s(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
@@ -307,8 +307,8 @@ __device__ __forceinline__ int items_per_slab(int nt) { return nt * (nt - 1) / 2
 // 2 = direct-to-LDS loads and raised wave priority over the MFMA block.
 template <int FMT, int VAR>
 __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, int64_t cap_bytes, int first, int count,
-                                          const ItemWaves &w, int64_t b_begin, int64_t b_end, float *__restrict__ out,
-                                          uint8_t *lds) {
+                                          int first_b, int count_b, const ItemWaves &w, int64_t b_begin,
+                                          int64_t b_end, float *__restrict__ out, uint8_t *lds) {
     constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
     constexpr bool kFp8 = FMT == PRNU_PREC_FP8;
     constexpr bool DMA = VAR != 1;
@@ -328,13 +328,16 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
         const int r = r0 + 64 * it;
-        const int loc = (r < kTileS) ? tile.x * kTileS + r : tile.y * kTileS + (r - kTileS);
-        valid |= (uint32_t)(r < nrows && loc < count) << it;
+        const bool in_a = r < kTileS;
+        const int loc = in_a ? tile.x * kTileS + r : tile.y * kTileS + (r - kTileS);
+        valid |= (uint32_t)(r < nrows && loc < (in_a ? count : count_b)) << it;
     }
-    // per-row byte offsets inside one block column (rows clamped into the set)
+    // per-row byte offsets inside one block column (rows clamped into their
+    // range: LDS rows [0,256) come from slots [first, first+count), rows
+    // [256,512) from [first_b, first_b+count_b))
     auto row_off = [&](int it) -> int64_t {
         if (it < 4) return (int64_t)(first + min(tile.x * kTileS + r0 + 64 * it, count - 1)) * kBlkBytes;
-        return (int64_t)(first + min(tile.y * kTileS + r0 + 64 * (it - 4), count - 1)) * kBlkBytes;
+        return (int64_t)(first_b + min(tile.y * kTileS + r0 + 64 * (it - 4), count_b - 1)) * kBlkBytes;
     };
 
     floatx4 acc[8][4];
@@ -492,12 +495,28 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 }
 
 // Run item u of slab s: partial tile t of slab s lives at partials + (t*S + s) * 256*256.
+// ntb > 0: a rectangle of nt x ntb tiles between slots [first, first+count)
+// (rows) and [first_b, first_b+count_b) (columns), item u = tile
+// (u / ntb, u % ntb); ntb == 0: the triangle of [first, first+count).
+__device__ __forceinline__ ItemWaves rect_waves(int u, int ntb, int wave) {
+    ItemWaves r;
+    const int wr = wave >> 2, wc = wave & 3;
+    r.rows_x = u / ntb;
+    r.rows_y = u - (u / ntb) * ntb;
+    r.a_off = wr * 128;
+    r.b_off = kTileS + wc * 64;
+    r.out_tile = u;
+    r.out_off = wr * 128 * kTileS + wc * 64;
+    return r;
+}
+
 template <int FMT, int VAR>
-__device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int nt,
-                                         int S, int64_t nkb, int64_t kcb, float *partials, int u, int s,
-                                         uint8_t *lds) {
+__device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int first_b,
+                                         int count_b, int nt, int ntb, int S, int64_t nkb, int64_t kcb,
+                                         float *partials, int u, int s, uint8_t *lds) {
     // wave-uniform: keep it in scalar registers
-    ItemWaves w = item_waves(u, nt, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    ItemWaves w = ntb > 0 ? rect_waves(u, ntb, wave) : item_waves(u, nt, wave);
     w.rows_x = __builtin_amdgcn_readfirstlane(w.rows_x);
     w.rows_y = __builtin_amdgcn_readfirstlane(w.rows_y);
     w.a_off = __builtin_amdgcn_readfirstlane(w.a_off >> 6) << 6;  // multiples of 64 rows: the swizzle
@@ -508,7 +527,7 @@ __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_byte
     const int64_t b_end = min(nkb, b_begin + kcb);
     float *out = w.out_tile < 0 ? nullptr
                                 : partials + ((int64_t)w.out_tile * S + s) * (kTileS * kTileS) + w.out_off;
-    gram_item<FMT, VAR>(blocks, cap_bytes, first, count, w, b_begin, b_end, out, lds);
+    gram_item<FMT, VAR>(blocks, cap_bytes, first, count, first_b, count_b, w, b_begin, b_end, out, lds);
 }
 
 // ------------------------------------------------ 4-wave split-BF16 item
@@ -688,7 +707,8 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
         s = bid / a.T;
     }
     if (s >= a.S) return;
-    run_item<FMT, VAR>(a.blocks, a.cap_bytes, a.first, a.count, a.nt, a.S, a.nkb, a.kcb, a.partials, u, s, lds);
+    run_item<FMT, VAR>(a.blocks, a.cap_bytes, a.first, a.count, a.first, a.count, a.nt, 0, a.S, a.nkb, a.kcb,
+                       a.partials, u, s, lds);
 }
 
 // ------------------------------------------------------------ grouped plan
@@ -706,7 +726,8 @@ struct GroupDev {
     const uint8_t *blocks;
     float *partials;
     int64_t cap_bytes, nkb, kcb, v_begin;
-    int first, count, nt, T, S, pad;
+    int first, count, nt, T, S;
+    int first_b, count_b, ntb;  // ntb > 0: rectangle rows x [first_b, first_b+count_b)
 };
 
 template <int FMT, int VAR>
@@ -714,15 +735,20 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
                                                                     int ngroups) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
     const int64_t v = blockIdx.x;
-    int gi = 0;
-    while (gi + 1 < ngroups && v >= groups[gi + 1].v_begin) ++gi;
-    const GroupDev &g = groups[gi];
+    int lo = 0, hi = ngroups - 1;  // last group with v_begin <= v
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (groups[mid].v_begin <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    const GroupDev &g = groups[lo];
     const int64_t vl = v - g.v_begin;
     const int local = (int)(vl >> 3);
     const int u = local % g.T;
     const int s = (local / g.T) * 8 + (int)(vl & 7);
     if (s >= g.S) return;  // padding item
-    run_item<FMT, VAR>(g.blocks, g.cap_bytes, g.first, g.count, g.nt, g.S, g.nkb, g.kcb, g.partials, u, s, lds);
+    run_item<FMT, VAR>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.nt, g.ntb, g.S, g.nkb,
+                       g.kcb, g.partials, u, s, lds);
 }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
@@ -730,7 +756,7 @@ __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restri
                                                            int count, const double *__restrict__ sum,
                                                            const double *__restrict__ sumsq,
                                                            const float *__restrict__ scale,
-                                                           double *__restrict__ out) {
+                                                           double *__restrict__ out, int64_t ld) {
     const int t = blockIdx.y;
     const int e = blockIdx.x * 256 + threadIdx.x;
     const int2 tile = tri_tile_s(t, nt);
@@ -745,8 +771,34 @@ __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restri
     const double di = sqrt(sumsq[i] - sum[i] * sum[i]);
     const double dj = sqrt(sumsq[j] - sum[j] * sum[j]);
     const double v = g / (di * dj);
-    out[(int64_t)i * count + j] = v;
-    out[(int64_t)j * count + i] = v;
+    out[(int64_t)i * ld + j] = v;
+    out[(int64_t)j * ld + i] = v;
+}
+
+// Rectangle: element (i, j) of tile t = (I, J) is rows i of [first_a..) and
+// column j of [first_b..); no mirroring; out[i * ld + j].
+__global__ __launch_bounds__(256) void store_reduce_rect_kernel(const float *__restrict__ partials, int S, int ntb,
+                                                                int count_a, int count_b,
+                                                                const double *__restrict__ sum_a,
+                                                                const double *__restrict__ sumsq_a,
+                                                                const double *__restrict__ sum_b,
+                                                                const double *__restrict__ sumsq_b,
+                                                                const float *__restrict__ scale_a,
+                                                                const float *__restrict__ scale_b,
+                                                                double *__restrict__ out, int64_t ld) {
+    const int t = blockIdx.y;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int I = t / ntb, J = t - (t / ntb) * ntb;
+    const int i = I * kTileS + (e >> 8);
+    const int j = J * kTileS + (e & 255);
+    if (i >= count_a || j >= count_b) return;
+    const float *p = partials + (int64_t)t * S * (kTileS * kTileS) + e;
+    double g = 0.0;
+    for (int q = 0; q < S; ++q) g += (double)p[(int64_t)q * (kTileS * kTileS)];
+    if (scale_a) g *= (double)scale_a[i] * (double)scale_b[j];
+    const double di = sqrt(sumsq_a[i] - sum_a[i] * sum_a[i]);
+    const double dj = sqrt(sumsq_b[j] - sum_b[j] * sum_b[j]);
+    out[(int64_t)i * ld + j] = g / (di * dj);
 }
 
 }  // namespace
@@ -911,7 +963,7 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
         PRNU_CHECK_LAUNCH();
         time_end(c, st);
         hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials,
-                           a.S, a.nt, count, s->sum + first, s->sumsq + first, nullptr, d_ncc);
+                           a.S, a.nt, count, s->sum + first, s->sumsq + first, nullptr, d_ncc, (int64_t)count);
         PRNU_CHECK_LAUNCH();
         return PRNU_OK;
     }
@@ -934,16 +986,31 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     PRNU_CHECK_LAUNCH();
     time_end(c, st);
     hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials, a.S,
-                       a.nt, count, s->sum + first, s->sumsq + first, s->scale ? s->scale + first : nullptr, d_ncc);
+                       a.nt, count, s->sum + first, s->sumsq + first, s->scale ? s->scale + first : nullptr, d_ncc,
+                       (int64_t)count);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
 
 // ------------------------------------------------------------ grouped plan
+// One block of a plan: the triangle of a slot range (tri) or the rectangle
+// rows x cols of two slot ranges of one store.
+struct PlanBlock {
+    const Store *store;
+    int row_first, row_count, col_first, col_count;
+    double *out;
+    int64_t ld;
+    bool tri;
+    int nta() const { return (row_count + kTileS - 1) / kTileS; }
+    int ntb() const { return (col_count + kTileS - 1) / kTileS; }
+    int items() const { return tri ? host_items_per_slab(nta()) : nta() * ntb(); }
+    int tiles() const { return tri ? nta() * (nta() + 1) / 2 : nta() * ntb(); }
+};
+
 struct GramPlan {
     Context *c = nullptr;
     int format = PRNU_PREC_SPLIT_BF16;
-    std::vector<prnu_gram_group> groups;
+    std::vector<PlanBlock> groups;
     std::vector<GroupDev> gd;        // host copy of the device descriptors
     GroupDev *d_groups = nullptr;
     float *partials = nullptr;
@@ -952,7 +1019,7 @@ struct GramPlan {
     int64_t v_total = 0, items = 0;  // workgroups launched, work items (padding excluded)
     int cus = 0;
     int var = 0;                     // load variant (the context's config of the format)
-    double flop = 0.0;               // one product over every group's triangle (incl. diagonal)
+    double flop = 0.0;               // one product over every block (triangles incl. diagonal)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -976,24 +1043,26 @@ int plan_layout(GramPlan *p, int64_t kcb) {
     std::vector<GroupDev> gd(p->groups.size());
     int64_t v = 0, items = 0, part = 0;
     for (size_t i = 0; i < p->groups.size(); ++i) {
-        const prnu_gram_group &g = p->groups[i];
-        const Store *s = reinterpret_cast<const Store *>(g.store);
+        const PlanBlock &g = p->groups[i];
+        const Store *s = g.store;
         GroupDev &d = gd[i];
         d.blocks = s->blocks;
         d.cap_bytes = (int64_t)s->cap_pad * kBlkBytes;
         d.nkb = s->nkb;
         d.kcb = std::min<int64_t>(kcb, s->nkb);
-        d.first = g.first_slot;
-        d.count = g.count;
-        d.nt = (g.count + kTileS - 1) / kTileS;
-        d.T = host_items_per_slab(d.nt);
+        d.first = g.row_first;
+        d.count = g.row_count;
+        d.nt = g.nta();
+        d.first_b = g.tri ? g.row_first : g.col_first;
+        d.count_b = g.tri ? g.row_count : g.col_count;
+        d.ntb = g.tri ? 0 : g.ntb();
+        d.T = g.items();
         d.S = (int)((s->nkb + d.kcb - 1) / d.kcb);
-        d.pad = 0;
         d.v_begin = v;
         d.partials = reinterpret_cast<float *>((uintptr_t)part);  // offset for now
         v += (int64_t)d.T * ((d.S + 7) / 8 * 8);
         items += (int64_t)d.T * d.S;
-        part += tile_elems * sizeof(float) * (d.nt * (d.nt + 1) / 2) * d.S;
+        part += tile_elems * sizeof(float) * g.tiles() * d.S;
     }
     PRNU_REQUIRE(v < (int64_t(1) << 31), "gram_plan: %lld work items; use longer slabs", (long long)v);
     PRNU_REQUIRE((size_t)part <= kPlanPartialCap,
@@ -1026,12 +1095,10 @@ int plan_layout(GramPlan *p, int64_t kcb) {
 int64_t plan_default_kcb(const GramPlan *p) {
     std::vector<int> items, tiles;
     std::vector<int64_t> nkb;
-    for (const prnu_gram_group &g : p->groups) {
-        const Store *s = reinterpret_cast<const Store *>(g.store);
-        const int nt = (g.count + kTileS - 1) / kTileS;
-        items.push_back(host_items_per_slab(nt));
-        tiles.push_back(nt * (nt + 1) / 2);
-        nkb.push_back(s->nkb);
+    for (const PlanBlock &g : p->groups) {
+        items.push_back(g.items());
+        tiles.push_back(g.tiles());
+        nkb.push_back(g.store->nkb);
     }
     return choose_kcb(items.data(), tiles.data(), nkb.data(), (int)items.size(), p->cus, kPlanPartialCap);
 }
@@ -1058,10 +1125,18 @@ int plan_launch(GramPlan *p, hipStream_t st) {
     PRNU_CHECK_LAUNCH();
     for (int i = 0; i < ng; ++i) {
         const GroupDev &d = p->gd[i];
-        const Store *s = reinterpret_cast<const Store *>(p->groups[i].store);
-        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, d.nt * (d.nt + 1) / 2), dim3(256), 0, st,
-                           d.partials, d.S, d.nt, d.count, s->sum + d.first, s->sumsq + d.first,
-                           s->scale ? s->scale + d.first : nullptr, p->groups[i].d_ncc);
+        const PlanBlock &g = p->groups[i];
+        const Store *s = g.store;
+        if (g.tri) {
+            hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, g.tiles()), dim3(256), 0, st,
+                               d.partials, d.S, d.nt, d.count, s->sum + d.first, s->sumsq + d.first,
+                               s->scale ? s->scale + d.first : nullptr, g.out, g.ld);
+        } else {
+            hipLaunchKernelGGL(store_reduce_rect_kernel, dim3(kTileS * kTileS / 256, g.tiles()), dim3(256), 0, st,
+                               d.partials, d.S, d.ntb, d.count, d.count_b, s->sum + d.first, s->sumsq + d.first,
+                               s->sum + d.first_b, s->sumsq + d.first_b, s->scale ? s->scale + d.first : nullptr,
+                               s->scale ? s->scale + d.first_b : nullptr, g.out, g.ld);
+        }
         PRNU_CHECK_LAUNCH();
     }
     return PRNU_OK;
@@ -1108,21 +1183,24 @@ void plan_free(GramPlan *p) {
 
 }  // namespace
 
-int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int64_t slab_values, GramPlan **out) {
+int gram_plan_create_blocks(Context *c, const std::vector<PlanBlock> &blocks, int64_t slab_values, GramPlan **out) {
     *out = nullptr;
-    PRNU_REQUIRE(groups && ngroups >= 1 && ngroups <= kMaxPlanGroups, "gram_plan: 1..%d groups, got %d",
-                 kMaxPlanGroups, ngroups);
+    const int nblocks = (int)blocks.size();
+    PRNU_REQUIRE(nblocks >= 1 && nblocks <= kMaxPlanGroups, "gram_plan: 1..%d blocks, got %d", kMaxPlanGroups,
+                 nblocks);
     PRNU_REQUIRE(slab_values >= 0, "gram_plan: slab_values must be non-negative");
     int format = -1;
-    for (int i = 0; i < ngroups; ++i) {
-        const prnu_gram_group &g = groups[i];
-        PRNU_REQUIRE(g.store && g.d_ncc, "gram_plan: group %d has a null store or output", i);
-        const Store *s = reinterpret_cast<const Store *>(g.store);
-        PRNU_REQUIRE(g.count >= 1 && g.first_slot >= 0 && g.first_slot + g.count <= s->capacity,
-                     "gram_plan: group %d slots [%d, %d) outside the capacity %d", i, g.first_slot,
-                     g.first_slot + g.count, s->capacity);
+    for (int i = 0; i < nblocks; ++i) {
+        const PlanBlock &g = blocks[i];
+        PRNU_REQUIRE(g.store && g.out, "gram_plan: block %d has a null store or output", i);
+        const Store *s = g.store;
+        PRNU_REQUIRE(g.row_count >= 1 && g.row_first >= 0 && g.row_first + g.row_count <= s->capacity &&
+                         g.col_count >= 1 && g.col_first >= 0 && g.col_first + g.col_count <= s->capacity,
+                     "gram_plan: block %d slots rows [%d, %d) x cols [%d, %d) outside the capacity %d", i,
+                     g.row_first, g.row_first + g.row_count, g.col_first, g.col_first + g.col_count, s->capacity);
+        PRNU_REQUIRE(g.ld >= (g.tri ? g.row_count : g.col_count), "gram_plan: block %d output stride too small", i);
         if (format < 0) format = s->format;
-        PRNU_REQUIRE(s->format == format, "gram_plan: every group's store must have the same format");
+        PRNU_REQUIRE(s->format == format, "gram_plan: every block's store must have the same format");
     }
     GramPlan *p = new (std::nothrow) GramPlan();
     if (!p) {
@@ -1132,14 +1210,11 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
     p->c = c;
     p->format = format;
     p->var = c->gram_cfg[format].variant;
-    p->groups.assign(groups, groups + ngroups);
+    p->groups = blocks;
     p->cus = device_cus(c->device);  // one 512-thread, 128 KiB-LDS workgroup per CU at a time
-    for (const prnu_gram_group &g : p->groups) {
-        const Store *s = reinterpret_cast<const Store *>(g.store);
-        const int nt = (g.count + kTileS - 1) / kTileS;
-        p->flop += 2.0 * (double)s->n * ((double)g.count * (g.count + 1) / 2.0);
-        (void)nt;
-    }
+    for (const PlanBlock &g : p->groups)
+        p->flop += 2.0 * (double)g.store->n *
+                   (g.tri ? (double)g.row_count * (g.row_count + 1) / 2.0 : (double)g.row_count * g.col_count);
     const int vals = block_values(format);
     if (slab_values % vals != 0) {
         set_error("gram_plan: slab_values must be a multiple of %d for this store format", vals);
@@ -1154,6 +1229,20 @@ int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int
     }
     *out = p;
     return PRNU_OK;
+}
+
+int gram_plan_create(Context *c, const prnu_gram_group *groups, int ngroups, int64_t slab_values, GramPlan **out) {
+    *out = nullptr;
+    PRNU_REQUIRE(groups && ngroups >= 1 && ngroups <= kMaxPlanGroups, "gram_plan: 1..%d groups, got %d",
+                 kMaxPlanGroups, ngroups);
+    std::vector<PlanBlock> blocks;
+    for (int i = 0; i < ngroups; ++i) {
+        const prnu_gram_group &g = groups[i];
+        PRNU_REQUIRE(g.store, "gram_plan: group %d has a null store", i);
+        blocks.push_back(PlanBlock{reinterpret_cast<const Store *>(g.store), g.first_slot, g.count, g.first_slot,
+                                   g.count, g.d_ncc, (int64_t)g.count, true});
+    }
+    return gram_plan_create_blocks(c, blocks, slab_values, out);
 }
 
 // Times every candidate slab length (eager launches, best of `reps`) and keeps
@@ -1259,6 +1348,25 @@ int prnu_gram_plan_create(prnu_ctx *ctx, const prnu_gram_group *groups, int ngro
     PRNU_REQUIRE(ctx && out, "gram_plan_create: null pointer");
     GramPlan *p = nullptr;
     PRNU_TRY(gram_plan_create(reinterpret_cast<Context *>(ctx), groups, ngroups, slab_values, &p));
+    *out = reinterpret_cast<prnu_gram_plan *>(p);
+    return PRNU_OK;
+}
+
+int prnu_gram_plan_create_blocks(prnu_ctx *ctx, const prnu_gram_block *blocks, int nblocks, int64_t slab_values,
+                                 prnu_gram_plan **out) {
+    PRNU_REQUIRE(ctx && out && blocks, "gram_plan_create_blocks: null pointer");
+    PRNU_REQUIRE(nblocks >= 1 && nblocks <= kMaxPlanGroups, "gram_plan: 1..%d blocks, got %d", kMaxPlanGroups,
+                 nblocks);
+    std::vector<PlanBlock> v;
+    for (int i = 0; i < nblocks; ++i) {
+        const prnu_gram_block &b = blocks[i];
+        PRNU_REQUIRE(b.store, "gram_plan: block %d has a null store", i);
+        const bool tri = b.row_first == b.col_first && b.row_count == b.col_count;
+        v.push_back(PlanBlock{reinterpret_cast<const Store *>(b.store), b.row_first, b.row_count, b.col_first,
+                              b.col_count, b.d_out, b.ld_out, tri});
+    }
+    GramPlan *p = nullptr;
+    PRNU_TRY(gram_plan_create_blocks(reinterpret_cast<Context *>(ctx), v, slab_values, &p));
     *out = reinterpret_cast<prnu_gram_plan *>(p);
     return PRNU_OK;
 }

@@ -129,3 +129,36 @@ def all_pairs(count: int, kind: str = "hilbert", param: int = 1, include_diagona
     _lib.check(lib.prnu_index_pairs(k, count, param, int(include_diagonal),
                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(out), ctypes.byref(n)))
     return out[: n.value]
+
+
+def tile_blocks(count: int, tile: int = 256, kind: str = "hilbert", param: int = 1) -> list:
+    """Rocket's pair scheduling lifted to GPU tiles: the `count` keys are cut
+    into ceil(count/tile) tiles, the index space enumerates the tile pairs
+    (I, J), I <= J (diagonal included), in its curve order, and each pair
+    becomes a block (row_offset, row_length, col_offset, col_length) of the
+    key triangle; I == J blocks are triangles, the rest rectangles."""
+    nt = (count + tile - 1) // tile
+    out = []
+    for I, J in all_pairs(nt, kind, param, include_diagonal=True):
+        r0, c0 = int(I) * tile, int(J) * tile
+        out.append((r0, min(tile, count - r0), c0, min(tile, count - c0)))
+    return out
+
+
+def ncc_by_index_space(gram, store, count: int, first: int = 0, kind: str = "hilbert", param: int = 1,
+                       tile: int = 256, slab_values: int = 0, use_graph: bool = True):
+    """All-pairs NCC of slots [first, first+count) of a PatternStore, scheduled
+    the way a Rocket host divides the pair triangle (index space over tiles of
+    `tile` keys, curve order) but run as ONE launch: every tile pair becomes a
+    block of a grouped plan writing straight into a (count, count) FP64 CUDA
+    matrix.  Returns (matrix, blocks); the upper triangle (and the diagonal
+    blocks' lower halves) are filled."""
+    import torch
+    from .csi import GramPlan
+    blocks = tile_blocks(count, tile, kind, param)
+    full = torch.zeros((count, count), dtype=torch.float64, device="cuda")
+    spec = [(store, first + r0, rn, first + c0, cn, full[r0:r0 + rn, c0:c0 + cn]) for (r0, rn, c0, cn) in blocks]
+    plan = GramPlan(gram, blocks=spec, slab_values=slab_values)
+    plan.run(use_graph=use_graph)
+    plan.close()
+    return full, blocks

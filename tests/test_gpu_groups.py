@@ -211,3 +211,41 @@ def test_fp8_plan_two_sizes():
         assert np.abs(got - want)[off].max() <= b_off
         st.close()
     gram.close()
+
+
+def test_block_plan_rectangles_and_index_space_schedule():
+    """Rectangle blocks (overlapping slot ranges included) against the FP64
+    oracle, and the all-pairs NCC scheduled over an index space of 256-key
+    tiles in one launch, bitwise equal to the store Gram with the same slabs."""
+    import ctypes
+    from rocket_amd import _lib, indexspace
+    n, count = 120 * 97, 600
+    p = patterns(count, n, 5, seed=4242)
+    st = csi.PatternStore(n, count, "split_bf16")
+    st.put(p)
+    gram = csi.GramNCC(8, 8)
+    host = p.cpu().numpy()
+    out_a = torch.full((300, 300), -9.0, dtype=torch.float64, device="cuda")
+    big = torch.full((64, 400), -9.0, dtype=torch.float64, device="cuda")
+    plan = csi.GramPlan(gram, blocks=[(st, 5, 300, 100, 300, out_a), (st, 500, 64, 0, 333, big[:, 50:383])])
+    plan.run(use_graph=True)
+    plan.close()
+    full = O.ncc_matrix(host)
+    assert np.abs(out_a.cpu().numpy() - full[5:305, 100:400]).max() <= NCC_TOL
+    b = big.cpu().numpy()
+    assert np.abs(b[:, 50:383] - full[500:564, 0:333]).max() <= NCC_TOL
+    assert np.all(b[:, :50] == -9.0) and np.all(b[:, 383:] == -9.0)      # nothing outside the view
+    # index-space schedule vs the store Gram, same K slabs (S = 7)
+    nkb = (n + 31) // 32
+    S = 7
+    kcb = (nkb + S - 1) // S
+    cfg = (ctypes.c_int32 * 3)(S, 1, 0)
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, _lib.PREC_SPLIT_BF16, cfg))
+    ref = st.gram(gram, count).cpu().numpy()
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, _lib.PREC_SPLIT_BF16, None))
+    got, blocks = indexspace.ncc_by_index_space(gram, st, count, slab_values=kcb * 32)
+    assert len(blocks) == 3 * 4 // 2
+    iu = np.triu_indices(count, 1)
+    assert np.array_equal(got.cpu().numpy()[iu], ref[iu])
+    st.close()
+    gram.close()

@@ -68,3 +68,20 @@ def test_tiling_divide_tiles_then_entries():
         assert not dd.get_children()
         total += len(dd.get_entries())
     assert total == 45
+
+
+@pytest.mark.parametrize("count,tile", [(1024, 256), (1000, 256), (700, 256), (5000, 256), (3, 2)])
+def test_tile_blocks_cover_the_triangle_once(count, tile):
+    """Index space over tiles: every pair i < j lies in exactly one block."""
+    blocks = ix.tile_blocks(count, tile)
+    cov = np.zeros((count, count), dtype=np.int32)
+    for r0, rn, c0, cn in blocks:
+        assert 0 < rn <= tile and 0 < cn <= tile
+        if (r0, rn) == (c0, cn):
+            for i in range(rn):
+                cov[r0 + i, r0 + i + 1:r0 + rn] += 1
+        else:
+            assert c0 >= r0 + rn                       # strictly above the diagonal
+            cov[r0:r0 + rn, c0:c0 + cn] += 1
+    iu = np.triu_indices(count, 1)
+    assert (cov[iu] == 1).all() and cov.sum() == len(iu[0])
