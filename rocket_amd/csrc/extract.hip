@@ -14,10 +14,12 @@
 //
 // Layout in HBM: images and patterns are row-major h x w FP32.  The Wiener
 // filter runs its row FFTs in LDS (one row per workgroup), then its column
-// FFTs in LDS over strips of adjacent columns read row by row (no
-// transpose), computes the variance stencil on |F|^2 in the natural layout,
-// scales and inverse-transforms the columns the same way, and ends with the
-// inverse row FFTs.
+// FFTs: for columns up to 1024 long, in LDS over strips of >= 4 adjacent
+// columns read row by row (no transpose, variance stencil in the natural
+// layout); for longer columns through a transpose (w x h) so they are LDS row
+// FFTs, the stencil in that transposed domain (the zero-padded box-min is
+// transpose invariant), and a transpose back.  It ends with the inverse row
+// FFTs.
 #include "prnu_internal.hpp"
 #include "fft_device.hpp"
 #include "fft_host.hpp"
@@ -386,6 +388,54 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, c
     (void)h;
 }
 
+// ------------------------------------------- transposed-layout row kernels
+// Used when a column strip would be narrower than 4 columns (long columns):
+// the plane is transposed and columns become LDS row transforms.
+// In-place forward FFT of the rows of B (length d.n) and |F|^2 into P.
+// grid (rows, batch); planes `plane` elements apart.
+__global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, float2 *B, float *P, int64_t plane) {
+    extern __shared__ float2 buf[];
+    const int r = blockIdx.x;
+    const int n = d.n;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
+    __syncthreads();
+    fft_lds<-1>(buf, d);
+    for (int j = threadIdx.x; j < n; j += kFftThreads) {
+        const float2 f = buf[j];
+        row[j] = f;
+        prow[j] = f.x * f.x + f.y * f.y;
+    }
+}
+
+// Scale row r of B by S and run the inverse FFT (length d.n) in place.
+// grid (rows, B); planes `plane` apart.
+__global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d, float2 *B, const float *S,
+                                                                       int64_t plane) {
+    extern __shared__ float2 buf[];
+    const int r = blockIdx.x;
+    const int n = d.n;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = cscale(row[j], srow[j]);
+    __syncthreads();
+    fft_lds<1>(buf, d);
+    for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
+}
+
+// grid (rows, B)
+template <int DIR>
+__global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d, float2 *B, int64_t plane) {
+    extern __shared__ float2 buf[];
+    const int n = d.n;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * n;
+    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
+    __syncthreads();
+    fft_lds<DIR>(buf, d);
+    for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
+}
+
 // ------------------------------------------------------ column transforms
 // Column FFTs (length h = d.n) of a row-major plane, strip by strip: a
 // workgroup reads `sw` adjacent columns row by row (contiguous pieces of
@@ -425,11 +475,22 @@ __global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void cols_fft_strip_kerne
 }
 
 // ------------------------------------------------------------- workspace
+// Columns per strip: a power of two <= 16 with sw * h <= kFftMultiMax.
+int col_strip(int h) {
+    int sw = 16;
+    while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
+    return sw;
+}
+// Column transforms by strips when a strip holds at least 4 columns (32-byte
+// row pieces); longer columns go through transposes (measured faster at
+// h = 1536, where a strip would hold 2 columns).
+bool strips_for(int h) { return col_strip(h) >= 4; }
+
 // Per-image buffers, each an array over the batch with the given stride.
 struct ExtractWs {
     int nb;                 // images the workspace holds
     int64_t n;              // plane stride (elements) of A, P, S
-    float2 *A;              // complex plane of the Wiener filter
+    float2 *A, *B;          // complex planes of the Wiener filter (B: transposed path only)
     float *P, *S;           // power spectrum / scale (S also holds the FastNoise output)
     double *rowsq, *var, *colpart, *rowpart, *colmean, *rowmean, *g;
     int64_t cp_stride, rp_stride;
@@ -441,7 +502,8 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 size_t per_image_bytes(int h, int w) {
     const int64_t n = (int64_t)h * w;
     const int tx = (w + kFnTX - 1) / kFnTX, ty = (h + kFnTY - 1) / kFnTY;
-    return 16 * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) + 10 * 256;
+    return (strips_for(h) ? 16 : 24) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
+           11 * 256;
 }
 
 int get_ws(Context *c, int nb, ExtractWs &e) {
@@ -454,6 +516,7 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
     e.cp_stride = 2 * (int64_t)w * e.tiles_y;
     e.rp_stride = 2 * (int64_t)h * e.tiles_x;
     const size_t sz[] = {align256(sizeof(float2) * n * nb),
+                         strips_for(h) ? 0 : align256(sizeof(float2) * n * nb),
                          align256(sizeof(float) * n * nb),    align256(sizeof(float) * n * nb),
                          align256(sizeof(double) * h * nb),   align256(sizeof(double) * nb),
                          align256(sizeof(double) * e.cp_stride * nb), align256(sizeof(double) * e.rp_stride * nb),
@@ -463,10 +526,10 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
     for (size_t s : sz) total += s;
     PRNU_TRY(c->ws_filter.ensure(total));
     char *p = static_cast<char *>(c->ws_filter.ptr);
-    void **slots[] = {(void **)&e.A,       (void **)&e.P,       (void **)&e.S,
+    void **slots[] = {(void **)&e.A,       (void **)&e.B,       (void **)&e.P,       (void **)&e.S,
                       (void **)&e.rowsq,   (void **)&e.var,     (void **)&e.colpart, (void **)&e.rowpart,
                       (void **)&e.colmean, (void **)&e.rowmean, (void **)&e.g};
-    for (int k = 0; k < 10; ++k) {
+    for (int k = 0; k < 11; ++k) {
         *slots[k] = p;
         p += sz[k];
     }
@@ -511,13 +574,6 @@ int launch_zm_means(Context *c, ExtractWs &e, int nb, hipStream_t st) {
     return PRNU_OK;
 }
 
-// Columns per strip: a power of two <= 16 with sw * h <= kFftMultiMax.
-int col_strip(int h) {
-    int sw = 16;
-    while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
-    return sw;
-}
-
 template <int DIR, int MODE>
 int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int ncols, int nb, const float *S, float *P,
                     hipStream_t st) {
@@ -536,9 +592,10 @@ int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int nco
     return PRNU_OK;
 }
 
-// Wiener filter of nb images (img, img_stride floats apart) into out, all in
-// the natural h x w layout: row FFTs, column FFTs by strips (+ |F|^2), the
-// variance stencil, scaling + inverse column FFTs, inverse row FFTs.
+// Wiener filter of nb images (img, img_stride floats apart) into out: row
+// FFTs, column FFTs (+ |F|^2), the variance stencil, scaling + inverse column
+// FFTs, inverse row FFTs; columns by strips or through transposes
+// (strips_for).
 int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float *out,
                   int64_t out_stride, bool zm, hipStream_t st) {
     const int h = c->h, w = c->w;
@@ -548,13 +605,30 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     hipLaunchKernelGGL(wiener_rows_fwd_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img, img_stride,
                        h, e.A, e.rowsq, zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, w, w, nb, nullptr, e.P, st)));
-    hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
-    PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(varest_scale_kernel, dim3((w + kVT - 1) / kVT, (h + kVT - 1) / kVT, nb), dim3(256), 0, st, e.P,
-                       e.S, h, w, e.var);
-    PRNU_CHECK_LAUNCH();
-    PRNU_TRY((launch_cols_fft<1, 2>(dh, e.A, n, w, w, nb, e.S, nullptr, st)));
+    if (strips_for(h)) {
+        PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, w, w, nb, nullptr, e.P, st)));
+        hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(varest_scale_kernel, dim3((w + kVT - 1) / kVT, (h + kVT - 1) / kVT, nb), dim3(256), 0,
+                           st, e.P, e.S, h, w, e.var);
+        PRNU_CHECK_LAUNCH();
+        PRNU_TRY((launch_cols_fft<1, 2>(dh, e.A, n, w, w, nb, e.S, nullptr, st)));
+    } else {
+        // transposed layout: B = A^T (w x h), column FFTs as rows of B; the
+        // zero-padded box-min stencil is transpose invariant
+        PRNU_TRY(launch_transpose_c(e.A, e.B, h, w, nb, n, st));
+        hipLaunchKernelGGL(rows_fft_power_kernel, dim3(w, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.P, n);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(varest_scale_kernel, dim3((h + kVT - 1) / kVT, (w + kVT - 1) / kVT, nb), dim3(256), 0,
+                           st, e.P, e.S, w, h, e.var);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(rows_ifft_scaled_kernel, dim3(w, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.S,
+                           n);
+        PRNU_CHECK_LAUNCH();
+        PRNU_TRY(launch_transpose_c(e.B, e.A, w, h, nb, n, st));
+    }
     hipLaunchKernelGGL(rows_ifft_real_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, e.A, n, out,
                        out_stride, 1.0f / (float)(w * h));
     PRNU_CHECK_LAUNCH();
@@ -562,17 +636,33 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
 }
 
 // R2C spectrum of nb images into out (h x (w/2+1) complex each, out_stride
-// float2 apart): half-row FFTs straight into out, then column FFTs in place.
+// float2 apart): half-row FFTs, then column FFTs (strips in place in out, or
+// through transposes).
 int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float2 *out,
                     int64_t out_stride, hipStream_t st) {
     const int h = c->h, w = c->w, wh = w / 2 + 1;
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    (void)e;
+    if (strips_for(h)) {
+        hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img,
+                           img_stride, out, out_stride, h);
+        PRNU_CHECK_LAUNCH();
+        PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
+        return PRNU_OK;
+    }
+    const int64_t n = c->pixels();
     hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img, img_stride,
-                       out, out_stride, h);
+                       e.A, n, h);
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
+    PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
+    hipLaunchKernelGGL(rows_fft_inplace_kernel<-1>, dim3(wh, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, n);
+    PRNU_CHECK_LAUNCH();
+    if (nb == 1 || out_stride == n) {
+        PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st));
+    } else {
+        for (int b = 0; b < nb; ++b)
+            PRNU_TRY(launch_transpose_c(e.B + (int64_t)b * n, out + (int64_t)b * out_stride, wh, h, 1, 0, st));
+    }
     return PRNU_OK;
 }
 
