@@ -313,8 +313,8 @@ PRNU_EXPORT int prnu_fpstore_query(prnu_ctx *ctx, prnu_fpstore *store, const flo
  * file each, two K stages in flight, otherwise 8 waves, one K stage in
  * flight; for the store Gram and grouped plans, 0 = direct global-to-LDS
  * loads, 1 = loads staged through registers, 2 = direct loads with the wave
- * priority raised over the MFMA block, 3 (split-BF16 store Gram only) = 4
- * waves of 128 x 128 with accumulators in AGPRs. */
+ * priority raised over the MFMA block, 3 (store Gram of split-BF16 and FP8
+ * stores) = 4 waves of 128 x 128 with accumulators in AGPRs. */
 typedef struct {
     int32_t slabs;
     int32_t xcd_remap;

@@ -581,6 +581,7 @@ __device__ __forceinline__ ItemWaves item_waves4(int u, int nt, int wave) {
     return r;
 }
 
+template <int FMT>
 __global__ __launch_bounds__(kThr4, 1) void store_gram4_kernel(StoreGramArgs a) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
     const int bid = blockIdx.x;
@@ -645,23 +646,48 @@ __global__ __launch_bounds__(kThr4, 1) void store_gram4_kernel(StoreGramArgs a) 
     const bool active = out_tile >= 0;
     const int frow = lane & 15, ch = lane >> 4;
     auto compute = [&](const uint8_t *buf) {
-        bf16x8 bh[8], bl[8];
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
-            const int r = b_off + cb * 16 + frow;
-            bh[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
-            bl[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
-        }
-#pragma unroll
-        for (int rb = 0; rb < 8; ++rb) {
-            const int r = a_off + rb * 16 + frow;
-            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
-            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+        if constexpr (FMT == PRNU_PREC_FP8) {
+            // one 128-value block per stage; lane (row frow, k-group lane>>4)
+            // holds 32 bytes = 16-byte chunks 2*(lane>>4), +1
+            const int c8 = (lane >> 4) * 2;
+            intx8 bq[8];
 #pragma unroll
             for (int cb = 0; cb < 8; ++cb) {
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+                const int r = b_off + cb * 16 + frow;
+                const int4 lo = *reinterpret_cast<const int4 *>(buf + s_off(r, c8));
+                const int4 hi = *reinterpret_cast<const int4 *>(buf + s_off(r, c8 + 1));
+                bq[cb] = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            }
+#pragma unroll
+            for (int rb = 0; rb < 8; ++rb) {
+                const int r = a_off + rb * 16 + frow;
+                const int4 lo = *reinterpret_cast<const int4 *>(buf + s_off(r, c8));
+                const int4 hi = *reinterpret_cast<const int4 *>(buf + s_off(r, c8 + 1));
+                const intx8 aq = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aq, bq[cb], acc[rb][cb], 0, 0, 0,
+                                                                                   127, 0, 127);
+            }
+        } else {
+            bf16x8 bh[8], bl[8];
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const int r = b_off + cb * 16 + frow;
+                bh[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
+                bl[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+            }
+#pragma unroll
+            for (int rb = 0; rb < 8; ++rb) {
+                const int r = a_off + rb * 16 + frow;
+                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
+                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb) {
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+                }
             }
         }
     };
@@ -957,13 +983,17 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
     const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 + priority, 3 four waves (split)
-    if (var == 3 && s->format == PRNU_PREC_SPLIT_BF16) {
+    if (var == 3 && s->format != PRNU_PREC_FP32) {
         // same items per slab as the 8-wave packing (ceil(3nt/4) = ceil(6nt/8))
-        hipLaunchKernelGGL(store_gram4_kernel, grid, dim3(kThr4), 0, st, a);
+        if (s->format == PRNU_PREC_FP8)
+            hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_FP8>, grid, dim3(kThr4), 0, st, a);
+        else
+            hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_SPLIT_BF16>, grid, dim3(kThr4), 0, st, a);
         PRNU_CHECK_LAUNCH();
         time_end(c, st);
         hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials,
-                           a.S, a.nt, count, s->sum + first, s->sumsq + first, nullptr, d_ncc, (int64_t)count);
+                           a.S, a.nt, count, s->sum + first, s->sumsq + first, s->scale ? s->scale + first : nullptr,
+                           d_ncc, (int64_t)count);
         PRNU_CHECK_LAUNCH();
         return PRNU_OK;
     }

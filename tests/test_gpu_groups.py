@@ -154,7 +154,7 @@ def fp8_bounds(n):
     return 6 * 0.05 / np.sqrt(n), 1.5e-3 + 6 * 0.036 * np.sqrt(3.0 / n)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_fp8_store_gram_within_e4m3_bound(variant):
     """FP8 store (blocks of 128 values, per-pattern scale): n not a multiple
     of 128, ragged count, offset slots; both load variants; close to the
