@@ -560,7 +560,7 @@ def main():
         log(f"serving: p50 {serve['p50_ms']:.1f} ms, p99 {serve['p99_ms']:.1f} ms, {serve['qps']:.0f} q/s")
 
     line = {
-        "metric": "NCC pairs/s, split-BF16 MFMA vs FP32 at |dNCC|<=1e-5; query p50/p99 latency",
+        "metric": "NCC pairs/s, split-BF16 MFMA vs FP32 at |\u0394NCC|\u22641e-5; query p50/p99 latency",
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
