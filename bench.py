@@ -295,6 +295,10 @@ def groups_bench(torch, steps=5):
 
     ms_graph = timed(lambda: plan.run(use_graph=True, sync=False), steps)
     ms_eager = timed(lambda: plan.run(use_graph=False, sync=False), steps)
+    tuned_persistent = plan.persistent
+    plan.persistent = not tuned_persistent                  # the other launch form, same slab
+    ms_other = timed(lambda: plan.run(use_graph=True, sync=False), steps)
+    plan.persistent = tuned_persistent
     outs = [o.cpu().numpy() for o in plan.outputs]
     sep_out = [torch.empty((c, c), dtype=torch.float64, device="cuda") for c, _, _ in GROUPS_C3]
 
@@ -317,6 +321,8 @@ def groups_bench(torch, steps=5):
     return {"workload": "groups 2000@1024x1024 + 1500@2048x1536 + 596@4000x3000", "pairs": pairs,
             "pairs_per_s": pairs / (ms_graph / 1000.0), "ms_per_step": ms_graph, "eager_ms_per_step": ms_eager,
             "separate_launches_ms_per_step": ms_sep, "slab_values": slab, "work_items": info["work_items"],
+            "launch": "persistent" if tuned_persistent else "per_item",
+            ("per_item" if tuned_persistent else "persistent") + "_ms_per_step": ms_other,
             "mfma_tflops": flops / (ms_graph / 1000.0) / 1e12, "max_abs_dncc_vs_fp64_sample": dncc,
             "autotune_best_ms": tune_ms}
 

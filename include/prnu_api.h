@@ -252,10 +252,18 @@ PRNU_EXPORT int prnu_gram_plan_destroy(prnu_gram_plan *plan);
 /* use_graph != 0: the first run captures the launches into a HIP graph,
  * later runs replay it (one hipGraphLaunch); 0: plain launches. */
 PRNU_EXPORT int prnu_gram_plan_run(prnu_gram_plan *plan, int use_graph, prnu_stream stream);
-/* Times candidate slab lengths on the device (writes the outputs) and keeps
- * the fastest; drops a captured graph. */
+/* Times candidate slab lengths with both launch forms on the device (writes
+ * the outputs) and keeps the fastest pair; drops a captured graph. */
 PRNU_EXPORT int prnu_gram_plan_autotune(prnu_gram_plan *plan, prnu_stream stream, int64_t *best_slab_values,
                                         float *best_ms);
+/* Launch form of the plan's product: PRNU_PLAN_PER_ITEM (default) launches
+ * one workgroup per work item; PRNU_PLAN_PERSISTENT launches one workgroup
+ * per CU that takes work items from per-XCD queues until the tile list is
+ * done (direct-to-LDS loads whatever the load variant).  Results are
+ * bitwise the same.  Drops a captured graph. */
+typedef enum { PRNU_PLAN_PER_ITEM = 0, PRNU_PLAN_PERSISTENT = 1 } prnu_plan_launch;
+PRNU_EXPORT int prnu_gram_plan_set_launch(prnu_gram_plan *plan, int launch);
+PRNU_EXPORT int prnu_gram_plan_get_launch(const prnu_gram_plan *plan, int *launch);
 /* A block of a plan: NCC of slots [row_first, row_first + row_count) against
  * [col_first, col_first + col_count) of one store into d_out (row-major,
  * leading dimension ld_out).  Identical ranges make a triangle block: the
@@ -275,7 +283,8 @@ typedef struct prnu_gram_block {
  * prnu_gram_plan_create. */
 PRNU_EXPORT int prnu_gram_plan_create_blocks(prnu_ctx *ctx, const prnu_gram_block *blocks, int nblocks,
                                              int64_t slab_values, prnu_gram_plan **out);
-/* Work items, workgroups launched, slab length, partial bytes and the
+/* Work items, workgroups launched (padding items included for the per-item
+ * form), slab length, partial bytes and the
  * algorithmic FLOP of one product over every group's triangle (diagonal
  * included). */
 PRNU_EXPORT int prnu_gram_plan_info(const prnu_gram_plan *plan, int64_t *work_items, int *workgroups,

@@ -20,6 +20,7 @@ INPUT_RGB8, INPUT_GRAY8, INPUT_FLOAT32 = 0, 1, 2
 PREC_FP32, PREC_SPLIT_BF16, PREC_FP8 = 0, 1, 2
 PCE_REFERENCE, PCE_TEXTBOOK = 0, 1
 INDEX_HILBERT, INDEX_TILING = 0, 1
+PLAN_PER_ITEM, PLAN_PERSISTENT = 0, 1
 
 PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp8": PREC_FP8}
 
@@ -88,6 +89,8 @@ _SIGNATURES = {
     "prnu_gram_plan_destroy": (_c_int, [_c_void_p]),
     "prnu_gram_plan_run": (_c_int, [_c_void_p, _c_int, _c_void_p]),
     "prnu_gram_plan_autotune": (_c_int, [_c_void_p, _c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(ctypes.c_float)]),
+    "prnu_gram_plan_set_launch": (_c_int, [_c_void_p, _c_int]),
+    "prnu_gram_plan_get_launch": (_c_int, [_c_void_p, ctypes.POINTER(_c_int)]),
     "prnu_gram_plan_info": (_c_int, [_c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int), ctypes.POINTER(_c_i64),
                                      ctypes.POINTER(_c_size), ctypes.POINTER(ctypes.c_double)]),
     "prnu_fpstore_create": (_c_int, [_c_i64, _c_int, ctypes.POINTER(_c_void_p)]),

@@ -522,6 +522,18 @@ class GramPlan:
         _sync(True)
         return v.value, ms.value
 
+    @property
+    def persistent(self) -> bool:
+        """True: one workgroup per CU takes work items from per-XCD queues;
+        False: one workgroup per work item.  Results are bitwise the same."""
+        v = ctypes.c_int()
+        _lib.call("prnu_gram_plan_get_launch", self.handle, ctypes.byref(v))
+        return v.value == _lib.PLAN_PERSISTENT
+
+    @persistent.setter
+    def persistent(self, on: bool) -> None:
+        _lib.call("prnu_gram_plan_set_launch", self.handle, _lib.PLAN_PERSISTENT if on else _lib.PLAN_PER_ITEM)
+
     def info(self) -> dict:
         items, wg, slab, part, flop = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t(), \
             ctypes.c_double()

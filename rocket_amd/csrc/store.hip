@@ -308,7 +308,8 @@ __device__ __forceinline__ int items_per_slab(int nt) { return nt * (nt - 1) / 2
 template <int FMT, int VAR>
 __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, int64_t cap_bytes, int first, int count,
                                           int first_b, int count_b, const ItemWaves &w, int64_t b_begin,
-                                          int64_t b_end, float *__restrict__ out, uint8_t *lds) {
+                                          int64_t b_end, float *__restrict__ out, uint8_t *lds,
+                                          int tid = threadIdx.x) {
     constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
     constexpr bool kFp8 = FMT == PRNU_PREC_FP8;
     constexpr bool DMA = VAR != 1;
@@ -318,7 +319,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     const bool active = out != nullptr;
     const int2 tile = make_int2(w.rows_x, half ? w.rows_x : w.rows_y);
 
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int lane = tid & 63;
     const int wrow0 = w.a_off, wcol0 = w.b_off;
 
     // thread -> (row = (tid >> 3) + 64*it, chunk = tid & 7); rows past count
@@ -513,9 +514,9 @@ __device__ __forceinline__ ItemWaves rect_waves(int u, int ntb, int wave) {
 template <int FMT, int VAR>
 __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int first_b,
                                          int count_b, int nt, int ntb, int S, int64_t nkb, int64_t kcb,
-                                         float *partials, int u, int s, uint8_t *lds) {
+                                         float *partials, int u, int s, uint8_t *lds, int tid = threadIdx.x) {
     // wave-uniform: keep it in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     ItemWaves w = ntb > 0 ? rect_waves(u, ntb, wave) : item_waves(u, nt, wave);
     w.rows_x = __builtin_amdgcn_readfirstlane(w.rows_x);
     w.rows_y = __builtin_amdgcn_readfirstlane(w.rows_y);
@@ -527,7 +528,7 @@ __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_byte
     const int64_t b_end = min(nkb, b_begin + kcb);
     float *out = w.out_tile < 0 ? nullptr
                                 : partials + ((int64_t)w.out_tile * S + s) * (kTileS * kTileS) + w.out_off;
-    gram_item<FMT, VAR>(blocks, cap_bytes, first, count, first_b, count_b, w, b_begin, b_end, out, lds);
+    gram_item<FMT, VAR>(blocks, cap_bytes, first, count, first_b, count_b, w, b_begin, b_end, out, lds, tid);
 }
 
 // ------------------------------------------------ 4-wave split-BF16 item
@@ -776,6 +777,50 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
     run_item<FMT, VAR>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.nt, g.ntb, g.S, g.nkb,
                        g.kcb, g.partials, u, s, lds);
 }
+
+// Persistent form: a grid of one workgroup per CU walks the item list.
+// Items v with v & 7 == x form queue x, taken in order through the counter
+// queue[16 * x] by the workgroups with blockIdx.x & 7 == x (the XCD the
+// dispatcher gives them), so the items of one slab stay on one XCD's L2 as
+// in the one-item-per-workgroup launch, and a workgroup that finishes early
+// takes the next item instead of waiting for the launch's tail.  Every
+// workgroup leaves once its queue passes v_total; the counters must be zero
+// at launch (store_queue_reset_kernel).
+template <int FMT>
+__global__ __launch_bounds__(kThrS, 1) void store_gram_groups_persistent_kernel(const GroupDev *__restrict__ groups,
+                                                                               int ngroups, int64_t v_total,
+                                                                               int *__restrict__ queue) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
+    __shared__ int taken;
+    const int x = blockIdx.x & 7;
+    for (;;) {
+        if (threadIdx.x == 0) taken = atomicAdd(queue + 16 * x, 1);
+        __syncthreads();
+        const int64_t v = (int64_t)__builtin_amdgcn_readfirstlane(taken) * 8 + x;
+        __syncthreads();  // every thread has read `taken` before thread 0 writes it again
+        if (v >= v_total) break;
+        int lo = 0, hi = ngroups - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (groups[mid].v_begin <= v) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupDev &g = groups[lo];
+        const int64_t vl = v - g.v_begin;
+        const int local = (int)(vl >> 3);
+        const int u = local % g.T;
+        const int s = (local / g.T) * 8 + (int)(vl & 7);
+        if (s >= g.S) continue;  // padding item (uniform across the workgroup)
+        // the thread id is opaque per item, so its address terms are not
+        // hoisted out of this loop (they would stay live across every item)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        run_item<FMT, 0>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.nt, g.ntb, g.S, g.nkb,
+                         g.kcb, g.partials, u, s, lds, tid);
+    }
+}
+
+__global__ void store_queue_reset_kernel(int *queue) { queue[threadIdx.x] = 0; }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
 __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restrict__ partials, int S, int nt,
@@ -1049,6 +1094,8 @@ struct GramPlan {
     int64_t v_total = 0, items = 0;  // workgroups launched, work items (padding excluded)
     int cus = 0;
     int var = 0;                     // load variant (the context's config of the format)
+    int persistent = 0;              // 1: one workgroup per CU takes items from per-XCD queues
+    int *queue = nullptr;            // 8 counters, 64 bytes apart
     double flop = 0.0;               // one product over every block (triangles incl. diagonal)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1113,6 +1160,7 @@ int plan_layout(GramPlan *p, int64_t kcb) {
     for (GroupDev &d : gd)
         d.partials = reinterpret_cast<float *>(reinterpret_cast<uint8_t *>(p->partials) + (uintptr_t)d.partials);
     if (!p->d_groups) PRNU_CHECK_HIP(hipMalloc(&p->d_groups, sizeof(GroupDev) * kMaxPlanGroups));
+    if (!p->queue) PRNU_CHECK_HIP(hipMalloc(&p->queue, 128 * sizeof(int)));
     PRNU_CHECK_HIP(hipMemcpy(p->d_groups, gd.data(), sizeof(GroupDev) * gd.size(), hipMemcpyHostToDevice));
     p->gd = gd;
     p->kcb = kcb;
@@ -1133,12 +1181,26 @@ int64_t plan_default_kcb(const GramPlan *p) {
     return choose_kcb(items.data(), tiles.data(), nkb.data(), (int)items.size(), p->cus, kPlanPartialCap);
 }
 
+// Persistent launch: one workgroup per CU, a multiple of 8 so blockIdx.x & 7
+// is the XCD.
+int plan_persistent_grid(const GramPlan *p) {
+    return (int)std::max<int64_t>(8, std::min<int64_t>(p->v_total, p->cus / 8 * 8));
+}
+
 int plan_launch(GramPlan *p, hipStream_t st) {
     const int ng = (int)p->gd.size();
     const dim3 grid((unsigned)p->v_total);
+    const dim3 pgrid((unsigned)plan_persistent_grid(p));
+    if (p->persistent) {
+        hipLaunchKernelGGL(store_queue_reset_kernel, dim3(1), dim3(128), 0, st, p->queue);
+        PRNU_CHECK_LAUNCH();
+    }
 #define PRNU_GROUPS_GRAM(F)                                                                              \
     do {                                                                                                 \
-        if (p->var == 1)                                                                                 \
+        if (p->persistent)                                                                               \
+            hipLaunchKernelGGL((store_gram_groups_persistent_kernel<F>), pgrid, dim3(kThrS), 0, st,         \
+                               p->d_groups, ng, p->v_total, p->queue);                                   \
+        else if (p->var == 1)                                                                            \
             hipLaunchKernelGGL((store_gram_groups_kernel<F, 1>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
         else if (p->var == 2)                                                                            \
             hipLaunchKernelGGL((store_gram_groups_kernel<F, 2>), grid, dim3(kThrS), 0, st, p->d_groups, ng); \
@@ -1208,6 +1270,7 @@ void plan_free(GramPlan *p) {
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     if (p->partials) (void)hipFree(p->partials);
     if (p->d_groups) (void)hipFree(p->d_groups);
+    if (p->queue) (void)hipFree(p->queue);
     delete p;
 }
 
@@ -1285,30 +1348,35 @@ int gram_plan_autotune(GramPlan *p, hipStream_t st, int64_t *best_values, float 
     PRNU_CHECK_HIP(hipEventCreate(&e1));
     float best = 3.4e38f;
     int64_t best_kcb = p->kcb;
+    int best_launch = p->persistent;
     int rc = PRNU_OK;
     for (int64_t kcb : cands) {
         if (kcb < 16) continue;
         if (plan_layout(p, kcb) != PRNU_OK) continue;  // e.g. partials over the limit
-        rc = plan_launch(p, st);  // warm-up
-        if (rc != PRNU_OK) break;
-        float ms_min = 3.4e38f;
-        for (int r = 0; r < 3 && rc == PRNU_OK; ++r) {
-            (void)hipEventRecord(e0, st);
-            rc = plan_launch(p, st);
-            (void)hipEventRecord(e1, st);
-            if (hipEventSynchronize(e1) != hipSuccess) rc = PRNU_EDEVICE;
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            ms_min = std::min(ms_min, ms);
+        for (int launch = 0; launch < 2 && rc == PRNU_OK; ++launch) {
+            p->persistent = launch;
+            rc = plan_launch(p, st);  // warm-up
+            float ms_min = 3.4e38f;
+            for (int r = 0; r < 3 && rc == PRNU_OK; ++r) {
+                (void)hipEventRecord(e0, st);
+                rc = plan_launch(p, st);
+                (void)hipEventRecord(e1, st);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = PRNU_EDEVICE;
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                ms_min = std::min(ms_min, ms);
+            }
+            if (rc == PRNU_OK && ms_min < best) {
+                best = ms_min;
+                best_kcb = kcb;
+                best_launch = launch;
+            }
         }
         if (rc != PRNU_OK) break;
-        if (ms_min < best) {
-            best = ms_min;
-            best_kcb = kcb;
-        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    p->persistent = best_launch;
     if (rc != PRNU_OK) return rc;
     PRNU_TRY(plan_layout(p, best_kcb));
     if (best_values) *best_values = best_kcb * block_values(p->format);
@@ -1422,12 +1490,28 @@ int prnu_gram_plan_autotune(prnu_gram_plan *plan, prnu_stream stream, int64_t *b
     return finish(p->c, stream);
 }
 
+int prnu_gram_plan_set_launch(prnu_gram_plan *plan, int launch) {
+    PRNU_REQUIRE(plan, "gram_plan_set_launch: null plan");
+    PRNU_REQUIRE(launch == PRNU_PLAN_PER_ITEM || launch == PRNU_PLAN_PERSISTENT,
+                 "gram_plan_set_launch: launch must be 0 (per item) or 1 (persistent), got %d", launch);
+    GramPlan *p = reinterpret_cast<GramPlan *>(plan);
+    if (p->persistent != launch) plan_drop_graph(p);
+    p->persistent = launch;
+    return PRNU_OK;
+}
+
+int prnu_gram_plan_get_launch(const prnu_gram_plan *plan, int *launch) {
+    PRNU_REQUIRE(plan && launch, "gram_plan_get_launch: null pointer");
+    *launch = reinterpret_cast<const GramPlan *>(plan)->persistent;
+    return PRNU_OK;
+}
+
 int prnu_gram_plan_info(const prnu_gram_plan *plan, int64_t *work_items, int *workgroups, int64_t *slab_values,
                         size_t *partial_bytes, double *flop) {
     PRNU_REQUIRE(plan, "gram_plan_info: null plan");
     const GramPlan *p = reinterpret_cast<const GramPlan *>(plan);
     if (work_items) *work_items = p->items;
-    if (workgroups) *workgroups = (int)p->v_total;
+    if (workgroups) *workgroups = p->persistent ? plan_persistent_grid(p) : (int)p->v_total;
     if (slab_values) *slab_values = p->kcb * block_values(p->format);
     if (partial_bytes) *partial_bytes = p->partial_bytes;
     if (flop) *flop = p->flop;

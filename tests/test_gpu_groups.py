@@ -249,3 +249,43 @@ def test_block_plan_rectangles_and_index_space_schedule():
     assert np.array_equal(got.cpu().numpy()[iu], ref[iu])
     st.close()
     gram.close()
+
+
+@pytest.mark.parametrize("fmt", ["split_bf16", "fp32", "fp8"])
+def test_persistent_launch_bitwise_equal(fmt):
+    """The persistent launch (one workgroup per CU over per-XCD queues) gives
+    the per-item launch's results bitwise, for triangles of several stores and
+    rectangle blocks, eager and replayed from a graph twice (the queue
+    counters are reset inside the graph)."""
+    specs = [(64 * 64, 300), (100 * 90, 257)]
+    stores = []
+    for i, (n, count) in enumerate(specs):
+        p = patterns(count, n, 3, seed=990 + i)
+        st = csi.PatternStore(n, count, fmt)
+        st.put(p)
+        stores.append(st)
+    gram = csi.GramNCC(8, 8)
+    rect = torch.zeros((100, 200), dtype=torch.float64, device="cuda")
+    tri = torch.zeros((257, 257), dtype=torch.float64, device="cuda")
+    blocks = [(stores[0], 10, 100, 50, 200, rect), (stores[1], 0, 257, 0, 257, tri)]
+    plans = [csi.GramPlan(gram, [(st, 0, c) for st, (_, c) in zip(stores, specs)]), csi.GramPlan(gram, blocks=blocks)]
+    for plan in plans:
+        assert not plan.persistent
+        want = [o.clone() for o in plan.run(use_graph=False)]
+        plan.persistent = True
+        assert plan.persistent and plan.info()["workgroups"] % 8 == 0
+        for use_graph in (False, True, True):
+            for o in plan.outputs:
+                o.fill_(float("nan"))
+            got = plan.run(use_graph=use_graph)
+            for g, (a, b) in enumerate(zip(got, want)):
+                assert torch.equal(a, b), f"{fmt} graph={use_graph} block {g}"
+        plan.close()
+    from rocket_amd import _lib
+    small = csi.GramPlan(gram, [(stores[0], 0, 8)])
+    with pytest.raises(ValueError):                  # launch forms are 0 and 1
+        _lib.call("prnu_gram_plan_set_launch", small.handle, 2)
+    small.close()
+    for st in stores:
+        st.close()
+    gram.close()
