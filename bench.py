@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
     ap.add_argument("--only", choices=["pce", "extract", "groups", "serving"],
                     help="run one secondary leg alone and print its result (for profiling)")
+    ap.add_argument("--extract-shape", default=None, metavar="HxW",
+                    help="extraction leg: one frame shape only (default: 1536x2048 and 512x512)")
+    ap.add_argument("--extract-mode", default=None, choices=["ncc", "pce"],
+                    help="extraction leg: one output mode only (default: both)")
     return ap.parse_args()
 
 
@@ -327,16 +331,22 @@ def groups_bench(torch, steps=5):
             "autotune_best_ms": tune_ms}
 
 
-def extraction_bench(torch, batch=32):
+def extraction_bench(torch, batch=32, shape=None, only_mode=None):
     """PRNU extraction of RGB8 frames (one batched call per `batch` frames) in
-    images/s at 2048x1536 and 512x512, NCC patterns and PCE spectra."""
+    images/s at 2048x1536 and 512x512 (or `shape` = "HxW" alone), NCC
+    patterns and PCE spectra (or `only_mode` alone)."""
     from rocket_amd import _lib, csi
     res = {}
-    for h, w, tag in ((HEIGHT, WIDTH, ""), (512, 512, "_512")):
+    shapes = ((HEIGHT, WIDTH, ""), (512, 512, "_512"))
+    if shape:
+        sh, sw = (int(v) for v in shape.lower().split("x"))
+        shapes = ((sh, sw, "" if (sh, sw) == (HEIGHT, WIDTH) else f"_{sh}x{sw}"),)
+    modes = [(m, nm) for m, nm in ((_lib.MODE_NCC, "ncc"), (_lib.MODE_PCE, "pce")) if only_mode in (None, nm)]
+    for h, w, tag in shapes:
         n = h * w
         g = torch.Generator(device="cuda").manual_seed(77)
         imgs = torch.randint(0, 256, (batch, 3 * n), generator=g, device="cuda", dtype=torch.uint8)
-        for mode, name in ((_lib.MODE_NCC, "ncc"), (_lib.MODE_PCE, "pce")):
+        for mode, name in modes:
             f = csi.PRNUFilter(h, w, apply_spectral_filter=(mode == _lib.MODE_PCE))
             out = torch.empty((batch, f.get_output_size() // 4), device="cuda")
             f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8)            # warm-up, sizes the workspace
@@ -351,7 +361,7 @@ def extraction_bench(torch, batch=32):
             res[f"{name}_ms_per_image{tag}"] = 1000.0 * dt / batch
             f.cleanup()
             del out
-    res.update({"batch": batch, "input": f"RGB8 {WIDTH}x{HEIGHT} and 512x512"})
+    res.update({"batch": batch, "input": "RGB8 " + " and ".join(f"{w}x{h}" for h, w, _ in shapes)})
     return res
 
 
@@ -429,7 +439,7 @@ def main():
 
     lib = _lib.load()
     if args.only:
-        leg = {"pce": lambda: pce_bench(torch), "extract": lambda: extraction_bench(torch),
+        leg = {"pce": lambda: pce_bench(torch), "extract": lambda: extraction_bench(torch, shape=args.extract_shape, only_mode=args.extract_mode),
                "groups": lambda: groups_bench(torch),
                "serving": lambda: serving_bench(torch, args.serving_seconds)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
