@@ -288,18 +288,19 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
     const double *rowmean = zm ? rowmean_all + (int64_t)b * 2 * h : nullptr;
     const double *g = zm ? g_all + (int64_t)b * 4 : nullptr;
     double sq = 0.0;
-    for (int j = threadIdx.x; j < w; j += kFftThreads) {
-        float v = img[(int64_t)i * w + j];
-        if (zm) v = zm_value(v, i, j, h, w, colmean, rowmean, g);
-        sq += (double)v * (double)v;
-        buf[j] = make_float2(v, 0.f);
-    }
+    const float *irow = img + (int64_t)i * w;
+    float2 *arow = A_all + (int64_t)b * n + (int64_t)i * w;
+    fft_row<-1>(
+        buf, d,
+        [&](int j) {
+            float v = irow[j];
+            if (zm) v = zm_value(v, i, j, h, w, colmean, rowmean, g);
+            sq += (double)v * (double)v;
+            return make_float2(v, 0.f);
+        },
+        [&](int j, float2 f) { arow[j] = f; });
     sq = block_sum_256(sq, sh);
     if (threadIdx.x == 0) rowsq_all[(int64_t)b * h + i] = sq;
-    __syncthreads();
-    fft_lds<-1>(buf, d);
-    float2 *A = A_all + (int64_t)b * n;
-    for (int j = threadIdx.x; j < w; j += kFftThreads) A[(int64_t)i * w + j] = buf[j];
 }
 
 // var = n/(n-1) * sum(x^2) over the row partials, FP64, fixed order.  grid (B)
@@ -364,10 +365,7 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, 
     const int w = d.n;
     const float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * w;
     float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i * w;
-    for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = arow[j];
-    __syncthreads();
-    fft_lds<1>(buf, d);
-    for (int j = threadIdx.x; j < w; j += kFftThreads) out[j] = buf[j].x * inv_n;
+    fft_row<1>(buf, d, [&](int j) { return arow[j]; }, [&](int j, float2 f) { out[j] = f.x * inv_n; });
 }
 
 // ---------------------------------------------------------------- spectrum
@@ -379,12 +377,13 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, c
     extern __shared__ float2 buf[];
     const int i = blockIdx.x;
     const int w = d.n, wh = w / 2 + 1;
-    const float *img = img_all + (int64_t)blockIdx.y * img_stride;
-    for (int j = threadIdx.x; j < w; j += kFftThreads) buf[j] = make_float2(img[(int64_t)i * w + j], 0.f);
-    __syncthreads();
-    fft_lds<-1>(buf, d);
+    const float *irow = img_all + (int64_t)blockIdx.y * img_stride + (int64_t)i * w;
     float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * wh;
-    for (int j = threadIdx.x; j < wh; j += kFftThreads) arow[j] = buf[j];
+    fft_row<-1>(
+        buf, d, [&](int j) { return make_float2(irow[j], 0.f); },
+        [&](int j, float2 f) {
+            if (j < wh) arow[j] = f;
+        });
     (void)h;
 }
 
@@ -399,14 +398,12 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, 
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
     float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)r * n;
-    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
-    __syncthreads();
-    fft_lds<-1>(buf, d);
-    for (int j = threadIdx.x; j < n; j += kFftThreads) {
-        const float2 f = buf[j];
-        row[j] = f;
-        prow[j] = f.x * f.x + f.y * f.y;
-    }
+    fft_row<-1>(
+        buf, d, [&](int j) { return row[j]; },
+        [&](int j, float2 f) {
+            row[j] = f;
+            prow[j] = f.x * f.x + f.y * f.y;
+        });
 }
 
 // Scale row r of B by S and run the inverse FFT (length d.n) in place.
@@ -418,10 +415,7 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
     const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * n;
-    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = cscale(row[j], srow[j]);
-    __syncthreads();
-    fft_lds<1>(buf, d);
-    for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
+    fft_row<1>(buf, d, [&](int j) { return cscale(row[j], srow[j]); }, [&](int j, float2 f) { row[j] = f; });
 }
 
 // grid (rows, B)
@@ -430,10 +424,7 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d
     extern __shared__ float2 buf[];
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * n;
-    for (int j = threadIdx.x; j < n; j += kFftThreads) buf[j] = row[j];
-    __syncthreads();
-    fft_lds<DIR>(buf, d);
-    for (int j = threadIdx.x; j < n; j += kFftThreads) row[j] = buf[j];
+    fft_row<DIR>(buf, d, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
 }
 
 // ------------------------------------------------------ column transforms

@@ -15,10 +15,21 @@ struct FftPlan {
     float2 *d_tw = nullptr;
 };
 
+// Radices 8, 4, 2 first (the twiddle exponent's j mod Ns is then a mask),
+// then 3 and 5; a power-of-two part 8^k * 2 becomes 8^(k-1) * 4 * 4 (the
+// same number of passes, no radix-2 pass).
 bool fft_factor(int n, int *radix, int *passes) {
-    int p = 0;
-    while (n % 4 == 0 && p < kFftMaxPasses) { radix[p++] = 4; n /= 4; }
-    while (n % 2 == 0 && p < kFftMaxPasses) { radix[p++] = 2; n /= 2; }
+    int p = 0, e2 = 0;
+    while (n % 2 == 0) { ++e2; n /= 2; }
+    int n8 = e2 / 3, n4 = 0, n2 = 0;
+    if (e2 % 3 == 2) n4 = 1;
+    else if (e2 % 3 == 1) {
+        if (n8 > 0) { --n8; n4 = 2; } else n2 = 1;
+    }
+    for (int i = 0; i < n8 && p < kFftMaxPasses; ++i) radix[p++] = 8;
+    for (int i = 0; i < n4 && p < kFftMaxPasses; ++i) radix[p++] = 4;
+    for (int i = 0; i < n2 && p < kFftMaxPasses; ++i) radix[p++] = 2;
+    if (p < n8 + n4 + n2) return false;
     while (n % 3 == 0 && p < kFftMaxPasses) { radix[p++] = 3; n /= 3; }
     while (n % 5 == 0 && p < kFftMaxPasses) { radix[p++] = 5; n /= 5; }
     *passes = p;

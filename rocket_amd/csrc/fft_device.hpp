@@ -1,4 +1,4 @@
-// Mixed-radix (2, 3, 4, 5) Stockham FFT of one row held in LDS.
+// Mixed-radix (2, 3, 4, 5, 8) Stockham FFT of one row held in LDS.
 //
 // A length-n transform, n = prod(radix[p]), runs as `passes` autosorting
 // passes.  Pass p with radix R and Ns = radix[0]*...*radix[p-1] does, for each
@@ -118,6 +118,39 @@ __device__ __forceinline__ void dft<5, -1>(float2 *a) { dft5<-1>(a); }
 template <>
 __device__ __forceinline__ void dft<5, 1>(float2 *a) { dft5<1>(a); }
 
+// Radix 8 as two radix-4 DFTs (even and odd inputs) joined by W8^q,
+// W8 = exp(DIR * 2*pi*i / 8).
+template <int DIR>
+__device__ __forceinline__ void dft8(float2 *a) {
+    constexpr float h = 0.707106781186547524400844362104849039f;  // sqrt(2)/2
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    dft4<DIR>(e);
+    dft4<DIR>(o);
+    const float2 o1 = DIR < 0 ? make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x))
+                              : make_float2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y));
+    const float2 o2 = DIR < 0 ? make_float2(o[2].y, -o[2].x) : make_float2(-o[2].y, o[2].x);
+    const float2 o3 = DIR < 0 ? make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y))
+                              : make_float2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y));
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+}
+template <>
+__device__ __forceinline__ void dft<8, -1>(float2 *a) { dft8<-1>(a); }
+template <>
+__device__ __forceinline__ void dft<8, 1>(float2 *a) { dft8<1>(a); }
+
+// j mod Ns for the butterfly's twiddle exponent; Ns is a power of two in
+// every pass before the first radix-3 or radix-5 pass (fft_factor orders the
+// radices 8, 4, 2, 3, 5).
+__device__ __forceinline__ int fft_mod(int j, int Ns) { return (Ns & (Ns - 1)) == 0 ? (j & (Ns - 1)) : j % Ns; }
+
 // One Stockham pass of radix R over buf[0..n) with kFftThreads threads.
 template <int R, int DIR>
 __device__ __forceinline__ void fft_pass(float2 *buf, int n, int Ns, const float2 *tw) {
@@ -130,7 +163,7 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int n, int Ns, const float
     for (int b = 0; b < kMaxB; ++b) {
         const int j = tid + b * kFftThreads;
         if (j < nb) {
-            const int k = j % Ns;
+            const int k = fft_mod(j, Ns);
             v[b][0] = buf[j];
 #pragma unroll
             for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[j + r * nb], twiddle<DIR>(tw, r * k * twstep));
@@ -142,7 +175,7 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int n, int Ns, const float
         const int j = tid + b * kFftThreads;
         if (j < nb) {
             dft<R, DIR>(v[b]);
-            const int k = j % Ns;
+            const int k = fft_mod(j, Ns);
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; ++q) buf[base + q * Ns] = v[b][q];
@@ -158,7 +191,9 @@ __device__ inline void fft_lds(float2 *buf, const FftDesc &d) {
     int Ns = 1;
     for (int p = 0; p < d.passes; ++p) {
         const int R = d.radix[p];
-        if (R == 4)
+        if (R == 8)
+            fft_pass<8, DIR>(buf, d.n, Ns, d.tw);
+        else if (R == 4)
             fft_pass<4, DIR>(buf, d.n, Ns, d.tw);
         else if (R == 2)
             fft_pass<2, DIR>(buf, d.n, Ns, d.tw);
@@ -168,6 +203,87 @@ __device__ inline void fft_lds(float2 *buf, const FftDesc &d) {
             fft_pass<5, DIR>(buf, d.n, Ns, d.tw);
         Ns *= R;
     }
+}
+
+// One pass of fft_row: as fft_pass, but the first pass (FIRST, Ns = 1, no
+// twiddles) takes its inputs from get(i) instead of LDS and the last pass
+// (LAST, Ns * R = n) hands its outputs to put(i, v) instead of LDS.  Both
+// touch i = j + r * (n/R) and i = j + q * Ns for consecutive j across the
+// threads, so global accesses made by get/put are coalesced.
+template <int R, int DIR, bool FIRST, bool LAST, class Get, class Put>
+__device__ __forceinline__ void fft_pass_io(float2 *buf, int n, int Ns, const float2 *tw, Get &get, Put &put) {
+    constexpr int kMaxB = (kFftMaxN / R + kFftThreads - 1) / kFftThreads;
+    const int nb = n / R;
+    const int twstep = n / (Ns * R);
+    const int tid = threadIdx.x;
+    float2 v[kMaxB][R];
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nb) {
+            if constexpr (FIRST) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = get(j + r * nb);
+            } else {
+                const int k = fft_mod(j, Ns);
+                v[b][0] = buf[j];
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[j + r * nb], twiddle<DIR>(tw, r * k * twstep));
+            }
+        }
+    }
+    if constexpr (!FIRST) __syncthreads();  // every read of buf precedes the writes below
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nb) {
+            dft<R, DIR>(v[b]);
+            const int k = FIRST ? 0 : fft_mod(j, Ns);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if constexpr (LAST)
+                    put(base + q * Ns, v[b][q]);
+                else
+                    buf[base + q * Ns] = v[b][q];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int DIR, bool FIRST, bool LAST, class Get, class Put>
+__device__ __forceinline__ void fft_pass_io_r(int R, float2 *buf, int n, int Ns, const float2 *tw, Get &get,
+                                              Put &put) {
+    if (R == 8)
+        fft_pass_io<8, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+    else if (R == 4)
+        fft_pass_io<4, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+    else if (R == 2)
+        fft_pass_io<2, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+    else if (R == 3)
+        fft_pass_io<3, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+    else
+        fft_pass_io<5, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+}
+
+// Transform of one row of length d.n from get(i) (i in [0, n), each read
+// once) to put(i, X[i]) (each written once), through LDS buf (n values) for
+// the passes between; ends with a barrier, so the caller may reuse buf.
+template <int DIR, class Get, class Put>
+__device__ inline void fft_row(float2 *buf, const FftDesc &d, Get get, Put put) {
+    const int P = d.passes;
+    if (P == 1) {
+        fft_pass_io_r<DIR, true, true>(d.radix[0], buf, d.n, 1, d.tw, get, put);
+        return;
+    }
+    fft_pass_io_r<DIR, true, false>(d.radix[0], buf, d.n, 1, d.tw, get, put);
+    int Ns = d.radix[0];
+    for (int p = 1; p < P - 1; ++p) {
+        fft_pass_io_r<DIR, false, false>(d.radix[p], buf, d.n, Ns, d.tw, get, put);
+        Ns *= d.radix[p];
+    }
+    fft_pass_io_r<DIR, false, true>(d.radix[P - 1], buf, d.n, Ns, d.tw, get, put);
 }
 
 // Values held in LDS by one multi-transform call (cols * n), bounded so the
@@ -190,7 +306,7 @@ __device__ __forceinline__ void fft_pass_multi(float2 *buf, int n, int cols, int
         const int t = tid + b * kFftThreads;
         if (t < total) {
             const int c = t / nb, j = t - c * nb;
-            const int k = j % Ns;
+            const int k = fft_mod(j, Ns);
             const float2 *col = buf + c * n;
             v[b][0] = col[j];
 #pragma unroll
@@ -204,7 +320,7 @@ __device__ __forceinline__ void fft_pass_multi(float2 *buf, int n, int cols, int
         if (t < total) {
             const int c = t / nb, j = t - c * nb;
             dft<R, DIR>(v[b]);
-            const int k = j % Ns;
+            const int k = fft_mod(j, Ns);
             const int base = c * n + (j - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; ++q) buf[base + q * Ns] = v[b][q];
@@ -219,7 +335,9 @@ __device__ inline void fft_lds_multi(float2 *buf, const FftDesc &d, int cols) {
     int Ns = 1;
     for (int p = 0; p < d.passes; ++p) {
         const int R = d.radix[p];
-        if (R == 4)
+        if (R == 8)
+            fft_pass_multi<8, DIR>(buf, d.n, cols, Ns, d.tw);
+        else if (R == 4)
             fft_pass_multi<4, DIR>(buf, d.n, cols, Ns, d.tw);
         else if (R == 2)
             fft_pass_multi<2, DIR>(buf, d.n, cols, Ns, d.tw);
