@@ -273,6 +273,7 @@ __global__ __launch_bounds__(256) void zm_apply_kernel(float *img_all, int64_t i
 // ------------------------------------------------------------------ Wiener
 // Row i of image b: (optional zero-mean) -> sum of squares (FP64) -> forward
 // FFT (w).  grid (h, B).
+template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img_all,
                                                                       int64_t img_stride, int h, float2 *A_all,
                                                                       double *rowsq_all, const double *colmean_all,
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
     double sq = 0.0;
     const float *irow = img + (int64_t)i * w;
     float2 *arow = A_all + (int64_t)b * n + (int64_t)i * w;
-    fft_row<-1>(
+    fft_row<-1, MAXN>(
         buf, d,
         [&](int j) {
             float v = irow[j];
@@ -358,6 +359,7 @@ __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, f
 
 // Inverse FFT (length w) of row i of A; out = Re / (w*h) (normalizeToReal).
 // grid (h, B)
+template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, const float2 *A, int64_t plane,
                                                                      float *out_all, int64_t out_stride, float inv_n) {
     extern __shared__ float2 buf[];
@@ -365,12 +367,13 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, 
     const int w = d.n;
     const float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * w;
     float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i * w;
-    fft_row<1>(buf, d, [&](int j) { return arow[j]; }, [&](int j, float2 f) { out[j] = f.x * inv_n; });
+    fft_row<1, MAXN>(buf, d, [&](int j) { return arow[j]; }, [&](int j, float2 f) { out[j] = f.x * inv_n; });
 }
 
 // ---------------------------------------------------------------- spectrum
 // Forward FFT (length w) of real row i; the first w/2+1 bins go to A (h x wh).
 // grid (h, B)
+template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img_all,
                                                                     int64_t img_stride, float2 *A, int64_t plane,
                                                                     int h) {
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, c
     const int w = d.n, wh = w / 2 + 1;
     const float *irow = img_all + (int64_t)blockIdx.y * img_stride + (int64_t)i * w;
     float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * wh;
-    fft_row<-1>(
+    fft_row<-1, MAXN>(
         buf, d, [&](int j) { return make_float2(irow[j], 0.f); },
         [&](int j, float2 f) {
             if (j < wh) arow[j] = f;
@@ -392,13 +395,14 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, c
 // the plane is transposed and columns become LDS row transforms.
 // In-place forward FFT of the rows of B (length d.n) and |F|^2 into P.
 // grid (rows, batch); planes `plane` elements apart.
+template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, float2 *B, float *P, int64_t plane) {
     extern __shared__ float2 buf[];
     const int r = blockIdx.x;
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
     float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)r * n;
-    fft_row<-1>(
+    fft_row<-1, MAXN>(
         buf, d, [&](int j) { return row[j]; },
         [&](int j, float2 f) {
             row[j] = f;
@@ -408,6 +412,7 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, 
 
 // Scale row r of B by S and run the inverse FFT (length d.n) in place.
 // grid (rows, B); planes `plane` apart.
+template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d, float2 *B, const float *S,
                                                                        int64_t plane) {
     extern __shared__ float2 buf[];
@@ -415,16 +420,16 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
     const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * n;
-    fft_row<1>(buf, d, [&](int j) { return cscale(row[j], srow[j]); }, [&](int j, float2 f) { row[j] = f; });
+    fft_row<1, MAXN>(buf, d, [&](int j) { return cscale(row[j], srow[j]); }, [&](int j, float2 f) { row[j] = f; });
 }
 
 // grid (rows, B)
-template <int DIR>
+template <int DIR, int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d, float2 *B, int64_t plane) {
     extern __shared__ float2 buf[];
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * n;
-    fft_row<DIR>(buf, d, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
+    fft_row<DIR, MAXN>(buf, d, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
 }
 
 // ------------------------------------------------------ column transforms
@@ -529,6 +534,16 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
 
 size_t fft_smem(const FftDesc &d) { return sizeof(float2) * d.n; }
 
+// Row-FFT kernel K<MAXN> over rows of length n: the small-register form up
+// to kFftSmallN values.
+#define PRNU_LAUNCH_ROW(K, n, grid, st, ...)                                                                    \
+    do {                                                                                                        \
+        if ((n) <= kFftSmallN)                                                                                  \
+            hipLaunchKernelGGL((K<kFftSmallN>), grid, dim3(kFftThreads), fft_row_smem(n), st, __VA_ARGS__);     \
+        else                                                                                                    \
+            hipLaunchKernelGGL((K<kFftMaxN>), grid, dim3(kFftThreads), fft_row_smem(n), st, __VA_ARGS__);       \
+    } while (0)
+
 template <int FMT>
 int launch_fastnoise_fmt(const void *in, int64_t in_stride, float *out, int64_t out_stride, int h, int w, int nb,
                          double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, hipStream_t st) {
@@ -593,8 +608,8 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     const int64_t n = c->pixels();
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    hipLaunchKernelGGL(wiener_rows_fwd_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img, img_stride,
-                       h, e.A, e.rowsq, zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
+    PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(h, nb), st, dw, img, img_stride, h, e.A, e.rowsq,
+                    zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
     PRNU_CHECK_LAUNCH();
     if (strips_for(h)) {
         PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, w, w, nb, nullptr, e.P, st)));
@@ -608,20 +623,18 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
         // transposed layout: B = A^T (w x h), column FFTs as rows of B; the
         // zero-padded box-min stencil is transpose invariant
         PRNU_TRY(launch_transpose_c(e.A, e.B, h, w, nb, n, st));
-        hipLaunchKernelGGL(rows_fft_power_kernel, dim3(w, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.P, n);
+        PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(w, nb), st, dh, e.B, e.P, n);
         PRNU_CHECK_LAUNCH();
         hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
         PRNU_CHECK_LAUNCH();
         hipLaunchKernelGGL(varest_scale_kernel, dim3((h + kVT - 1) / kVT, (w + kVT - 1) / kVT, nb), dim3(256), 0,
                            st, e.P, e.S, w, h, e.var);
         PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(rows_ifft_scaled_kernel, dim3(w, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, e.S,
-                           n);
+        PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(w, nb), st, dh, e.B, e.S, n);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY(launch_transpose_c(e.B, e.A, w, h, nb, n, st));
     }
-    hipLaunchKernelGGL(rows_ifft_real_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, e.A, n, out,
-                       out_stride, 1.0f / (float)(w * h));
+    PRNU_LAUNCH_ROW(rows_ifft_real_kernel, w, dim3(h, nb), st, dw, e.A, n, out, out_stride, 1.0f / (float)(w * h));
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -635,18 +648,21 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
     if (strips_for(h)) {
-        hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img,
-                           img_stride, out, out_stride, h);
+        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3(h, nb), st, dw, img, img_stride, out, out_stride, h);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
         return PRNU_OK;
     }
     const int64_t n = c->pixels();
-    hipLaunchKernelGGL(rows_fft_half_kernel, dim3(h, nb), dim3(kFftThreads), fft_smem(dw), st, dw, img, img_stride,
-                       e.A, n, h);
+    PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3(h, nb), st, dw, img, img_stride, e.A, n, h);
     PRNU_CHECK_LAUNCH();
     PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
-    hipLaunchKernelGGL(rows_fft_inplace_kernel<-1>, dim3(wh, nb), dim3(kFftThreads), fft_smem(dh), st, dh, e.B, n);
+    if (h <= kFftSmallN)
+        hipLaunchKernelGGL((rows_fft_inplace_kernel<-1, kFftSmallN>), dim3(wh, nb), dim3(kFftThreads),
+                           fft_row_smem(h), st, dh, e.B, n);
+    else
+        hipLaunchKernelGGL((rows_fft_inplace_kernel<-1, kFftMaxN>), dim3(wh, nb), dim3(kFftThreads),
+                           fft_row_smem(h), st, dh, e.B, n);
     PRNU_CHECK_LAUNCH();
     if (nb == 1 || out_stride == n) {
         PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st));

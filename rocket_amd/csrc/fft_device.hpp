@@ -210,9 +210,18 @@ __device__ inline void fft_lds(float2 *buf, const FftDesc &d) {
 // (LAST, Ns * R = n) hands its outputs to put(i, v) instead of LDS.  Both
 // touch i = j + r * (n/R) and i = j + q * Ns for consecutive j across the
 // threads, so global accesses made by get/put are coalesced.
-template <int R, int DIR, bool FIRST, bool LAST, class Get, class Put>
+// LDS bytes fft_row needs for a row of n values.  (A pad slot per 8 values
+// against bank conflicts of the strided writes measured slower: the padded
+// addresses cost registers, hence occupancy.)
+inline size_t fft_row_smem(int n) { return sizeof(float2) * (size_t)n; }
+
+// Rows up to kFftSmallN values run with per-thread butterfly arrays sized
+// for that length (fewer registers, more waves per SIMD).
+constexpr int kFftSmallN = 2048;
+
+template <int R, int DIR, bool FIRST, bool LAST, int MAXN, class Get, class Put>
 __device__ __forceinline__ void fft_pass_io(float2 *buf, int n, int Ns, const float2 *tw, Get &get, Put &put) {
-    constexpr int kMaxB = (kFftMaxN / R + kFftThreads - 1) / kFftThreads;
+    constexpr int kMaxB = (MAXN / R + kFftThreads - 1) / kFftThreads;
     const int nb = n / R;
     const int twstep = n / (Ns * R);
     const int tid = threadIdx.x;
@@ -252,38 +261,39 @@ __device__ __forceinline__ void fft_pass_io(float2 *buf, int n, int Ns, const fl
     __syncthreads();
 }
 
-template <int DIR, bool FIRST, bool LAST, class Get, class Put>
+template <int DIR, bool FIRST, bool LAST, int MAXN, class Get, class Put>
 __device__ __forceinline__ void fft_pass_io_r(int R, float2 *buf, int n, int Ns, const float2 *tw, Get &get,
                                               Put &put) {
     if (R == 8)
-        fft_pass_io<8, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+        fft_pass_io<8, DIR, FIRST, LAST, MAXN>(buf, n, Ns, tw, get, put);
     else if (R == 4)
-        fft_pass_io<4, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+        fft_pass_io<4, DIR, FIRST, LAST, MAXN>(buf, n, Ns, tw, get, put);
     else if (R == 2)
-        fft_pass_io<2, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+        fft_pass_io<2, DIR, FIRST, LAST, MAXN>(buf, n, Ns, tw, get, put);
     else if (R == 3)
-        fft_pass_io<3, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+        fft_pass_io<3, DIR, FIRST, LAST, MAXN>(buf, n, Ns, tw, get, put);
     else
-        fft_pass_io<5, DIR, FIRST, LAST>(buf, n, Ns, tw, get, put);
+        fft_pass_io<5, DIR, FIRST, LAST, MAXN>(buf, n, Ns, tw, get, put);
 }
 
-// Transform of one row of length d.n from get(i) (i in [0, n), each read
-// once) to put(i, X[i]) (each written once), through LDS buf (n values) for
-// the passes between; ends with a barrier, so the caller may reuse buf.
-template <int DIR, class Get, class Put>
+// Transform of one row of length d.n <= MAXN from get(i) (i in [0, n), each
+// read once) to put(i, X[i]) (each written once), through LDS buf
+// (fft_row_smem(n) bytes) for the passes between; ends with a barrier, so
+// the caller may reuse buf.
+template <int DIR, int MAXN, class Get, class Put>
 __device__ inline void fft_row(float2 *buf, const FftDesc &d, Get get, Put put) {
     const int P = d.passes;
     if (P == 1) {
-        fft_pass_io_r<DIR, true, true>(d.radix[0], buf, d.n, 1, d.tw, get, put);
+        fft_pass_io_r<DIR, true, true, MAXN>(d.radix[0], buf, d.n, 1, d.tw, get, put);
         return;
     }
-    fft_pass_io_r<DIR, true, false>(d.radix[0], buf, d.n, 1, d.tw, get, put);
+    fft_pass_io_r<DIR, true, false, MAXN>(d.radix[0], buf, d.n, 1, d.tw, get, put);
     int Ns = d.radix[0];
     for (int p = 1; p < P - 1; ++p) {
-        fft_pass_io_r<DIR, false, false>(d.radix[p], buf, d.n, Ns, d.tw, get, put);
+        fft_pass_io_r<DIR, false, false, MAXN>(d.radix[p], buf, d.n, Ns, d.tw, get, put);
         Ns *= d.radix[p];
     }
-    fft_pass_io_r<DIR, false, true>(d.radix[P - 1], buf, d.n, Ns, d.tw, get, put);
+    fft_pass_io_r<DIR, false, true, MAXN>(d.radix[P - 1], buf, d.n, Ns, d.tw, get, put);
 }
 
 // Values held in LDS by one multi-transform call (cols * n), bounded so the
