@@ -23,8 +23,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HEIGHT, WIDTH, COUNT, CAMERAS = 1536, 2048, 1024, 16
-PEAK_TFLOPS = {"split_bf16": 2500.0, "fp32": 157.3, "fp8": 5000.0}
-PRODUCTS = {"split_bf16": 3, "fp32": 1, "fp8": 1}
+PEAK_TFLOPS = {"split_bf16": 2500.0, "fp32": 157.3, "fp8": 5000.0, "fp16": 2500.0}
+PRODUCTS = {"split_bf16": 3, "fp32": 1, "fp8": 1, "fp16": 1}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -400,13 +400,17 @@ def pce_bench(torch):
     return res
 
 
-def fp8_bench(torch, p, gram, reps=5):
-    """All-pairs NCC of the bench patterns in FP8-e4m3 (per-pattern scale) from
-    a K-blocked FP8 store; max |dNCC| on the FP64 sample."""
+LOWP_STORES = {"fp8": "K-blocked e4m3, 128 values per block, scale max|x|/448 per pattern",
+               "fp16": "K-blocked FP16, 64 values per block, power-of-two scale per pattern, one MFMA product"}
+
+
+def lowp_bench(torch, p, gram, fmt="fp8", reps=5):
+    """All-pairs NCC of the bench patterns from a K-blocked FP8-e4m3 or FP16
+    store (per-pattern scale); max |dNCC| on the FP64 sample."""
     from rocket_amd import _lib, csi
     lib = _lib.load()
     out = torch.empty((COUNT, COUNT), dtype=torch.float64, device="cuda")
-    st = csi.PatternStore(HEIGHT * WIDTH, COUNT, "fp8")
+    st = csi.PatternStore(HEIGHT * WIDTH, COUNT, fmt)
     st.put(p)
     ms = ctypes.c_float()
     st.gram(gram, COUNT, out=out)                                   # warm-up
@@ -421,10 +425,10 @@ def fp8_bench(torch, p, gram, reps=5):
     step = 1000.0 * (time.perf_counter() - t1) / reps
     kernel = float(np.mean(kms))
     flops = 2.0 * HEIGHT * WIDTH * COUNT * (COUNT + 1) / 2.0
-    res = {"fp8_pairs_per_s": COUNT * (COUNT - 1) / 2 / (step / 1000.0), "fp8_ms_per_step": step,
-           "fp8_tile_kernel_ms": kernel, "fp8_tflops": flops / (kernel / 1000.0) / 1e12,
-           "fp8_max_abs_dncc": fp64_check(torch, p, out.cpu().numpy()),
-           "fp8_store": "K-blocked e4m3, 128 values per block, scale max|x|/448 per pattern"}
+    res = {f"{fmt}_pairs_per_s": COUNT * (COUNT - 1) / 2 / (step / 1000.0), f"{fmt}_ms_per_step": step,
+           f"{fmt}_tile_kernel_ms": kernel, f"{fmt}_tflops": flops / (kernel / 1000.0) / 1e12,
+           f"{fmt}_max_abs_dncc": fp64_check(torch, p, out.cpu().numpy()),
+           f"{fmt}_store": LOWP_STORES[fmt]}
     st.close()
     return res
 
@@ -519,9 +523,11 @@ def main():
         mp, kp = timed(ptr_step(prec), 3, 1)
         extra.update({"pointer_table_ms_per_step": mp, "pointer_table_tile_kernel_ms": kp})
         log(f"{prec} pointer table: {mp:.3f} ms/step")
-        f8 = fp8_bench(torch, p, gram)
-        extra.update(f8)
-        log(f"fp8: {f8['fp8_ms_per_step']:.3f} ms/step, max |dNCC| {f8['fp8_max_abs_dncc']:.2e}")
+        for fmt in ("fp16", "fp8"):
+            lp = lowp_bench(torch, p, gram, fmt)
+            extra.update(lp)
+            log(f"{fmt}: {lp[f'{fmt}_ms_per_step']:.3f} ms/step, max |dNCC| {lp[f'{fmt}_max_abs_dncc']:.2e}")
+        extra["fp16_speedup_vs_fp32"] = ms32 / extra["fp16_ms_per_step"]
     for st in stores.values():
         st.close()
 

@@ -65,7 +65,9 @@ typedef enum {
 typedef enum {
     PRNU_PREC_FP32 = 0,       /* FP32 MFMA (v_mfma_f32_16x16x4_f32) */
     PRNU_PREC_SPLIT_BF16 = 1, /* hi+lo BF16, three MFMA products, FP32 accumulate */
-    PRNU_PREC_FP8 = 2         /* e4m3 with per-pattern scale, FP32 accumulate */
+    PRNU_PREC_FP8 = 2,        /* e4m3 with per-pattern scale, FP32 accumulate */
+    PRNU_PREC_FP16 = 3        /* FP16 with a per-pattern power-of-two scale, one MFMA product, FP32
+                                 accumulate; pattern stores only */
 } prnu_precision;
 
 typedef enum {
@@ -199,9 +201,11 @@ PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int
  * device cache slots Rocket keeps preprocessed patterns in
  * (R/cache/DeviceCache.java:19-44), laid out for the Gram: each pattern is cut
  * into 128-byte blocks and block b of all slots is contiguous, stored as
- * split-BF16 (hi, lo; 32 values per block), FP32 (32 values) or FP8 e4m3
- * (128 values of x / scale, scale = max|x| / 448 per pattern) for `format` =
- * PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32 or PRNU_PREC_FP8.  Inserting computes
+ * split-BF16 (hi, lo; 32 values per block), FP32 (32 values), FP8 e4m3
+ * (128 values of x / scale, scale = max|x| / 448 per pattern) or FP16 (64
+ * values of x / scale, scale the power of two with max|x| / scale in
+ * [2^14, 2^15)) for `format` = PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32,
+ * PRNU_PREC_FP8 or PRNU_PREC_FP16.  Inserting computes
  * the per-pattern FP64 statistics (FP64 sums of FP32 products, as the
  * reference's NCC), so the NCC epilogue is exact whatever the format. */
 typedef struct prnu_store prnu_store;

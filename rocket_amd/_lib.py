@@ -17,12 +17,12 @@ HEADER = os.path.join(ROOT, "include", "prnu_api.h")
 OK, EINVAL, ENOMEM, EDEVICE, ENOTSUP = 0, 1, 2, 3, 4
 MODE_NCC, MODE_PCE = 0, 1
 INPUT_RGB8, INPUT_GRAY8, INPUT_FLOAT32 = 0, 1, 2
-PREC_FP32, PREC_SPLIT_BF16, PREC_FP8 = 0, 1, 2
+PREC_FP32, PREC_SPLIT_BF16, PREC_FP8, PREC_FP16 = 0, 1, 2, 3
 PCE_REFERENCE, PCE_TEXTBOOK = 0, 1
 INDEX_HILBERT, INDEX_TILING = 0, 1
 PLAN_PER_ITEM, PLAN_PERSISTENT = 0, 1
 
-PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp8": PREC_FP8}
+PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp8": PREC_FP8, "fp16": PREC_FP16}
 
 
 class PrnuError(RuntimeError):

@@ -409,7 +409,8 @@ class GramNCC:
 
 class PatternStore:
     """Device-resident, K-blocked pattern set for the batched NCC path (see
-    ``prnu_store_*`` in include/prnu_api.h): ``format`` "split_bf16" or "fp32".
+    ``prnu_store_*`` in include/prnu_api.h): ``format`` "split_bf16", "fp32",
+    "fp8" (e4m3, scale max|x|/448) or "fp16" (power-of-two scale, one product).
     The role of Rocket's device cache slots (``R/cache/DeviceCache.java``),
     laid out so that one Gram K stage reads contiguous memory."""
 

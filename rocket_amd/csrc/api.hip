@@ -254,7 +254,7 @@ int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int count, int 
 
 int prnu_gram_set_config(prnu_ctx *ctx, int precision, const prnu_gram_config *cfg) {
     PRNU_REQUIRE(ctx, "gram_set_config: null context");
-    PRNU_REQUIRE(precision >= 0 && precision <= 2, "gram_set_config: unknown precision %d", precision);
+    PRNU_REQUIRE(precision >= 0 && precision <= 3, "gram_set_config: unknown precision %d", precision);
     Context *c = reinterpret_cast<Context *>(ctx);
     if (!cfg) {
         c->gram_cfg[precision] = prnu_gram_config{0, 1, 0};

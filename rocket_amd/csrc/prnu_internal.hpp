@@ -64,7 +64,7 @@ struct Context {
     Workspace ws_fp8;       // FP8 copy of a pattern set and its scales
     FftPlan *fft_w = nullptr;       // length-w plan
     FftPlan *fft_h = nullptr;       // length-h plan
-    prnu_gram_config gram_cfg[3] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}};  // per precision; slabs 0 = automatic
+    prnu_gram_config gram_cfg[4] = {{0, 1, 0}, {0, 1, 0}, {0, 1, 0}, {0, 1, 0}};  // per precision; slabs 0 = automatic
     bool timing = false;            // record events around the dominant kernel
     bool timed = false;             // ev_begin/ev_end hold a measurement
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;

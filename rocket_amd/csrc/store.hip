@@ -6,7 +6,9 @@
 // A block is 128 bytes: split-BF16 stores hi = rne(x) (32 bf16) then
 // lo = rne(x - hi) (32 bf16); FP32 stores the 32 floats.  The FP8 format
 // instead cuts patterns into blocks of 128 values and stores e4m3(x / scale),
-// scale = max|x| / 448 per pattern.  Values past n are 0.
+// scale = max|x| / 448 per pattern; the FP16 format into blocks of 64 values
+// of rne(x / scale), scale = the power of two with max|x| / scale in
+// [2^14, 2^15).  Values past n are 0.
 //
 // Why.  One K stage of a 256 x 256 Gram tile reads 32 values of 512 patterns.
 // With row-major patterns those are 512 reads 4*n bytes apart, one page each;
@@ -31,7 +33,7 @@ struct Store {
     uint8_t *blocks = nullptr;
     double *sum = nullptr, *sumsq = nullptr;
     float *absmax = nullptr;
-    float *scale = nullptr;  // FP8: max|x| / 448 per slot
+    float *scale = nullptr;  // FP8: max|x| / 448, FP16: a power of two, per slot
     Workspace ws_part;  // statistics partials
 };
 
@@ -40,9 +42,11 @@ namespace {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int intx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBlk = 32;        // values per block (split-BF16, FP32)
 constexpr int kBlk8 = 128;      // values per block (FP8)
+constexpr int kBlk16 = 64;      // values per block (FP16)
 constexpr float kE4M3Max = 448.0f;
 constexpr int kBlkBytes = 128;  // bytes per block
 constexpr int kTileS = 256;
@@ -172,6 +176,51 @@ __global__ __launch_bounds__(256) void store_put_fp8_kernel(const float *const *
                                            (sub & 3) * 32);
     dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
     dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// FP16 insert: thread -> 32 values (half a 64-value block) of one pattern,
+// multiplied by the power of two 2^(15 - e), max|x| < 2^e, and rounded to
+// nearest; grid (ceil(2*nkb/256), count).  The exponent is clamped from
+// below so the multiplier of a tiny pattern stays a finite float.
+__global__ __launch_bounds__(256) void store_put_fp16_kernel(const float *const *ptrs, int count, int64_t n,
+                                                             int64_t nkb, uint8_t *blocks, int cap_pad, int first,
+                                                             const float *absmax, float *scale) {
+    const int p = blockIdx.y;
+    const int64_t sub = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const float am = absmax[first + p];
+    int e = 0;
+    (void)frexpf(am, &e);  // am = m * 2^e, m in [0.5, 1): am < 2^e
+    e = am > 0.f ? max(-100, e) : 15;
+    const float mult = ldexpf(1.0f, 15 - e);
+    if (sub == 0) scale[first + p] = ldexpf(1.0f, e - 15);
+    if (sub >= 2 * nkb) return;
+    const float *x = ptrs[p];
+    const int64_t k0 = sub * 32;
+    float v[32];
+    if (k0 + 32 <= n && ((reinterpret_cast<uintptr_t>(x + k0) & 15) == 0)) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 f = reinterpret_cast<const float4 *>(x + k0)[q];
+            v[4 * q] = f.x;
+            v[4 * q + 1] = f.y;
+            v[4 * q + 2] = f.z;
+            v[4 * q + 3] = f.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) v[q] = (k0 + q < n) ? x[k0 + q] : 0.f;
+    }
+    uint32_t w[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint16_t h0 = __builtin_bit_cast(uint16_t, (_Float16)(v[2 * q] * mult));
+        const uint16_t h1 = __builtin_bit_cast(uint16_t, (_Float16)(v[2 * q + 1] * mult));
+        w[q] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(blocks + ((size_t)(sub >> 1) * cap_pad + first + p) * kBlkBytes +
+                                           (sub & 1) * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
 }
 
 // One 256-thread workgroup per pattern: strided partial sums, then a fixed
@@ -312,6 +361,7 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
                                           int tid = threadIdx.x) {
     constexpr bool kSplit = FMT == PRNU_PREC_SPLIT_BF16;
     constexpr bool kFp8 = FMT == PRNU_PREC_FP8;
+    constexpr bool kF16 = FMT == PRNU_PREC_FP16;
     constexpr bool DMA = VAR != 1;
     constexpr bool kPrio = VAR == 2;
     const bool half = w.rows_y < 0;
@@ -423,6 +473,28 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
                 for (int cb = 0; cb < 4; ++cb)
                     acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aq, bq[cb], acc[rb][cb], 0, 0, 0,
                                                                                    127, 0, 127);
+            }
+        } else if (kF16) {
+            // one 64-value block per stage as two K = 32 halves: chunks
+            // 0..3 and 4..7, lane k-group ch = lane >> 4 in each
+            const int ch = lane >> 4;
+            halfx8 b0[4], b1[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const int r = wcol0 + cb * 16 + frow;
+                b0[cb] = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch));
+                b1[cb] = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch + 4));
+            }
+#pragma unroll
+            for (int rb = 0; rb < 8; ++rb) {
+                const int r = wrow0 + rb * 16 + frow;
+                const halfx8 a0 = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch));
+                const halfx8 a1 = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch + 4));
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b0[cb], acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b1[cb], acc[rb][cb], 0, 0, 0);
+                }
             }
         } else if (kSplit) {
             const int ch = lane >> 4;  // hi chunks 0..3, lo chunks 4..7
@@ -670,6 +742,26 @@ __global__ __launch_bounds__(kThr4, 1) void store_gram4_kernel(StoreGramArgs a) 
                     acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aq, bq[cb], acc[rb][cb], 0, 0, 0,
                                                                                    127, 0, 127);
             }
+        } else if constexpr (FMT == PRNU_PREC_FP16) {
+            // one 64-value block per stage as two K = 32 halves (chunks ch, ch + 4)
+            halfx8 b0[8], b1[8];
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const int r = b_off + cb * 16 + frow;
+                b0[cb] = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch));
+                b1[cb] = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch + 4));
+            }
+#pragma unroll
+            for (int rb = 0; rb < 8; ++rb) {
+                const int r = a_off + rb * 16 + frow;
+                const halfx8 a0 = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch));
+                const halfx8 a1 = *reinterpret_cast<const halfx8 *>(buf + s_off(r, ch + 4));
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb) {
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b0[cb], acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b1[cb], acc[rb][cb], 0, 0, 0);
+                }
+            }
         } else {
             bf16x8 bh[8], bl[8];
 #pragma unroll
@@ -875,7 +967,9 @@ __global__ __launch_bounds__(256) void store_reduce_rect_kernel(const float *__r
 }  // namespace
 
 // Values per 128-byte block of a store format.
-int block_values(int format) { return format == PRNU_PREC_FP8 ? kBlk8 : kBlk; }
+int block_values(int format) {
+    return format == PRNU_PREC_FP8 ? kBlk8 : format == PRNU_PREC_FP16 ? kBlk16 : kBlk;
+}
 
 // Work items per K slab: off-diagonal tiles plus diagonal packs (item_waves).
 int host_items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
@@ -917,8 +1011,10 @@ int64_t choose_kcb(const int *items, const int *tiles, const int64_t *nkb, int n
 int store_create(int64_t n, int capacity, int format, Store **out) {
     *out = nullptr;
     PRNU_REQUIRE(n >= 1 && capacity >= 1, "store_create: n and capacity must be positive");
-    PRNU_REQUIRE(format == PRNU_PREC_SPLIT_BF16 || format == PRNU_PREC_FP32 || format == PRNU_PREC_FP8,
-                 "store_create: format must be PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32 or PRNU_PREC_FP8");
+    PRNU_REQUIRE(format == PRNU_PREC_SPLIT_BF16 || format == PRNU_PREC_FP32 || format == PRNU_PREC_FP8 ||
+                     format == PRNU_PREC_FP16,
+                 "store_create: format must be PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32, PRNU_PREC_FP8 or "
+                 "PRNU_PREC_FP16");
     Store *s = new (std::nothrow) Store();
     if (!s) {
         set_error("store_create: out of host memory");
@@ -936,7 +1032,8 @@ int store_create(int64_t n, int capacity, int format, Store **out) {
     if (e == hipSuccess) e = hipMalloc(&s->sum, sizeof(double) * s->cap_pad);
     if (e == hipSuccess) e = hipMalloc(&s->sumsq, sizeof(double) * s->cap_pad);
     if (e == hipSuccess) e = hipMalloc(&s->absmax, sizeof(float) * s->cap_pad);
-    if (e == hipSuccess && format == PRNU_PREC_FP8) e = hipMalloc(&s->scale, sizeof(float) * s->cap_pad);
+    if (e == hipSuccess && (format == PRNU_PREC_FP8 || format == PRNU_PREC_FP16))
+        e = hipMalloc(&s->scale, sizeof(float) * s->cap_pad);
     // empty slots read as zeros (the Gram masks them anyway)
     if (e == hipSuccess) e = hipMemset(s->blocks, 0, bytes);
     if (e != hipSuccess) {
@@ -976,6 +1073,17 @@ int store_put(Store *s, const float *const *d_ptrs, int count, int first, hipStr
                                       s->ws_part, st));
         const dim3 grid8((unsigned)((4 * s->nkb + 255) / 256), (unsigned)count);
         hipLaunchKernelGGL(store_put_fp8_kernel, grid8, dim3(256), 0, st, d_ptrs, count, s->n, s->nkb, s->blocks,
+                           s->cap_pad, first, s->absmax, s->scale);
+        PRNU_CHECK_LAUNCH();
+        return PRNU_OK;
+    }
+    if (s->format == PRNU_PREC_FP16) {
+        // statistics (and max|x|, which sets the power-of-two scale) first
+        PRNU_REQUIRE(count <= 65535, "store_put: at most 65535 FP16 patterns per call");
+        PRNU_TRY(launch_pattern_stats(d_ptrs, count, s->n, s->sum + first, s->sumsq + first, s->absmax + first,
+                                      s->ws_part, st));
+        const dim3 grid16((unsigned)((2 * s->nkb + 255) / 256), (unsigned)count);
+        hipLaunchKernelGGL(store_put_fp16_kernel, grid16, dim3(256), 0, st, d_ptrs, count, s->n, s->nkb, s->blocks,
                            s->cap_pad, first, s->absmax, s->scale);
         PRNU_CHECK_LAUNCH();
         return PRNU_OK;
@@ -1027,11 +1135,13 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const int64_t S8 = a.xcd_remap ? (a.S + 7) / 8 * 8 : a.S;
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
-    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 + priority, 3 four waves (split)
+    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 + priority, 3 four waves (not FP32)
     if (var == 3 && s->format != PRNU_PREC_FP32) {
         // same items per slab as the 8-wave packing (ceil(3nt/4) = ceil(6nt/8))
         if (s->format == PRNU_PREC_FP8)
             hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_FP8>, grid, dim3(kThr4), 0, st, a);
+        else if (s->format == PRNU_PREC_FP16)
+            hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_FP16>, grid, dim3(kThr4), 0, st, a);
         else
             hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_SPLIT_BF16>, grid, dim3(kThr4), 0, st, a);
         PRNU_CHECK_LAUNCH();
@@ -1055,6 +1165,8 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
         PRNU_STORE_GRAM(PRNU_PREC_SPLIT_BF16);
     else if (s->format == PRNU_PREC_FP8)
         PRNU_STORE_GRAM(PRNU_PREC_FP8);
+    else if (s->format == PRNU_PREC_FP16)
+        PRNU_STORE_GRAM(PRNU_PREC_FP16);
     else
         PRNU_STORE_GRAM(PRNU_PREC_FP32);
 #undef PRNU_STORE_GRAM
@@ -1211,6 +1323,8 @@ int plan_launch(GramPlan *p, hipStream_t st) {
         PRNU_GROUPS_GRAM(PRNU_PREC_SPLIT_BF16);
     else if (p->format == PRNU_PREC_FP8)
         PRNU_GROUPS_GRAM(PRNU_PREC_FP8);
+    else if (p->format == PRNU_PREC_FP16)
+        PRNU_GROUPS_GRAM(PRNU_PREC_FP16);
     else
         PRNU_GROUPS_GRAM(PRNU_PREC_FP32);
 #undef PRNU_GROUPS_GRAM

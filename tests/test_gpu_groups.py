@@ -251,7 +251,7 @@ def test_block_plan_rectangles_and_index_space_schedule():
     gram.close()
 
 
-@pytest.mark.parametrize("fmt", ["split_bf16", "fp32", "fp8"])
+@pytest.mark.parametrize("fmt", ["split_bf16", "fp32", "fp8", "fp16"])
 def test_persistent_launch_bitwise_equal(fmt):
     """The persistent launch (one workgroup per CU over per-XCD queues) gives
     the per-item launch's results bitwise, for triangles of several stores and
@@ -288,4 +288,63 @@ def test_persistent_launch_bitwise_equal(fmt):
     small.close()
     for st in stores:
         st.close()
+    gram.close()
+
+
+def fp16_bound(n):
+    """FP16 rounding (unit roundoff u = 2^-11) of both operands: the NCC error
+    of unit-variance patterns has standard deviation about u*sqrt(2/3)/sqrt(n)
+    off the diagonal and 2u/sqrt(3n)*sqrt(3) on it; 6 sigma, plus 2e-6 for
+    the FP32 accumulation inside the MFMA."""
+    u = 2.0 ** -11
+    return 6 * u * np.sqrt(2.0 / 3.0) / np.sqrt(n) + 2e-6, 6 * 2 * u / np.sqrt(n) + 2e-6
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_fp16_store_gram_within_rounding_bound(variant):
+    """FP16 store (blocks of 64 values, power-of-two scale per pattern): n not
+    a multiple of 64, ragged count, offset slots, every load variant; within
+    the FP16 rounding bound of the FP64 oracle and of the split-BF16 store."""
+    import ctypes
+    from rocket_amd import _lib
+    n, count, cap, first = 200 * 321, 300, 310, 3
+    p = patterns(count, n, 4, seed=1601)
+    st = csi.PatternStore(n, cap, "fp16")
+    st.put(p, first_slot=first)
+    gram = csi.GramNCC(200, 321)
+    cfg = (ctypes.c_int32 * 3)(0, 1, variant)
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, _lib.PREC_FP16, cfg))
+    got = st.gram(gram, count, first).cpu().numpy()
+    _lib.check(_lib.load().prnu_gram_set_config(gram._ctx.handle, _lib.PREC_FP16, None))
+    want = O.ncc_matrix(p.cpu().numpy())
+    off = ~np.eye(count, dtype=bool)
+    b_off, b_diag = fp16_bound(n)
+    assert np.abs(got - want)[off].max() <= b_off
+    assert np.abs(np.diag(got) - np.diag(want)).max() <= b_diag
+    assert np.array_equal(got, got.T)
+    split = csi.PatternStore(n, cap, "split_bf16")
+    split.put(p, first_slot=first)
+    ref = split.gram(gram, count, first).cpu().numpy()
+    assert np.abs(got - ref)[off].max() <= b_off
+    s, ss, am = st.stats(count, first)
+    assert np.array_equal(am, np.abs(p.cpu().numpy()).max(axis=1))
+    split.close()
+    st.close()
+    gram.close()
+
+
+def test_fp16_store_scale_is_exact():
+    """The per-pattern scale is a power of two, so patterns multiplied by 2^-20
+    or 2^40 (exact in FP32, and their FP32 squares stay normal) store the same
+    FP16 values and give bitwise the same NCC."""
+    n, count = 96 * 70, 40
+    p = patterns(count, n, 2, seed=1602)
+    gram = csi.GramNCC(8, 8)
+    outs = []
+    for k in (0, -20, 40):
+        st = csi.PatternStore(n, count, "fp16")
+        st.put(p * (2.0 ** k))
+        outs.append(st.gram(gram, count).cpu().numpy())
+        st.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     gram.close()
