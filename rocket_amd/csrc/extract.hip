@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void gray_kernel(const uint8_t *in, float *out
 }
 
 // ---------------------------------------------------------------- FastNoise
-constexpr int kFnTX = 64, kFnTY = 16;  // output tile
+constexpr int kFnTX = 64, kFnTY = 32;  // output tile
 
 // One-sided at the borders, central inside (fastnoisefilter.cu:355-381).
 __device__ __forceinline__ float grad1(float left, float center, float right, int pos, int len) {
@@ -77,7 +77,7 @@ __device__ __forceinline__ float grad1(float left, float center, float right, in
     return 0.5f * (right - left);
 }
 
-// Partial sums of the zero-mean pass for one 64 x 16 output tile held in O:
+// Partial sums of the zero-mean pass for one 64 x 32 output tile held in O:
 // per column the sums of even and odd rows, per row the sums of even and odd
 // columns (FP64).
 __device__ __forceinline__ void zm_tile_sums(const float (*O)[kFnTX + 1], int x0, int y0, int h, int w,
@@ -108,20 +108,32 @@ __device__ __forceinline__ void zm_tile_sums(const float (*O)[kFnTX + 1], int x0
     }
 }
 
-// Gray conversion + FastNoise for a 64 x 16 tile with a 2-pixel halo, and
-// optionally the zero-mean partial sums of the output tile.
-// grid (tiles_x, tiles_y, batch).
+// Gray conversion + FastNoise for a 64 x 32 tile with a 2-pixel halo, and
+// optionally the zero-mean partial sums of the output tile.  Thread = one
+// output column (lane) and the rows wave + 4k, k < 8, all of one parity:
+// its column partial is a register sum, and the even- and odd-column sums of
+// a row are one 5-step shuffle reduction of the wave (offsets 2..32 keep the
+// lane parity apart).  Tiles are numbered image by image, row by row; XCD x
+// (workgroup ids go to the XCDs round-robin) takes tiles [x * per, (x+1) *
+// per) in order, so neighbouring tiles share halo lines in its L2.
+// grid (8 * per)
 template <int FMT>
 __global__ __launch_bounds__(256) void fastnoise_kernel(const uint8_t *in_all, int64_t in_stride, float *out_all,
                                                         int64_t out_stride, int h, int w, double *colpart_all,
-                                                        int64_t cp_stride, double *rowpart_all, int64_t rp_stride) {
+                                                        int64_t cp_stride, double *rowpart_all, int64_t rp_stride,
+                                                        int ntx, int nty, int ntiles) {
     __shared__ float G[kFnTY + 4][kFnTX + 4];
     __shared__ float U[kFnTY + 2][kFnTX + 2];
     __shared__ float V[kFnTY + 2][kFnTX + 2];
-    __shared__ float O[kFnTY][kFnTX + 1];
-    const uint8_t *in = in_all + (int64_t)blockIdx.z * in_stride;
-    float *out = out_all + (int64_t)blockIdx.z * out_stride;
-    const int x0 = blockIdx.x * kFnTX, y0 = blockIdx.y * kFnTY;
+    __shared__ double csum[4][kFnTX];
+    const int per = (ntiles + 7) / 8;
+    const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    const int bz = t / (ntx * nty), rem = t - bz * (ntx * nty);
+    const int tyi = rem / ntx, txi = rem - (rem / ntx) * ntx;
+    const uint8_t *in = in_all + (int64_t)bz * in_stride;
+    float *out = out_all + (int64_t)bz * out_stride;
+    const int x0 = txi * kFnTX, y0 = tyi * kFnTY;
     const int tid = threadIdx.x;
 
     for (int idx = tid; idx < (kFnTY + 4) * (kFnTX + 4); idx += 256) {
@@ -140,8 +152,10 @@ __global__ __launch_bounds__(256) void fastnoise_kernel(const uint8_t *in_all, i
             const int cy = ly + 1, cx = lx + 1;  // position in G
             const float dx = grad1(G[cy][cx - 1], G[cy][cx], G[cy][cx + 1], gx, w);
             const float dy = grad1(G[cy - 1][cx], G[cy][cx], G[cy + 1][cx], gy, h);
-            const float norm = sqrtf(dx * dx + dy * dy);
-            const float scale = 1.0f / (1.0f + norm);
+            // hardware square root and reciprocal (about 1 ulp each) in place
+            // of the correctly rounded library sequences
+            const float norm = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+            const float scale = __builtin_amdgcn_rcpf(1.0f + norm);
             u = scale * dx;
             v = scale * dy;
         }
@@ -149,23 +163,44 @@ __global__ __launch_bounds__(256) void fastnoise_kernel(const uint8_t *in_all, i
         V[ly][lx] = v;
     }
     __syncthreads();
-    for (int idx = tid; idx < kFnTY * kFnTX; idx += 256) {
-        const int ly = idx / kFnTX, lx = idx % kFnTX;
-        const int gy = y0 + ly, gx = x0 + lx;
+    // G is free now: it holds the output tile O (32 x 64) for the row sums
+    float(*O)[kFnTX + 4] = G;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int gx = x0 + lane;
+    const bool zm = colpart_all != nullptr;
+    double col = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFnTY / 4; ++k) {
+        const int ly = wave + 4 * k, gy = y0 + ly;
         float o = 0.f;
         if (gy < h && gx < w) {
-            const int cy = ly + 1, cx = lx + 1;  // position in U/V
+            const int cy = ly + 1, cx = lane + 1;  // position in U/V
             const float a = grad1(U[cy][cx - 1], U[cy][cx], U[cy][cx + 1], gx, w);
             const float b = grad1(V[cy - 1][cx], V[cy][cx], V[cy + 1][cx], gy, h);
             o = a + b;
             out[(int64_t)gy * w + gx] = o;
         }
-        O[ly][lx] = o;
+        col += (double)o;
+        O[ly][lane] = o;
     }
-    if (!colpart_all) return;
+    if (!zm) return;
+    csum[wave][lane] = col;
     __syncthreads();
-    zm_tile_sums(O, x0, y0, h, w, colpart_all + (int64_t)blockIdx.z * cp_stride,
-                 rowpart_all + (int64_t)blockIdx.z * rp_stride);
+    if (tid < kFnTX && gx < w) {
+        // y0 is even, so waves 0 and 2 hold the even rows
+        double *cp = colpart_all + (int64_t)bz * cp_stride + ((int64_t)tyi * w + gx) * 2;
+        cp[0] = csum[0][tid] + csum[2][tid];
+        cp[1] = csum[1][tid] + csum[3][tid];
+    }
+    // row r, column parity p: four threads (quad c = 0..3) sum 8 columns each
+    const int r = tid >> 3, p = (tid >> 2) & 1, c = tid & 3;
+    double rs = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rs += (double)O[r][p + 16 * c + 2 * i];
+    rs += __shfl_xor(rs, 1, 64);
+    rs += __shfl_xor(rs, 2, 64);
+    const int gy = y0 + r;
+    if (c == 0 && gy < h) rowpart_all[(int64_t)bz * rp_stride + ((int64_t)txi * h + gy) * 2 + p] = rs;
 }
 
 // -------------------------------------------------------------- zero-mean
@@ -271,37 +306,61 @@ __global__ __launch_bounds__(256) void zm_apply_kernel(float *img_all, int64_t i
 }
 
 // ------------------------------------------------------------------ Wiener
-// Row i of image b: (optional zero-mean) -> sum of squares (FP64) -> forward
-// FFT (w).  grid (h, B).
+// The input is real, so its 2-D spectrum F is Hermitian and the filter runs
+// on half spectra (h x (w/2+1)): Re IFFT2(F S) = IFFT2(F S'), with
+// S'(k) = (S(k) + S(-k)) / 2, and F S' is Hermitian again, so one complex
+// inverse transform over the w/2+1 columns and a real inverse over the rows
+// give the reference's real part.  Rows go two at a time through one complex
+// transform z = a + i b of length w: A_k = (Z_k + conj Z_{w-k}) / 2 and
+// B_k = (Z_k - conj Z_{w-k}) / 2i.
+
+// Rows 2r and 2r+1 of image b: (optional zero-mean) -> sums of squares
+// (FP64) -> forward FFT (w) -> half spectra into A (h x (w/2+1)).
+// grid (ceil(h/2), B).
 template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img_all,
                                                                       int64_t img_stride, int h, float2 *A_all,
-                                                                      double *rowsq_all, const double *colmean_all,
+                                                                      int64_t plane, double *rowsq_all,
+                                                                      const double *colmean_all,
                                                                       const double *rowmean_all, const double *g_all) {
     extern __shared__ float2 buf[];
     __shared__ double sh[4];
-    const int i = blockIdx.x, b = blockIdx.y;
-    const int w = d.n;
-    const int64_t n = (int64_t)h * w;
+    const int i0 = 2 * blockIdx.x, i1 = i0 + 1, b = blockIdx.y;
+    const bool two = i1 < h;
+    const int w = d.n, wh = w / 2 + 1;
     const float *img = img_all + (int64_t)b * img_stride;
     const bool zm = colmean_all != nullptr;
     const double *colmean = zm ? colmean_all + (int64_t)b * 2 * w : nullptr;
     const double *rowmean = zm ? rowmean_all + (int64_t)b * 2 * h : nullptr;
     const double *g = zm ? g_all + (int64_t)b * 4 : nullptr;
-    double sq = 0.0;
-    const float *irow = img + (int64_t)i * w;
-    float2 *arow = A_all + (int64_t)b * n + (int64_t)i * w;
+    double sq0 = 0.0, sq1 = 0.0;
+    const float *r0 = img + (int64_t)i0 * w;
+    const float *r1 = img + (int64_t)(two ? i1 : i0) * w;
     fft_row<-1, MAXN>(
         buf, d,
         [&](int j) {
-            float v = irow[j];
-            if (zm) v = zm_value(v, i, j, h, w, colmean, rowmean, g);
-            sq += (double)v * (double)v;
-            return make_float2(v, 0.f);
+            float va = r0[j], vb = two ? r1[j] : 0.f;
+            if (zm) {
+                va = zm_value(va, i0, j, h, w, colmean, rowmean, g);
+                if (two) vb = zm_value(vb, i1, j, h, w, colmean, rowmean, g);
+            }
+            sq0 += (double)va * (double)va;
+            sq1 += (double)vb * (double)vb;
+            return make_float2(va, vb);
         },
-        [&](int j, float2 f) { arow[j] = f; });
-    sq = block_sum_256(sq, sh);
-    if (threadIdx.x == 0) rowsq_all[(int64_t)b * h + i] = sq;
+        [&](int j, float2 f) { buf[j] = f; });  // the last pass has read buf before it writes
+    float2 *a0 = A_all + (int64_t)b * plane + (int64_t)i0 * wh;
+    for (int k = threadIdx.x; k < wh; k += kFftThreads) {
+        const float2 zk = buf[k], zm_ = buf[k == 0 ? 0 : w - k];
+        a0[k] = make_float2(0.5f * (zk.x + zm_.x), 0.5f * (zk.y - zm_.y));
+        if (two) a0[wh + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
+    }
+    sq0 = block_sum_256(sq0, sh);
+    sq1 = block_sum_256(sq1, sh);
+    if (threadIdx.x == 0) {
+        rowsq_all[(int64_t)b * h + i0] = sq0;
+        if (two) rowsq_all[(int64_t)b * h + i1] = sq1;
+    }
 }
 
 // var = n/(n-1) * sum(x^2) over the row partials, FP64, fixed order.  grid (B)
@@ -316,58 +375,137 @@ __global__ __launch_bounds__(256) void wiener_var_kernel(const double *rowsq_all
 
 // Variance estimate: min over f in {3,5,7,9} of the zero-padded f x f mean of
 // P (wienerfilter.cu:164-247), then S = var / max(var, varest).
-// grid (ceil(cols/32), ceil(rows/32), B)
-constexpr int kVT = 32, kVB = 4;
+// A workgroup takes a 64 x 32 output tile with its 4-pixel halo in LDS; a
+// thread owns one output column and 8 output rows: it forms the horizontal
+// 3/5/7/9 sums of the 16 LDS rows those need (9 LDS reads a row) and sums
+// them vertically in registers (18 LDS reads an output instead of 80 for
+// ring-by-ring 2-D sums).  Tiles are numbered image by image, row by row;
+// workgroup ids go to the 8 XCDs round-robin, so XCD x is given the tile
+// range [x * per, (x + 1) * per) in order and neighbouring tiles share halo
+// lines in its L2.  grid (8 * per)
+// P holds half of the power spectrum, which is symmetric, P(-k) = P(k):
+// MIRROR_ROWS, rows [0, rows/2] of length cols (full row r > rows/2 is row
+// rows - r reversed: P(r, c) = P(rows - r, (cols - c) % cols)); otherwise
+// all rows with columns [0, cols/2] (P(r, c) = P((rows - r) % rows, cols - c)
+// for c > cols/2).  S is written on the full rows x cols grid.
+constexpr int kVX = 64, kVY = 32, kVB = 4;
+template <bool MIRROR_ROWS>
+__device__ __forceinline__ float power_full(const float *P, int r, int c, int rows, int cols) {
+    if (MIRROR_ROWS) {
+        if (r <= rows / 2) return P[(int64_t)r * cols + c];
+        return P[(int64_t)(rows - r) * cols + (c == 0 ? 0 : cols - c)];
+    }
+    const int ch = cols / 2 + 1;
+    if (c <= cols / 2) return P[(int64_t)r * ch + c];
+    return P[(int64_t)(r == 0 ? 0 : rows - r) * ch + (cols - c)];
+}
+
+template <bool MIRROR_ROWS>
 __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, float *S_all, int rows, int cols,
-                                                           const double *var_all) {
-    __shared__ float T[kVT + 2 * kVB][kVT + 2 * kVB + 1];
+                                                           const double *var_all, int ntx, int nty, int ntiles) {
+    __shared__ float T[kVY + 2 * kVB][kVX + 2 * kVB];
+    const int per = (ntiles + 7) / 8;
+    const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    const int b = t / (ntx * nty), rem = t - b * (ntx * nty);
+    const int r0 = (rem / ntx) * kVY, c0 = (rem % ntx) * kVX;
     const int64_t plane = (int64_t)rows * cols;
-    const float *P = P_all + (int64_t)blockIdx.z * plane;
-    float *S = S_all + (int64_t)blockIdx.z * plane;
-    const int c0 = blockIdx.x * kVT, r0 = blockIdx.y * kVT;
-    for (int idx = threadIdx.x; idx < (kVT + 2 * kVB) * (kVT + 2 * kVB); idx += 256) {
-        const int ly = idx / (kVT + 2 * kVB), lx = idx % (kVT + 2 * kVB);
+    const float *P = P_all + (int64_t)b * plane;
+    float *S = S_all + (int64_t)b * plane;
+    constexpr int LW = kVX + 2 * kVB, LH = kVY + 2 * kVB;
+    for (int idx = threadIdx.x; idx < LW * LH; idx += 256) {
+        const int ly = idx / LW, lx = idx - (idx / LW) * LW;
         const int gy = r0 + ly - kVB, gx = c0 + lx - kVB;
-        T[ly][lx] = (gy >= 0 && gy < rows && gx >= 0 && gx < cols) ? P[(int64_t)gy * cols + gx] : 0.f;
+        T[ly][lx] = (gy >= 0 && gy < rows && gx >= 0 && gx < cols) ? power_full<MIRROR_ROWS>(P, gy, gx, rows, cols)
+                                                                   : 0.f;
     }
     __syncthreads();
-    const float v = (float)var_all[blockIdx.z];
-    for (int idx = threadIdx.x; idx < kVT * kVT; idx += 256) {
-        const int ly = idx / kVT, lx = idx % kVT;
-        const int gy = r0 + ly, gx = c0 + lx;
-        if (gy >= rows || gx >= cols) continue;
-        const int cy = ly + kVB, cx = lx + kVB;
-        float sum = T[cy][cx];
-        float best = 3.40282347e+38f;
+    const int c = threadIdx.x & 63, oy0 = (threadIdx.x >> 6) * 8;
+    // horizontal sums of LDS rows oy0 .. oy0 + 15 around column c + kVB
+    float h3[16], h5[16], h7[16], h9[16];
 #pragma unroll
-        for (int r = 1; r <= kVB; ++r) {
-            const int f = 2 * r + 1;
-            for (int k = -r; k <= r; ++k) {
-                sum += T[cy - r][cx + k];
-                sum += T[cy + r][cx + k];
-            }
-            for (int k = -r + 1; k <= r - 1; ++k) {
-                sum += T[cy + k][cx - r];
-                sum += T[cy + k][cx + r];
-            }
-            const float avg = sum / (float)(f * f);
-            best = avg < best ? avg : best;
-        }
-        S[(int64_t)gy * cols + gx] = v / fmaxf(v, best);
+    for (int i = 0; i < 16; ++i) {
+        const float *row = &T[oy0 + i][c + kVB];
+        h3[i] = row[-1] + row[0] + row[1];
+        h5[i] = h3[i] + row[-2] + row[2];
+        h7[i] = h5[i] + row[-3] + row[3];
+        h9[i] = h7[i] + row[-4] + row[4];
+    }
+    const float v = (float)var_all[b];
+    const int gx = c0 + c;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        // output row oy0 + j sits at LDS row oy0 + j + 4 = local index j + 4
+        float s3 = 0.f, s5 = 0.f, s7 = 0.f, s9 = 0.f;
+#pragma unroll
+        for (int d = -1; d <= 1; ++d) s3 += h3[j + 4 + d];
+#pragma unroll
+        for (int d = -2; d <= 2; ++d) s5 += h5[j + 4 + d];
+#pragma unroll
+        for (int d = -3; d <= 3; ++d) s7 += h7[j + 4 + d];
+#pragma unroll
+        for (int d = -4; d <= 4; ++d) s9 += h9[j + 4 + d];
+        // means by the rounded reciprocals of f^2 (within an ulp of the
+        // quotient; the divisions were most of this kernel's VALU time)
+        float best = s3 * (1.f / 9.f);
+        best = fminf(best, s5 * (1.f / 25.f));
+        best = fminf(best, s7 * (1.f / 49.f));
+        best = fminf(best, s9 * (1.f / 81.f));
+        const int gy = r0 + oy0 + j;
+        if (gy < rows && gx < cols) S[(int64_t)gy * cols + gx] = v / fmaxf(v, best);
     }
 }
 
-// Inverse FFT (length w) of row i of A; out = Re / (w*h) (normalizeToReal).
-// grid (h, B)
+int launch_varest(const float *P, float *S, int rows, int cols, bool mirror_rows, int nb, const double *var,
+                  hipStream_t st) {
+    const int ntx = (cols + kVX - 1) / kVX, nty = (rows + kVY - 1) / kVY;
+    const int64_t ntiles = (int64_t)ntx * nty * nb;
+    PRNU_REQUIRE(ntiles < (int64_t(1) << 30), "wiener: %lld variance tiles", (long long)ntiles);
+    const int per = (int)((ntiles + 7) / 8);
+    if (mirror_rows)
+        hipLaunchKernelGGL(varest_scale_kernel<true>, dim3((unsigned)(8 * per)), dim3(256), 0, st, P, S, rows, cols,
+                           var, ntx, nty, (int)ntiles);
+    else
+        hipLaunchKernelGGL(varest_scale_kernel<false>, dim3((unsigned)(8 * per)), dim3(256), 0, st, P, S, rows, cols,
+                           var, ntx, nty, (int)ntiles);
+    PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+// Rows 2r and 2r+1 from their half spectra in A (h x (w/2+1)): one inverse
+// transform of Z_k = A_k + i B_k (upper half from A_{w-k} = conj A_k; the
+// imaginary parts of bins 0 and w/2 dropped, as a real transform does);
+// out = z / (w*h), row 2r the real, row 2r+1 the imaginary part
+// (normalizeToReal).  grid (ceil(h/2), B)
 template <int MAXN>
-__global__ __launch_bounds__(kFftThreads) void rows_ifft_real_kernel(FftDesc d, const float2 *A, int64_t plane,
-                                                                     float *out_all, int64_t out_stride, float inv_n) {
+__global__ __launch_bounds__(kFftThreads) void rows_c2r_kernel(FftDesc d, const float2 *A, int64_t plane, int h,
+                                                               float *out_all, int64_t out_stride, float inv_n) {
     extern __shared__ float2 buf[];
-    const int i = blockIdx.x;
-    const int w = d.n;
-    const float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * w;
-    float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i * w;
-    fft_row<1, MAXN>(buf, d, [&](int j) { return arow[j]; }, [&](int j, float2 f) { out[j] = f.x * inv_n; });
+    const int i0 = 2 * blockIdx.x, i1 = i0 + 1;
+    const bool two = i1 < h;
+    const int w = d.n, wh = w / 2 + 1;
+    const float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * wh;
+    float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i0 * w;
+    fft_row<1, MAXN>(
+        buf, d,
+        [&](int k) {
+            const bool up = k > w / 2;
+            const int kk = up ? w - k : k;
+            float2 x = a0[kk], y = two ? a0[wh + kk] : make_float2(0.f, 0.f);
+            if (kk == 0 || 2 * kk == w) {
+                x.y = 0.f;
+                y.y = 0.f;
+            }
+            if (up) {
+                x.y = -x.y;
+                y.y = -y.y;
+            }
+            return make_float2(x.x - y.y, x.y + y.x);
+        },
+        [&](int j, float2 z) {
+            out[j] = z.x * inv_n;
+            if (two) out[w + j] = z.y * inv_n;
+        });
 }
 
 // ---------------------------------------------------------------- spectrum
@@ -410,17 +548,21 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_power_kernel(FftDesc d, 
         });
 }
 
-// Scale row r of B by S and run the inverse FFT (length d.n) in place.
-// grid (rows, B); planes `plane` apart.
+// Row r of B (r <= R/2, length d.n = C) times S'(r, c) = (S(r, c) +
+// S((R - r) % R, (C - c) % C)) / 2 from the full R x C scale S, then the
+// inverse FFT in place.  grid (R/2 + 1, B); planes `plane` apart.
 template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d, float2 *B, const float *S,
-                                                                       int64_t plane) {
+                                                                       int64_t plane, int R) {
     extern __shared__ float2 buf[];
     const int r = blockIdx.x;
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * n;
     const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * n;
-    fft_row<1, MAXN>(buf, d, [&](int j) { return cscale(row[j], srow[j]); }, [&](int j, float2 f) { row[j] = f; });
+    const float *mrow = S + (int64_t)blockIdx.y * plane + (int64_t)(r == 0 ? 0 : R - r) * n;
+    fft_row<1, MAXN>(
+        buf, d, [&](int j) { return cscale(row[j], 0.5f * (srow[j] + mrow[j == 0 ? 0 : n - j])); },
+        [&](int j, float2 f) { row[j] = f; });
 }
 
 // grid (rows, B)
@@ -437,13 +579,16 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d
 // workgroup reads `sw` adjacent columns row by row (contiguous pieces of
 // each row, no transpose), transforms them together in LDS (column-major
 // there) and writes them back in place.  MODE 0: plain; MODE 1: also write
-// |F|^2 to P; MODE 2: multiply by S on the way in.  BIG: one column longer
+// |F|^2 to P (laid out as A); MODE 2: multiply element (i, j) by
+// (S(i, j) + S((h - i) % h, (scols - j) % scols)) / 2 on the way in, S the
+// full h x scols scale.  BIG: one column longer
 // than kFftMultiMax through fft_lds.  grid (ceil(ncols/sw), images); planes
 // `plane` elements apart, rows `ld` elements apart.
 template <int DIR, int MODE, bool BIG>
 __global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void cols_fft_strip_kernel(FftDesc d, float2 *A, int64_t plane,
                                                                                    int ld, int ncols, int sw,
-                                                                                   const float *S, float *P) {
+                                                                                   const float *S, float *P,
+                                                                                   int scols) {
     extern __shared__ float2 buf[];
     const int h = d.n;
     const int j0 = blockIdx.x * sw;
@@ -453,7 +598,13 @@ __global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void cols_fft_strip_kerne
         const int i = idx / ncol, c = idx - (idx / ncol) * ncol;
         const int64_t at = (int64_t)i * ld + j0 + c;
         float2 v = a[at];
-        if (MODE == 2) v = cscale(v, S[(int64_t)blockIdx.y * plane + at]);
+        if (MODE == 2) {
+            const float *Sb = S + (int64_t)blockIdx.y * plane;
+            const int j = j0 + c;
+            const float s1 = Sb[(int64_t)i * scols + j];
+            const float s2 = Sb[(int64_t)(i == 0 ? 0 : h - i) * scols + (j == 0 ? 0 : scols - j)];
+            v = cscale(v, 0.5f * (s1 + s2));
+        }
         buf[c * h + i] = v;
     }
     __syncthreads();
@@ -547,9 +698,13 @@ size_t fft_smem(const FftDesc &d) { return sizeof(float2) * d.n; }
 template <int FMT>
 int launch_fastnoise_fmt(const void *in, int64_t in_stride, float *out, int64_t out_stride, int h, int w, int nb,
                          double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, hipStream_t st) {
-    dim3 grid((w + kFnTX - 1) / kFnTX, (h + kFnTY - 1) / kFnTY, nb);
-    hipLaunchKernelGGL(fastnoise_kernel<FMT>, grid, dim3(256), 0, st, static_cast<const uint8_t *>(in), in_stride,
-                       out, out_stride, h, w, colpart, cp_stride, rowpart, rp_stride);
+    const int ntx = (w + kFnTX - 1) / kFnTX, nty = (h + kFnTY - 1) / kFnTY;
+    const int64_t ntiles = (int64_t)ntx * nty * nb;
+    PRNU_REQUIRE(ntiles < (int64_t(1) << 30), "fastnoise: %lld tiles", (long long)ntiles);
+    const int per = (int)((ntiles + 7) / 8);
+    hipLaunchKernelGGL(fastnoise_kernel<FMT>, dim3((unsigned)(8 * per)), dim3(256), 0, st,
+                       static_cast<const uint8_t *>(in), in_stride, out, out_stride, h, w, colpart, cp_stride, rowpart,
+                       rp_stride, ntx, nty, (int)ntiles);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -582,7 +737,7 @@ int launch_zm_means(Context *c, ExtractWs &e, int nb, hipStream_t st) {
 
 template <int DIR, int MODE>
 int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int ncols, int nb, const float *S, float *P,
-                    hipStream_t st) {
+                    hipStream_t st, int scols = 0) {
     const int h = dh.n;
     const bool big = h > kFftMultiMax;
     const int sw = big ? 1 : col_strip(h);
@@ -590,51 +745,49 @@ int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int nco
     const size_t smem = sizeof(float2) * (size_t)sw * h;
     if (big)
         hipLaunchKernelGGL((cols_fft_strip_kernel<DIR, MODE, true>), grid, dim3(kFftThreads), smem, st, dh, A, plane,
-                           ld, ncols, sw, S, P);
+                           ld, ncols, sw, S, P, scols);
     else
         hipLaunchKernelGGL((cols_fft_strip_kernel<DIR, MODE, false>), grid, dim3(kFftThreads), smem, st, dh, A, plane,
-                           ld, ncols, sw, S, P);
+                           ld, ncols, sw, S, P, scols);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
 
-// Wiener filter of nb images (img, img_stride floats apart) into out: row
-// FFTs, column FFTs (+ |F|^2), the variance stencil, scaling + inverse column
-// FFTs, inverse row FFTs; columns by strips or through transposes
-// (strips_for).
+// Wiener filter of nb images (img, img_stride floats apart) into out, on
+// half spectra: row FFTs two rows at a time, column FFTs (+ |F|^2) over the
+// w/2+1 columns, the variance stencil on the full grid (mirrored reads of
+// the half power spectrum), the symmetrized scale + inverse column FFTs,
+// real inverse row FFTs two rows at a time; columns by strips or through
+// transposes (strips_for).
 int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float *out,
                   int64_t out_stride, bool zm, hipStream_t st) {
-    const int h = c->h, w = c->w;
+    const int h = c->h, w = c->w, wh = w / 2 + 1;
     const int64_t n = c->pixels();
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(h, nb), st, dw, img, img_stride, h, e.A, e.rowsq,
+    const int pairs = (h + 1) / 2;
+    PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(pairs, nb), st, dw, img, img_stride, h, e.A, n, e.rowsq,
                     zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
     PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
+    PRNU_CHECK_LAUNCH();
     if (strips_for(h)) {
-        PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, w, w, nb, nullptr, e.P, st)));
-        hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
-        PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(varest_scale_kernel, dim3((w + kVT - 1) / kVT, (h + kVT - 1) / kVT, nb), dim3(256), 0,
-                           st, e.P, e.S, h, w, e.var);
-        PRNU_CHECK_LAUNCH();
-        PRNU_TRY((launch_cols_fft<1, 2>(dh, e.A, n, w, w, nb, e.S, nullptr, st)));
+        PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, wh, wh, nb, nullptr, e.P, st)));
+        PRNU_TRY(launch_varest(e.P, e.S, h, w, false, nb, e.var, st));
+        PRNU_TRY((launch_cols_fft<1, 2>(dh, e.A, n, wh, wh, nb, e.S, nullptr, st, w)));
     } else {
-        // transposed layout: B = A^T (w x h), column FFTs as rows of B; the
+        // transposed layout: B = A^T (wh x h), column FFTs as rows of B; the
         // zero-padded box-min stencil is transpose invariant
-        PRNU_TRY(launch_transpose_c(e.A, e.B, h, w, nb, n, st));
-        PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(w, nb), st, dh, e.B, e.P, n);
+        PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
+        PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(wh, nb), st, dh, e.B, e.P, n);
         PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
+        PRNU_TRY(launch_varest(e.P, e.S, w, h, true, nb, e.var, st));
+        PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(wh, nb), st, dh, e.B, e.S, n, w);
         PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(varest_scale_kernel, dim3((h + kVT - 1) / kVT, (w + kVT - 1) / kVT, nb), dim3(256), 0,
-                           st, e.P, e.S, w, h, e.var);
-        PRNU_CHECK_LAUNCH();
-        PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(w, nb), st, dh, e.B, e.S, n);
-        PRNU_CHECK_LAUNCH();
-        PRNU_TRY(launch_transpose_c(e.B, e.A, w, h, nb, n, st));
+        PRNU_TRY(launch_transpose_c(e.B, e.A, wh, h, nb, n, st));
     }
-    PRNU_LAUNCH_ROW(rows_ifft_real_kernel, w, dim3(h, nb), st, dw, e.A, n, out, out_stride, 1.0f / (float)(w * h));
+    PRNU_LAUNCH_ROW(rows_c2r_kernel, w, dim3(pairs, nb), st, dw, e.A, n, h, out, out_stride,
+                    1.0f / (float)((double)w * h));
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
