@@ -509,23 +509,28 @@ __global__ __launch_bounds__(kFftThreads) void rows_c2r_kernel(FftDesc d, const 
 }
 
 // ---------------------------------------------------------------- spectrum
-// Forward FFT (length w) of real row i; the first w/2+1 bins go to A (h x wh).
-// grid (h, B)
+// Forward FFTs (length w) of real rows 2r and 2r+1 as one complex transform
+// (z = a + i b, split as in wiener_rows_fwd_kernel); the w/2+1 bins of each
+// row go to A (h x wh, rows wh apart).  grid (ceil(h/2), B)
 template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img_all,
                                                                     int64_t img_stride, float2 *A, int64_t plane,
                                                                     int h) {
     extern __shared__ float2 buf[];
-    const int i = blockIdx.x;
+    const int i0 = 2 * blockIdx.x, i1 = i0 + 1;
+    const bool two = i1 < h;
     const int w = d.n, wh = w / 2 + 1;
-    const float *irow = img_all + (int64_t)blockIdx.y * img_stride + (int64_t)i * w;
-    float2 *arow = A + (int64_t)blockIdx.y * plane + (int64_t)i * wh;
+    const float *r0 = img_all + (int64_t)blockIdx.y * img_stride + (int64_t)i0 * w;
+    const float *r1 = two ? r0 + w : r0;
     fft_row<-1, MAXN>(
-        buf, d, [&](int j) { return make_float2(irow[j], 0.f); },
-        [&](int j, float2 f) {
-            if (j < wh) arow[j] = f;
-        });
-    (void)h;
+        buf, d, [&](int j) { return make_float2(r0[j], two ? r1[j] : 0.f); },
+        [&](int j, float2 f) { buf[j] = f; });  // the last pass has read buf before it writes
+    float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * wh;
+    for (int k = threadIdx.x; k < wh; k += kFftThreads) {
+        const float2 zk = buf[k], zm_ = buf[k == 0 ? 0 : w - k];
+        a0[k] = make_float2(0.5f * (zk.x + zm_.x), 0.5f * (zk.y - zm_.y));
+        if (two) a0[wh + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
+    }
 }
 
 // ------------------------------------------- transposed-layout row kernels
@@ -566,12 +571,12 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
 }
 
 // grid (rows, B)
-template <int DIR, int MAXN>
-__global__ __launch_bounds__(kFftThreads) void rows_fft_inplace_kernel(FftDesc d, float2 *B, int64_t plane) {
+template <int MAXN>
+__global__ __launch_bounds__(kFftThreads) void rows_fft_fwd_inplace_kernel(FftDesc d, float2 *B, int64_t plane) {
     extern __shared__ float2 buf[];
     const int n = d.n;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * n;
-    fft_row<DIR, MAXN>(buf, d, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
+    fft_row<-1, MAXN>(buf, d, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
 }
 
 // ------------------------------------------------------ column transforms
@@ -685,12 +690,14 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
 
 size_t fft_smem(const FftDesc &d) { return sizeof(float2) * d.n; }
 
-// Row-FFT kernel K<MAXN> over rows of length n: the small-register form up
-// to kFftSmallN values.
+// Row-FFT kernel K<MAXN> over rows of length n: the register budget of the
+// smallest of kFftSmallN, kFftMidN, kFftMaxN that holds the row.
 #define PRNU_LAUNCH_ROW(K, n, grid, st, ...)                                                                    \
     do {                                                                                                        \
         if ((n) <= kFftSmallN)                                                                                  \
             hipLaunchKernelGGL((K<kFftSmallN>), grid, dim3(kFftThreads), fft_row_smem(n), st, __VA_ARGS__);     \
+        else if ((n) <= kFftMidN)                                                                               \
+            hipLaunchKernelGGL((K<kFftMidN>), grid, dim3(kFftThreads), fft_row_smem(n), st, __VA_ARGS__);       \
         else                                                                                                    \
             hipLaunchKernelGGL((K<kFftMaxN>), grid, dim3(kFftThreads), fft_row_smem(n), st, __VA_ARGS__);       \
     } while (0)
@@ -801,21 +808,16 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
     if (strips_for(h)) {
-        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3(h, nb), st, dw, img, img_stride, out, out_stride, h);
+        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, out, out_stride, h);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
         return PRNU_OK;
     }
     const int64_t n = c->pixels();
-    PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3(h, nb), st, dw, img, img_stride, e.A, n, h);
+    PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h);
     PRNU_CHECK_LAUNCH();
     PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
-    if (h <= kFftSmallN)
-        hipLaunchKernelGGL((rows_fft_inplace_kernel<-1, kFftSmallN>), dim3(wh, nb), dim3(kFftThreads),
-                           fft_row_smem(h), st, dh, e.B, n);
-    else
-        hipLaunchKernelGGL((rows_fft_inplace_kernel<-1, kFftMaxN>), dim3(wh, nb), dim3(kFftThreads),
-                           fft_row_smem(h), st, dh, e.B, n);
+    PRNU_LAUNCH_ROW(rows_fft_fwd_inplace_kernel, h, dim3(wh, nb), st, dh, e.B, n);
     PRNU_CHECK_LAUNCH();
     if (nb == 1 || out_stride == n) {
         PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st));

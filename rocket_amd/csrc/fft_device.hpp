@@ -215,9 +215,10 @@ __device__ inline void fft_lds(float2 *buf, const FftDesc &d) {
 // addresses cost registers, hence occupancy.)
 inline size_t fft_row_smem(int n) { return sizeof(float2) * (size_t)n; }
 
-// Rows up to kFftSmallN values run with per-thread butterfly arrays sized
-// for that length (fewer registers, more waves per SIMD).
+// Rows up to kFftSmallN (kFftMidN) values run with per-thread butterfly
+// arrays sized for that length (fewer registers, more waves per SIMD).
 constexpr int kFftSmallN = 2048;
+constexpr int kFftMidN = 4096;
 
 template <int R, int DIR, bool FIRST, bool LAST, int MAXN, class Get, class Put>
 __device__ __forceinline__ void fft_pass_io(float2 *buf, int n, int Ns, const float2 *tw, Get &get, Put &put) {

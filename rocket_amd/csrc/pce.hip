@@ -125,38 +125,30 @@ __device__ __forceinline__ float2 herm_at(const float2 *X, const float2 *Y, int 
     return make_float2(0.5f * (c.x + m.x), 0.5f * (c.y - m.y));
 }
 
-// Rows [i0, i0 + rw) of Z = H_a + i H_b for couple blockIdx.y (pairs 2q,
-// 2q+1 of the batch; the second may be absent), inverse row FFTs in LDS.
-// grid (ceil(h/rw), couples), LDS rw*w complex.
-template <int VARIANT, bool BIG>
-__global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_rows2_kernel(FftDesc d, int h, int rw, int nb,
-                                                                             PceBatch bt, float2 *__restrict__ Aall,
-                                                                             int64_t pstride) {
+// Row i of Z = H_a + i H_b for couple blockIdx.y (pairs 2q, 2q+1 of the
+// batch; the second may be absent), inverse row FFT: the first pass reads
+// the Hermitian parts straight from the half spectra, the last pass writes
+// the row of A.  grid (h, couples), LDS w complex.
+template <int VARIANT, int MAXN>
+__global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h, int nb, PceBatch bt,
+                                                                float2 *__restrict__ Aall, int64_t pstride) {
     extern __shared__ float2 buf[];
-    const int q = blockIdx.y, i0 = blockIdx.x * rw;
+    const int q = blockIdx.y, i = blockIdx.x;
     const int w = d.n;
-    const int nrow = min(rw, h - i0);
     const float2 *Xa, *Ya, *Xb = nullptr, *Yb = nullptr;
     pce_operands(bt, 2 * q, Xa, Ya);
     const bool two = 2 * q + 1 < nb;
     if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
-    for (int idx = threadIdx.x; idx < nrow * w; idx += kFftThreads) {
-        const int r = idx / w, j = idx - (idx / w) * w;
-        const float2 ha = herm_at<VARIANT>(Xa, Ya, h, w, i0 + r, j);
-        float2 z = ha;
-        if (two) {
-            const float2 hb = herm_at<VARIANT>(Xb, Yb, h, w, i0 + r, j);
-            z = make_float2(ha.x - hb.y, ha.y + hb.x);
-        }
-        buf[idx] = z;
-    }
-    __syncthreads();
-    if constexpr (BIG)
-        fft_lds<1>(buf, d);  // one row longer than kFftMultiMax (rw == 1)
-    else
-        fft_lds_multi<1>(buf, d, nrow);
-    float2 *A = Aall + (int64_t)q * pstride + (int64_t)i0 * w;
-    for (int idx = threadIdx.x; idx < nrow * w; idx += kFftThreads) A[idx] = buf[idx];
+    float2 *A = Aall + (int64_t)q * pstride + (int64_t)i * w;
+    fft_row<1, MAXN>(
+        buf, d,
+        [&](int j) {
+            const float2 ha = herm_at<VARIANT>(Xa, Ya, h, w, i, j);
+            if (!two) return ha;
+            const float2 hb = herm_at<VARIANT>(Xb, Yb, h, w, i, j);
+            return make_float2(ha.x - hb.y, ha.y + hb.x);
+        },
+        [&](int j, float2 f) { A[j] = f; });
 }
 
 constexpr int kMaxStrip = 16;
@@ -299,20 +291,16 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
     }
 }
 
-// Columns per strip of the column pass and rows per workgroup of the row
-// pass: powers of two with the transforms of one workgroup (<= kFftMultiMax
-// values, 32 KiB) transformed together; a transform longer than that runs
-// alone through fft_lds (the BIG kernels, up to kFftMaxN).
+// Columns per strip of the column pass: a power of two with the transforms
+// of one workgroup (<= kFftMultiMax values, 32 KiB) transformed together; a
+// column longer than that runs alone through fft_lds (the BIG kernel, up to
+// kFftMaxN).  The row pass runs one row per workgroup through fft_row (rows
+// grouped per workgroup measured slower: 184 VGPRs, 2 waves per SIMD).
 int pce_strip(int h) {
     int sw = kMaxStrip;
     while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
     return sw;
 }
-// The row pass reads the half spectra (two rows each for the reference
-// variant's Hermitian part) and is bound by those reads; one row per
-// workgroup at ~107 VGPRs keeps 4 waves per SIMD in flight, which measured
-// faster than grouping rows (184 VGPRs, 2 waves per SIMD).
-int pce_rows_per_block(int) { return 1; }
 
 int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, hipStream_t st) {
     const int h = c->h, w = c->w;
@@ -329,18 +317,19 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     PeakPart *part = reinterpret_cast<PeakPart *>(base + plane * couples + rbytes * nb);
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    const int rw = pce_rows_per_block(w);
-    const bool big_w = true, big_h = h > kFftMultiMax;  // rows: one per workgroup through fft_lds
-    const dim3 rgrid((h + rw - 1) / rw, couples);
-    const size_t rsm = sizeof(float2) * rw * w;
-#define PRNU_PCE_ROWS(V, B) \
-    hipLaunchKernelGGL((pce_rows2_kernel<V, B>), rgrid, dim3(kFftThreads), rsm, st, dw, h, rw, nb, bt, A, pstride)
+    const bool big_h = h > kFftMultiMax;
+    const dim3 rgrid(h, couples);
+    const size_t rsm = fft_row_smem(w);
+#define PRNU_PCE_ROWS(V, M) \
+    hipLaunchKernelGGL((pce_rows2_kernel<V, M>), rgrid, dim3(kFftThreads), rsm, st, dw, h, nb, bt, A, pstride)
     if (variant == PRNU_PCE_REFERENCE) {
-        if (big_w) PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, true);
-        else PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, false);
+        if (w <= kFftSmallN) PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, kFftSmallN);
+        else if (w <= kFftMidN) PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, kFftMidN);
+        else PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, kFftMaxN);
     } else {
-        if (big_w) PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, true);
-        else PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, false);
+        if (w <= kFftSmallN) PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, kFftSmallN);
+        else if (w <= kFftMidN) PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, kFftMidN);
+        else PRNU_PCE_ROWS(PRNU_PCE_TEXTBOOK, kFftMaxN);
     }
 #undef PRNU_PCE_ROWS
     PRNU_CHECK_LAUNCH();
