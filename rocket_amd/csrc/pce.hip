@@ -7,11 +7,12 @@
 // couple of pairs, inputs are R2C half spectra h x (w/2+1):
 //   1. pce_rows2_kernel  builds row i of Z and runs the inverse row FFT
 //                        (length w) in LDS;
-//   2. transpose         A (h x w) -> AT (w x h);
-//   3. pce_cols_t_kernel inverse FFT of each column (length h, a row of AT);
-//                        writes Re c of both pairs and per column the max
-//                        |Re c| with its index and sum (Re c)^2 in FP64;
-//   4. pce_final_kernel  per pair: argmax over columns, energy = total -
+//   2. pce_cols2_kernel  loads a strip of columns (coalesced rows of the
+//                        strip, no transpose), inverse FFT of each column
+//                        (length h) in LDS; writes Re c of both pairs and per
+//                        column the max |Re c| with its index and sum (Re c)^2
+//                        in FP64;
+//   3. pce_final_kernel  per pair: argmax over columns, energy = total -
 //                        window (FP64), peak value, PCE.
 // A pair's result depends on its couple partner only through FP32 rounding
 // (a fixed pair list always forms the same couples).
@@ -150,60 +151,79 @@ __global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h
         [&](int j, float2 f) { A[j] = f; });
 }
 
-// Column j of couple blockIdx.y from the transposed row-pass output AT
-// (w x h, column j contiguous): inverse FFT (length h) through fft_row, whose
-// last pass writes Re / Im straight to the two pairs' correlation planes
-// (column-major, R[j*h + i]) and folds them into per-thread energy (FP64)
-// and |value| argmax; fixed-order block reductions give the column's
-// PeakPart.  grid (w, couples), LDS h complex.
-template <int MAXN>
-__global__ __launch_bounds__(kFftThreads) void pce_cols_t_kernel(FftDesc d, int w, int nb,
-                                                                 const float2 *__restrict__ ATall, int64_t pstride,
-                                                                 float *__restrict__ Rall, int64_t rstride,
-                                                                 PeakPart *__restrict__ partall) {
+constexpr int kMaxStrip = 16;
+
+// Columns [j0, j0 + sw) of couple blockIdx.y: rows of the strip are read
+// contiguously into column-major LDS, the columns are inverse-transformed
+// together (length h), and Re / Im go to the two pairs' correlation planes
+// (column-major, R[j*h + i]) with per-column energy and |value| argmax.
+// grid (ceil(w/sw), couples), LDS sw*h complex, sw <= kMaxStrip.
+template <bool BIG>
+__global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_cols2_kernel(FftDesc d, int w, int sw, int nb,
+                                                                             const float2 *__restrict__ Aall,
+                                                                             int64_t pstride, float *__restrict__ Rall,
+                                                                             int64_t rstride,
+                                                                             PeakPart *__restrict__ partall) {
     extern __shared__ float2 buf[];
-    __shared__ double shd[4];
-    __shared__ float shv[4];
-    __shared__ int shi[4];
-    const int h = d.n, q = blockIdx.y, j = blockIdx.x;
-    const bool two = 2 * q + 1 < nb;
-    const float2 *col = ATall + (int64_t)q * pstride + (int64_t)j * h;
-    float *Ra = Rall + (int64_t)(2 * q) * rstride + (int64_t)j * h;
-    float *Rb = two ? Ra + rstride : Ra;
-    double ea = 0.0, eb = 0.0;
-    float ba = -1.0f, bb = -1.0f;
-    int ia = 0x7fffffff, ib = 0x7fffffff;
-    fft_row<1, MAXN>(
-        buf, d, [&](int i) { return col[i]; },
-        [&](int i, float2 z) {
-            Ra[i] = z.x;
-            ea += (double)z.x * (double)z.x;
-            argmax_merge(ba, ia, fabsf(z.x), i * w + j);
-            if (two) {
-                Rb[i] = z.y;
-                eb += (double)z.y * (double)z.y;
-                argmax_merge(bb, ib, fabsf(z.y), i * w + j);
-            }
-        });
-    ea = block_sum_256(ea, shd);
-    block_argmax_256(ba, ia, shv, shi);
-    if (threadIdx.x == 0) {
-        PeakPart p;
-        p.energy = ea;
-        p.absmax = ba;
-        p.index = ia;
-        partall[(int64_t)(2 * q) * w + j] = p;
+    __shared__ double sh_e[2 * kMaxStrip][4];
+    __shared__ float sh_v[2 * kMaxStrip][4];
+    __shared__ int sh_i[2 * kMaxStrip][4];
+    const int h = d.n, q = blockIdx.y;
+    const int j0 = blockIdx.x * sw;
+    const int ncol = min(sw, w - j0);
+    const float2 *A = Aall + (int64_t)q * pstride;
+    for (int idx = threadIdx.x; idx < h * ncol; idx += kFftThreads) {
+        const int i = idx / ncol, c = idx - (idx / ncol) * ncol;
+        buf[c * h + i] = A[(int64_t)i * w + j0 + c];
     }
-    if (!two) return;
-    __syncthreads();  // shd/shv/shi reused
-    eb = block_sum_256(eb, shd);
-    block_argmax_256(bb, ib, shv, shi);
-    if (threadIdx.x == 0) {
+    __syncthreads();
+    if constexpr (BIG)
+        fft_lds<1>(buf, d);  // one column longer than kFftMultiMax (sw == 1)
+    else
+        fft_lds_multi<1>(buf, d, ncol);
+    const int halves = 2 * q + 1 < nb ? 2 : 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < ncol; ++c) {
+        const int j = j0 + c;
+        for (int hf = 0; hf < halves; ++hf) {
+            float *R = Rall + (int64_t)(2 * q + hf) * rstride;
+            double e = 0.0;
+            float best = -1.0f;
+            int bi = 0x7fffffff;
+            for (int i = threadIdx.x; i < h; i += kFftThreads) {
+                const float2 z = buf[c * h + i];
+                const float v = hf ? z.y : z.x;
+                R[(int64_t)j * h + i] = v;
+                e += (double)v * (double)v;
+                argmax_merge(best, bi, fabsf(v), i * w + j);
+            }
+            e = wave_sum_d(e);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const float v2 = __shfl_xor(best, off, 64);
+                const int i2 = __shfl_xor(bi, off, 64);
+                argmax_merge(best, bi, v2, i2);
+            }
+            if (lane == 0) {
+                sh_e[2 * c + hf][wave] = e;
+                sh_v[2 * c + hf][wave] = best;
+                sh_i[2 * c + hf][wave] = bi;
+            }
+        }
+    }
+    __syncthreads();
+    // one thread per (column, pair): fixed-order merge of the 4 waves
+    const int t = threadIdx.x;
+    if (t < 2 * ncol && (t & 1) < halves) {
+        const int c = t >> 1, hf = t & 1;
         PeakPart p;
-        p.energy = eb;
-        p.absmax = bb;
-        p.index = ib;
-        partall[(int64_t)(2 * q + 1) * w + j] = p;
+        p.energy = (sh_e[t][0] + sh_e[t][1]) + (sh_e[t][2] + sh_e[t][3]);
+        float best = sh_v[t][0];
+        int bi = sh_i[t][0];
+        for (int k = 1; k < 4; ++k) argmax_merge(best, bi, sh_v[t][k], sh_i[t][k]);
+        p.absmax = best;
+        p.index = bi;
+        partall[(int64_t)(2 * q + hf) * w + j0 + c] = p;
     }
 }
 
@@ -271,10 +291,17 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
     }
 }
 
-// The row pass runs one row per workgroup through fft_row (rows grouped per
-// workgroup measured slower: 184 VGPRs, 2 waves per SIMD); the column pass
-// transposes first so that columns are contiguous rows for fft_row (strips
-// of 4 columns read in 32-byte row pieces ran at about 1 TB/s).
+// Columns per strip of the column pass: a power of two with the transforms
+// of one workgroup (<= kFftMultiMax values, 32 KiB) transformed together; a
+// column longer than that runs alone through fft_lds (the BIG kernel, up to
+// kFftMaxN).  The row pass runs one row per workgroup through fft_row (rows
+// grouped per workgroup measured slower: 184 VGPRs, 2 waves per SIMD).
+int pce_strip(int h) {
+    int sw = kMaxStrip;
+    while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
+    return sw;
+}
+
 int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, hipStream_t st) {
     const int h = c->h, w = c->w;
     const int64_t n = c->pixels();
@@ -283,14 +310,14 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     const size_t rbytes = ((sizeof(float) * n) + 255) & ~size_t(255);
     const size_t pbytes = ((sizeof(PeakPart) * w) + 255) & ~size_t(255);
     const int64_t pstride = (int64_t)(plane / sizeof(float2)), rstride = (int64_t)(rbytes / sizeof(float));
-    PRNU_TRY(c->ws_pce.ensure(2 * plane * couples + (rbytes + pbytes) * nb));
+    PRNU_TRY(c->ws_pce.ensure(plane * couples + (rbytes + pbytes) * nb));
     char *base = static_cast<char *>(c->ws_pce.ptr);
     float2 *A = reinterpret_cast<float2 *>(base);
-    float2 *AT = reinterpret_cast<float2 *>(base + plane * couples);
-    float *R = reinterpret_cast<float *>(base + 2 * plane * couples);
-    PeakPart *part = reinterpret_cast<PeakPart *>(base + 2 * plane * couples + rbytes * nb);
+    float *R = reinterpret_cast<float *>(base + plane * couples);
+    PeakPart *part = reinterpret_cast<PeakPart *>(base + plane * couples + rbytes * nb);
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
+    const bool big_h = h > kFftMultiMax;
     const dim3 rgrid(h, couples);
     const size_t rsm = fft_row_smem(w);
 #define PRNU_PCE_ROWS(V, M) \
@@ -306,19 +333,15 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     }
 #undef PRNU_PCE_ROWS
     PRNU_CHECK_LAUNCH();
-    // columns as contiguous rows: A (h x w) -> AT (w x h), then fft_row
-    PRNU_TRY(launch_transpose_c(A, AT, h, w, couples, pstride, st));
-    const dim3 cgrid(w, couples);
-    const size_t csm = fft_row_smem(h);
-    if (h <= kFftSmallN)
-        hipLaunchKernelGGL((pce_cols_t_kernel<kFftSmallN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, AT,
-                           pstride, R, rstride, part);
-    else if (h <= kFftMidN)
-        hipLaunchKernelGGL((pce_cols_t_kernel<kFftMidN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, AT, pstride,
-                           R, rstride, part);
+    const int sw = pce_strip(h);
+    const dim3 cgrid((w + sw - 1) / sw, couples);
+    const size_t csm = sizeof(float2) * sw * h;  // <= 64 KiB (one column of up to kFftMaxN)
+    if (big_h)
+        hipLaunchKernelGGL((pce_cols2_kernel<true>), cgrid, dim3(kFftThreads), csm, st, dh, w, sw, nb, A, pstride, R,
+                           rstride, part);
     else
-        hipLaunchKernelGGL((pce_cols_t_kernel<kFftMaxN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, AT, pstride,
-                           R, rstride, part);
+        hipLaunchKernelGGL((pce_cols2_kernel<false>), cgrid, dim3(kFftThreads), csm, st, dh, w, sw, nb, A, pstride, R,
+                           rstride, part);
     PRNU_CHECK_LAUNCH();
     if (variant == PRNU_PCE_REFERENCE)
         hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_REFERENCE>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,
@@ -356,7 +379,7 @@ int run_pce_pairs(Context *c, int variant, const void *const *d_tab, const int *
     PRNU_TRY(ensure_fft_plans(c));
     if (batch <= 0) {
         // about 1 GiB of planes
-        const int64_t per = 12 * c->pixels();  // half of A and of AT (complex) + a real plane per pair
+        const int64_t per = 12 * c->pixels();  // half a complex plane + a real plane per pair
         batch = (int)std::max<int64_t>(2, std::min<int64_t>(512, (int64_t(1) << 30) / per)) & ~1;
     }
     time_begin(c, st);
