@@ -248,7 +248,9 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
     }
     const double total = block_sum_256(e, shd);
     block_argmax_256(best, bi, shv, shi);
-    if (threadIdx.x == 0) s_peak = bi;
+    // no argmax at all (every value NaN): index 0 keeps the window and peak
+    // reads inside the plane; the energy, hence the PCE, is NaN anyway
+    if (threadIdx.x == 0) s_peak = (bi >= 0 && (int64_t)bi < (int64_t)h * w) ? bi : 0;
     __syncthreads();
     const int peak = s_peak;
     const int py = peak / w, px = peak - py * w;
