@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
-    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving"],
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks"],
                     help="run one secondary leg alone and print its result (for profiling)")
     ap.add_argument("--extract-shape", default=None, metavar="HxW",
                     help="extraction leg: one frame shape only (default: 1536x2048 and 512x512)")
@@ -70,15 +70,100 @@ def make_patterns(torch):
     return p
 
 
-def fp64_check(torch, p, ncc, sample=48):
-    """max |dNCC| of `ncc` on a random sample of patterns against FP64 dot products on the host."""
-    idx = np.sort(np.random.default_rng(7).choice(COUNT, sample, replace=False))
-    host = p[torch.as_tensor(idx, device="cuda")].cpu().numpy().astype(np.float64)
-    g = host @ host.T
-    s = host.sum(axis=1)
-    d = np.sqrt(np.einsum("ij,ij->i", host, host) - s * s)
-    want = g / np.outer(d, d)
+FP64_SAMPLE = 448   # 100,128 pairs (plus the diagonal) of the 1024
+_FP64_REF = {}
+
+
+def fp64_reference(torch, p, sample=FP64_SAMPLE):
+    """NCC of a seeded sample of the patterns in FP64 (reference semantics:
+    G / (sqrt(sum x^2 - (sum x)^2) sqrt(sum y^2 - (sum y)^2))): FP64 products
+    of the FP32 values, summed by rocBLAS DGEMM over K chunks on the GPU (an
+    FP64 GEMM over 3.1M values for 448 patterns would take minutes on the
+    host).  Returns (sample indices, FP64 matrix); cached per pattern set."""
+    key = (p.data_ptr(), sample)
+    if key not in _FP64_REF:
+        idx = np.sort(np.random.default_rng(7).choice(p.shape[0], sample, replace=False))
+        sel = torch.as_tensor(idx, device="cuda")
+        g = torch.zeros((sample, sample), dtype=torch.float64, device="cuda")
+        s = torch.zeros(sample, dtype=torch.float64, device="cuda")
+        for k0 in range(0, p.shape[1], 1 << 18):
+            x = p[sel, k0:k0 + (1 << 18)].double()
+            g += x @ x.t().contiguous()
+            s += x.sum(dim=1)
+        d = torch.sqrt(torch.diagonal(g) - s * s)
+        _FP64_REF[key] = (idx, (g / torch.outer(d, d)).cpu().numpy())
+        del g, x
+    return _FP64_REF[key]
+
+
+def fp64_check(torch, p, ncc, sample=FP64_SAMPLE):
+    """max |dNCC| of `ncc` over the FP64 reference sample (all pairs and the
+    diagonal of `sample` seeded patterns)."""
+    idx, want = fp64_reference(torch, p, sample)
     return float(np.abs(ncc[np.ix_(idx, idx)] - want).max())
+
+
+def c0_bench(torch):
+    """64 synthetic 8-bit frames of 512x512 from 4 cameras, extracted on the
+    GPU; max |dNCC| over all 2,016 pairs (and the diagonal) of each store
+    format against the FP64 NCC of the same patterns on the host."""
+    from rocket_amd import _lib, csi
+    h = w = 512
+    n = h * w
+    g = torch.Generator(device="cuda").manual_seed(20261202)
+    cams = torch.randn((4, n), generator=g, device="cuda")
+    imgs = synth_images(torch, cams, [i % 4 for i in range(64)], seed=64, h=h, w=w)
+    filt = csi.PRNUFilter(h, w)
+    pats = torch.empty((64, n), device="cuda")
+    filt.apply_gpu_batch(imgs.view(64, n), pats, _lib.INPUT_GRAY8)
+    host = pats.cpu().numpy().astype(np.float64)
+    gm = host @ host.T
+    sm = host.sum(axis=1)
+    d = np.sqrt(np.einsum("ij,ij->i", host, host) - sm * sm)
+    want = gm / np.outer(d, d)
+    gram = csi.GramNCC(h, w)
+    res = {"workload": "c0: 64 frames of 512x512, 4 cameras, 2016 pairs", "pairs": 2016}
+    for fmt in ("split_bf16", "fp32", "fp16", "fp8"):
+        st = csi.PatternStore(n, 64, fmt)
+        st.put(pats)
+        got = st.gram(gram, 64).cpu().numpy()
+        res[f"{fmt}_max_abs_dncc_all_pairs"] = float(np.abs(got - want).max())
+        st.close()
+    same = np.array([[i % 4 == j % 4 for j in range(64)] for i in range(64)]) & ~np.eye(64, dtype=bool)
+    res["same_camera_min_ncc"] = float(want[same].min())
+    res["other_camera_max_ncc"] = float(want[~same & ~np.eye(64, dtype=bool)].max())
+    gram.close()
+    filt.cleanup()
+    return res
+
+
+def measured_peaks(torch):
+    """What the box delivers at the clock it runs: a 4 GiB device copy (read +
+    write bytes / time) and 8192^3 BF16 and FP16 GEMMs through torch
+    (hipBLASLt), best of 5."""
+    def best(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+    x = torch.empty(1 << 30, device="cuda")
+    y = torch.empty_like(x)
+    t = best(lambda: y.copy_(x))
+    res = {"hbm_copy_GBps": 2 * 4 * (1 << 30) / t / 1e9}
+    del x, y
+    m = 8192
+    for dt, name in ((torch.bfloat16, "bf16"), (torch.float16, "fp16")):
+        a = torch.randn((m, m), device="cuda").to(dt)
+        b = torch.randn((m, m), device="cuda").to(dt)
+        t = best(lambda: a @ b)
+        res[f"{name}_gemm_TFLOPs"] = 2.0 * m ** 3 / t / 1e12
+        del a, b
+    return res
 
 
 def load_traffic(kernel: str):
@@ -445,7 +530,8 @@ def main():
     if args.only:
         leg = {"pce": lambda: pce_bench(torch), "extract": lambda: extraction_bench(torch, shape=args.extract_shape, only_mode=args.extract_mode),
                "groups": lambda: groups_bench(torch),
-               "serving": lambda: serving_bench(torch, args.serving_seconds)}[args.only]
+               "serving": lambda: serving_bench(torch, args.serving_seconds),
+               "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
     prec = args.precision
@@ -506,8 +592,10 @@ def main():
     value = pairs / (ms_per_step / 1000.0)
     ncc_main = out.cpu().numpy()
     log(f"{prec} store: {ms_per_step:.3f} ms/step, {value:.4g} pairs/s, tile kernel {kms:.3f} ms")
+    t_ref = time.time()
     dncc = fp64_check(torch, p, ncc_main)
-    log(f"max |dNCC| vs FP64 on a 48-pattern sample: {dncc:.3e}")
+    log(f"max |dNCC| vs FP64 over {FP64_SAMPLE * (FP64_SAMPLE - 1) // 2} sampled pairs: {dncc:.3e} "
+        f"(reference {time.time() - t_ref:.1f} s)")
 
     extra = {"store_insert_ms": put_ms}
     if not args.no_fp32 and prec != "fp32":
@@ -573,6 +661,13 @@ def main():
         log(f"PCE: {pce['reference_pairs_per_s']:.1f} pairs/s (reference), "
             f"{pce['textbook_pairs_per_s']:.1f} pairs/s (textbook)")
 
+    log("c0 accuracy (64 frames of 512x512, all pairs) ...")
+    c0 = c0_bench(torch)
+    log(f"c0: max |dNCC| over all pairs: split-BF16 {c0['split_bf16_max_abs_dncc_all_pairs']:.2e}, "
+        f"FP32 {c0['fp32_max_abs_dncc_all_pairs']:.2e}, FP16 {c0['fp16_max_abs_dncc_all_pairs']:.2e}")
+    peaks = measured_peaks(torch)
+    log(f"measured: copy {peaks['hbm_copy_GBps']:.0f} GB/s, BF16 GEMM {peaks['bf16_gemm_TFLOPs']:.0f} TFLOP/s")
+
     serve = None
     if not args.no_serving:
         gram.close()
@@ -598,6 +693,7 @@ def main():
         "config": {"workload": "ncc1024_2048x1536", "patterns": COUNT, "height": HEIGHT, "width": WIDTH,
                    "pairs": pairs, "precision": prec, "parallelism": "single GPU"},
         "max_abs_dncc_vs_fp64": dncc,
+        "fp64_sample_pairs": FP64_SAMPLE * (FP64_SAMPLE - 1) // 2,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_aux": cpu_aux,
@@ -605,6 +701,8 @@ def main():
         "extraction": extract,
         "pce": pce,
         "groups": groups,
+        "c0": c0,
+        "measured_peaks": peaks,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
