@@ -374,6 +374,36 @@ PRNU_EXPORT int prnu_index_divide(int kind, const prnu_index_node *node, int par
 PRNU_EXPORT int prnu_index_pairs(int kind, int count, int param, int include_diagonal, int32_t *pairs,
                                  int64_t pairs_cap, int64_t *n_pairs);
 
+/* ------------------------------------------------------------ JPEG input
+ * ReadJPEG.readJPEG / readJPEG (CSI/ReadJPEG.java:25-45, readjpeg/src/main/
+ * cpp/native.cpp:59-134): libjpeg's default decode of a 3-component JPEG to
+ * packed RGB8, here for a batch of files on the GPU, bit for bit what
+ * libjpeg returns (ISLOW inverse DCT, libjpeg's choice of fancy or
+ * replicating chroma upsampling, its YCbCr tables).  The reference rejects
+ * anything that does not decode to RGB (native.cpp:99-103); so does this,
+ * and it also rejects progressive, lossless, arithmetic-coded and 12-bit
+ * files and 1x2 chroma subsampling (PRNU_ENOTSUP for that file). */
+typedef struct prnu_jpeg_decoder prnu_jpeg_decoder;
+/* per-file status beyond prnu_status: decoded, but the entropy-coded data
+ * held an invalid code (libjpeg warns and decodes on; readJPEG succeeds) */
+#define PRNU_JPEG_CORRUPT_DATA 16
+PRNU_EXPORT int prnu_jpeg_decoder_create(prnu_jpeg_decoder **out);
+PRNU_EXPORT int prnu_jpeg_decoder_destroy(prnu_jpeg_decoder *dec);
+/* Width and height from the headers of a JPEG file in host memory. */
+PRNU_EXPORT int prnu_jpeg_info(const void *h_file, int64_t size, int32_t *width, int32_t *height);
+/* Decode `count` files (host memory: h_files[k], h_sizes[k] bytes) into
+ * d_rgb (device), file k at byte k * rgb_stride, rows of width * 3 bytes.
+ * h_width / h_height / h_status (optional, `count` each) receive each
+ * file's size and status (0, PRNU_JPEG_CORRUPT_DATA, or the prnu_status it
+ * was rejected with: EINVAL for a malformed file or one that does not fit
+ * rgb_stride, ENOTSUP for an unsupported kind).  Every accepted file is
+ * decoded; the call returns PRNU_OK when all were accepted, else the status
+ * of the first rejected one with its reason in prnu_last_error().  The call
+ * returns after the decode (synchronous on `stream`). */
+PRNU_EXPORT int prnu_jpeg_decode_batch(prnu_jpeg_decoder *dec, const void *const *h_files, const int64_t *h_sizes,
+                                       int count, uint8_t *d_rgb, int64_t rgb_stride, int64_t rgb_capacity,
+                                       int32_t *h_width, int32_t *h_height, int32_t *h_status, prnu_stream stream);
+
 /* --------------------------------------------------------------- timing
  * No reference counterpart (the reference times whole tasks on the host,
  * R/RocketLauncher.java:115-203).  When enabled, the context records HIP

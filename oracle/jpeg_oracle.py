@@ -144,6 +144,7 @@ def parse(data: bytes) -> dict:
         m = b[i]
         i += 1
         if m == 0xD9:
+            info["eoi"] = True
             break
         if m in (0x01,) or 0xD0 <= m <= 0xD7:
             continue
@@ -205,6 +206,10 @@ def parse(data: bytes) -> dict:
         # APPn, COM and others: skipped
     if "comps" not in info:
         raise JpegError("no SOF0/SOF1 frame")
+    if not info.get("eoi"):
+        # readJPEG's source errors when libjpeg asks for bytes past the end
+        # (native.cpp:26-29), which it does for a file without EOI
+        raise JpegError("truncated file (no EOI marker)")
     if len(info["comps"]) != 3:
         raise JpegError("image not in RGB color space (not 3 components)")
     if info["adobe"] not in (None, 1):

@@ -21,6 +21,7 @@ PREC_FP32, PREC_SPLIT_BF16, PREC_FP8, PREC_FP16 = 0, 1, 2, 3
 PCE_REFERENCE, PCE_TEXTBOOK = 0, 1
 INDEX_HILBERT, INDEX_TILING = 0, 1
 PLAN_PER_ITEM, PLAN_PERSISTENT = 0, 1
+JPEG_CORRUPT_DATA = 16
 
 PRECISIONS = {"fp32": PREC_FP32, "split_bf16": PREC_SPLIT_BF16, "fp8": PREC_FP8, "fp16": PREC_FP16}
 
@@ -108,6 +109,11 @@ _SIGNATURES = {
     "prnu_index_divide": (_c_int, [_c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, ctypes.POINTER(_c_int),
                                    _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
     "prnu_index_pairs": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
+    "prnu_jpeg_decoder_create": (_c_int, [ctypes.POINTER(_c_void_p)]),
+    "prnu_jpeg_decoder_destroy": (_c_int, [_c_void_p]),
+    "prnu_jpeg_info": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "prnu_jpeg_decode_batch": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64, _c_i64,
+                                        _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "prnu_ctx_set_timing": (_c_int, [_c_void_p, _c_int]),
     "prnu_ctx_last_kernel_ms": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "prnu_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,
