@@ -17,12 +17,22 @@
 // progressive, lossless, arithmetic-coded and 12-bit files and 1x2 chroma
 // subsampling (which libjpeg versions upsample differently).
 //
-// Work split.  The host parses the markers (tables, frame, scans) and finds
-// the restart-interval segments of each scan with memchr; the files go to
-// the device as they are (byte-stuffed).  jpeg_huff_kernel decodes one
-// segment per thread into 8x8 coefficient blocks (a whole scan when the file
-// has no restart markers); jpeg_idct_kernel runs one block per thread;
-// jpeg_color_kernel one output pixel per thread.
+// Work split.  The host parses the markers (tables, frame, scans), finds the
+// restart-interval segments of each scan and copies their entropy-coded
+// bytes, unstuffed, to the device.  Huffman decoding is parallel inside a
+// segment (a whole scan when the file has no restart markers): segments are
+// cut into chunks of kChunkBytes, and every chunk is decoded from a guessed
+// state (bit position, block of the MCU, coefficient index); the exit state
+// of chunk t becomes the entry state of chunk t+1, and chunks whose entry
+// changed are decoded again until no entry changes.  Huffman codes
+// resynchronise within a few symbols, so a guess that is wrong at the start
+// of a chunk gives the right state at its end and two rounds usually settle
+// every chunk; the result does not depend on that speed, only on the exact
+// fixed point.  Then a prefix sum over each segment's chunks gives each
+// chunk its first block, a write pass stores the coefficients (DC as
+// differences), and a per-component scan turns the differences into DC
+// values (the prediction chain).  jpeg_idct_kernel runs one block per
+// thread, jpeg_color_kernel one output pixel per thread.
 #include "prnu_internal.hpp"
 
 #include <algorithm>
@@ -62,12 +72,18 @@ struct JImage {
     int64_t rgb;     // output offset (bytes)
 };
 
+constexpr int kMaxBlocksPerMcu = 10;  // T.81 B.2.3
+constexpr int kChunkBytes = 256;      // unstuffed entropy-coded bytes per chunk
+
 struct JSeg {
-    int image, ns;
-    int comp[4], dc[4], ac[4];  // component index, global DC / AC table
+    int image, ns, nbu;         // components in the scan, blocks per MCU
+    int comp[4], dc[4], ac[4];  // per scan slot: component index, global DC / AC table
+    int uslot[kMaxBlocksPerMcu];  // per block of the MCU: scan slot, block row and column offsets
+    int uv[kMaxBlocksPerMcu], uh[kMaxBlocksPerMcu];
     int mcu0, mcus;             // first MCU (block, for ns == 1) and count
     int mcux;                   // MCU (block, for ns == 1) columns of the scan
-    int64_t begin, end;         // stuffed entropy-coded bytes in the batch buffer
+    int chunk0, nchunk;         // this segment's chunks
+    int64_t begin, end;         // unstuffed entropy-coded bytes in the batch buffer
 };
 
 // zigzag position -> natural (row-major) index; positions past 63 (corrupt
@@ -78,27 +94,29 @@ __constant__ uint8_t kZigzag[80] = {
     39, 46, 53, 60, 61, 54, 47, 55, 62, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63};
 
 // ----------------------------------------------------------- bit reading
-// MSB-first bits of a byte-stuffed segment: 0xFF 0x00 reads as 0xFF; at a
-// marker or the segment end the stream continues with zero bits (libjpeg
-// does the same at a premature marker).
+// MSB-first bits of an unstuffed segment; past its end the stream continues
+// with zero bits (libjpeg does the same at a premature marker).  pos() is
+// the absolute bit index of the next unread bit.
 struct BitReader {
-    const uint8_t *p, *end;
+    const uint8_t *base, *p, *end;
     uint64_t acc;
     int n;
+    __device__ void init(const uint8_t *data, int64_t bit, const uint8_t *seg_end) {
+        base = data;
+        p = data + (bit >> 3);
+        end = seg_end;
+        acc = 0;
+        n = 0;
+        fill();
+        const int sk = (int)(bit & 7);
+        acc <<= sk;
+        n -= sk;
+    }
+    __device__ int64_t pos() const { return (int64_t)(p - base) * 8 - n; }
     __device__ void fill() {
         while (n <= 56) {
-            uint64_t b = 0;
-            if (p < end) {
-                b = *p++;
-                if (b == 0xFF) {
-                    if (p < end && *p == 0x00) {
-                        ++p;
-                    } else {
-                        b = 0;
-                        p = end;
-                    }
-                }
-            }
+            const uint64_t b = p < end ? *p : 0;
+            ++p;
             acc |= b << (56 - n);
             n += 8;
         }
@@ -137,59 +155,202 @@ struct BitReader {
 
 __device__ __forceinline__ int extend(int v, int s) { return (s && v < (1 << (s - 1))) ? v - (1 << s) + 1 : v; }
 
-__device__ void decode_block(BitReader &br, const HuffTab &dc, const HuffTab &ac, int &pred, int16_t *blk,
-                             int &bad) {
-    uint4 *z = reinterpret_cast<uint4 *>(blk);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) z[i] = make_uint4(0u, 0u, 0u, 0u);
-    const int s = br.huff(dc, bad) & 15;
-    pred += extend(br.get(s), s);
-    blk[0] = (int16_t)pred;
-    for (int k = 1; k < 64;) {
-        const int rs = br.huff(ac, bad);
-        const int r = rs >> 4, sz = rs & 15;
-        if (sz) {
-            k += r;
-            if (k > 63) bad = 1;
-            blk[kZigzag[min(k, 79)]] = (int16_t)extend(br.get(sz), sz);
-            ++k;
+// Decoder state between two symbols: bit position, block of the MCU, next
+// coefficient index (0: a DC symbol is next).
+__device__ __forceinline__ int64_t pack_state(int64_t pos, int u, int k) { return (pos << 16) | (u << 8) | k; }
+
+// Coefficient block of the segment's block number b (decode order).
+__device__ __forceinline__ int16_t *block_at(const JSeg &g, const JImage &im, int16_t *coef, int b) {
+    if (g.ns == 1) {
+        const JComp &c = im.c[g.comp[0]];
+        const int q = g.mcu0 + b;
+        return coef + (c.coef + (int64_t)(q / g.mcux) * c.bw + q % g.mcux) * 64;
+    }
+    const int m = g.mcu0 + b / g.nbu, u = b % g.nbu;
+    const JComp &c = im.c[g.comp[g.uslot[u]]];
+    return coef + (c.coef + (int64_t)((m / g.mcux) * c.v + g.uv[u]) * c.bw + (m % g.mcux) * c.h + g.uh[u]) * 64;
+}
+
+// Decode symbols from `state` while the position is before `stop`.  WRITE:
+// store coefficients (DC as the difference) of blocks [0, total) into
+// `coef`; the block in progress at the entry is number first - 1, the next
+// one `first`.  Returns the exit state; *started = DC symbols decoded.
+template <bool WRITE>
+__device__ int64_t run_chunk(const JSeg &g, const JImage &im, const HuffTab *__restrict__ tabs,
+                             const uint8_t *__restrict__ data, int64_t state, int64_t stop, int *started, int first,
+                             int16_t *__restrict__ coef, int &bad) {
+    BitReader br;
+    br.init(data, state >> 16, data + g.end);
+    int u = (int)((state >> 8) & 0xFF), k = (int)(state & 0xFF);
+    const int total = g.mcus * g.nbu;
+    int cur = first - 1;  // the block in progress (k > 0) or the one before the next DC symbol
+    int n = 0;
+    int16_t *blk = (WRITE && cur >= 0 && cur < total) ? block_at(g, im, coef, cur) : nullptr;
+    while (br.pos() < stop) {
+        const int slot = g.uslot[u];
+        if (k == 0) {
+            const int s = br.huff(tabs[g.dc[slot]], bad) & 15;
+            const int diff = extend(br.get(s), s);
+            ++cur;
+            ++n;
+            if (WRITE) {
+                blk = cur < total ? block_at(g, im, coef, cur) : nullptr;
+                if (blk) blk[0] = (int16_t)diff;
+            }
+            k = 1;
         } else {
-            if (r != 15) break;
-            k += 16;
+            const int rs = br.huff(tabs[g.ac[slot]], bad);
+            const int r = rs >> 4, sz = rs & 15;
+            if (sz) {
+                k += r;
+                const int v = extend(br.get(sz), sz);
+                if (WRITE && blk) blk[kZigzag[min(k, 79)]] = (int16_t)v;
+                ++k;
+            } else {
+                k = (r == 15) ? k + 16 : 64;  // ZRL or end of block
+            }
         }
+        if (k >= 64) {
+            k = 0;
+            u = (u + 1 == g.nbu) ? 0 : u + 1;
+            // the segment's last block is complete: what follows is the
+            // 1-bit padding before the marker, never decoded (by libjpeg
+            // either), so it cannot raise the corrupt-data flag
+            if (WRITE && cur + 1 >= total) break;
+        }
+    }
+    *started = n;
+    return pack_state(br.pos(), u, k);
+}
+
+__device__ __forceinline__ int64_t chunk_start_bit(const JSeg &g, int ch) {
+    return (g.begin + (int64_t)(ch - g.chunk0) * kChunkBytes) * 8;
+}
+__device__ __forceinline__ int64_t chunk_stop_bit(const JSeg &g, int ch) {
+    return min(g.end, g.begin + (int64_t)(ch - g.chunk0 + 1) * kChunkBytes) * 8;
+}
+
+// Every chunk starts from its own first bit, at the start of an MCU (exact
+// for each segment's first chunk, a guess for the others).
+__global__ __launch_bounds__(256) void jpeg_chunk_init_kernel(const JSeg *__restrict__ segs,
+                                                              const int *__restrict__ chunk_seg, int nchunk,
+                                                              int64_t *__restrict__ entry, int *__restrict__ dirty) {
+    const int ch = blockIdx.x * 256 + threadIdx.x;
+    if (ch >= nchunk) return;
+    entry[ch] = pack_state(chunk_start_bit(segs[chunk_seg[ch]], ch), 0, 0);
+    dirty[ch] = 1;
+}
+
+// Decode the chunks whose entry state changed; record exit states.
+__global__ __launch_bounds__(64) void jpeg_chunk_sync_kernel(const JSeg *__restrict__ segs,
+                                                             const int *__restrict__ chunk_seg, int nchunk,
+                                                             const JImage *__restrict__ imgs,
+                                                             const HuffTab *__restrict__ tabs,
+                                                             const uint8_t *__restrict__ data,
+                                                             const int64_t *__restrict__ entry,
+                                                             int64_t *__restrict__ exit_state,
+                                                             int *__restrict__ started, int *__restrict__ dirty) {
+    const int ch = blockIdx.x * 64 + threadIdx.x;
+    if (ch >= nchunk || !dirty[ch]) return;
+    const JSeg &g = segs[chunk_seg[ch]];
+    int bad = 0, n = 0;
+    exit_state[ch] = run_chunk<false>(g, imgs[g.image], tabs, data, entry[ch], chunk_stop_bit(g, ch), &n, 0,
+                                      nullptr, bad);
+    started[ch] = n;
+    dirty[ch] = 0;
+}
+
+// entry[t] = exit[t-1] inside a segment; flags what changed.
+__global__ __launch_bounds__(256) void jpeg_chunk_link_kernel(const JSeg *__restrict__ segs,
+                                                              const int *__restrict__ chunk_seg, int nchunk,
+                                                              const int64_t *__restrict__ exit_state,
+                                                              int64_t *__restrict__ entry, int *__restrict__ dirty,
+                                                              int *__restrict__ changed) {
+    const int ch = blockIdx.x * 256 + threadIdx.x;
+    if (ch >= nchunk || ch == segs[chunk_seg[ch]].chunk0) return;
+    const int64_t e = exit_state[ch - 1];
+    if (e != entry[ch]) {
+        entry[ch] = e;
+        dirty[ch] = 1;
+        *changed = 1;
     }
 }
 
-// One segment (restart interval, or a whole scan) per thread.
-__global__ __launch_bounds__(64) void jpeg_huff_kernel(const JSeg *__restrict__ segs, int nseg,
-                                                       const JImage *__restrict__ imgs,
-                                                       const HuffTab *__restrict__ tabs,
-                                                       const uint8_t *__restrict__ data, int16_t *__restrict__ coef,
-                                                       int *__restrict__ bad_out) {
+// First block of each chunk: prefix sum of the blocks started before it in
+// its segment.  One thread per segment.
+__global__ __launch_bounds__(64) void jpeg_chunk_scan_kernel(const JSeg *__restrict__ segs, int nseg,
+                                                             const int *__restrict__ started,
+                                                             int *__restrict__ first) {
     const int si = blockIdx.x * 64 + threadIdx.x;
     if (si >= nseg) return;
-    const JSeg g = segs[si];
+    const JSeg &g = segs[si];
+    int run = 0;
+    for (int ch = g.chunk0; ch < g.chunk0 + g.nchunk; ++ch) {
+        first[ch] = run;
+        run += started[ch];
+    }
+}
+
+__global__ __launch_bounds__(64) void jpeg_chunk_write_kernel(const JSeg *__restrict__ segs,
+                                                              const int *__restrict__ chunk_seg, int nchunk,
+                                                              const JImage *__restrict__ imgs,
+                                                              const HuffTab *__restrict__ tabs,
+                                                              const uint8_t *__restrict__ data,
+                                                              const int64_t *__restrict__ entry,
+                                                              const int *__restrict__ first,
+                                                              int16_t *__restrict__ coef, int *__restrict__ bad_out) {
+    const int ch = blockIdx.x * 64 + threadIdx.x;
+    if (ch >= nchunk) return;
+    const JSeg &g = segs[chunk_seg[ch]];
+    int bad = 0, n = 0;
+    (void)run_chunk<true>(g, imgs[g.image], tabs, data, entry[ch], chunk_stop_bit(g, ch), &n, first[ch], coef, bad);
+    if (bad) bad_out[g.image] = 1;
+}
+
+// DC prediction: per (segment, scan slot), the component's blocks in decode
+// order hold differences; replace them by running sums (reset at each
+// segment, i.e. restart interval).  grid (segments, 4), 256 threads: each
+// thread sums a contiguous run, then a block-wide exclusive scan.
+__global__ __launch_bounds__(256) void jpeg_dc_scan_kernel(const JSeg *__restrict__ segs,
+                                                           const JImage *__restrict__ imgs,
+                                                           int16_t *__restrict__ coef) {
+    __shared__ int part[256];
+    const JSeg &g = segs[blockIdx.x];
+    const int slot = blockIdx.y;
+    if (slot >= g.ns) return;
     const JImage &im = imgs[g.image];
-    BitReader br{data + g.begin, data + g.end, 0ull, 0};
-    int pred[4] = {0, 0, 0, 0};
-    int bad = 0;
-    for (int m = g.mcu0; m < g.mcu0 + g.mcus; ++m) {
-        const int my = m / g.mcux, mx = m - (m / g.mcux) * g.mcux;
+    const JComp &c = im.c[g.comp[slot]];
+    const int per_mcu = g.ns == 1 ? 1 : c.v * c.h;
+    const int count = g.mcus * per_mcu;
+    const int span = (count + 255) / 256;
+    const int i0 = min(count, (int)threadIdx.x * span), i1 = min(count, i0 + span);
+    auto at = [&](int i) -> int16_t * {
         if (g.ns == 1) {
-            const JComp &c = im.c[g.comp[0]];
-            decode_block(br, tabs[g.dc[0]], tabs[g.ac[0]], pred[0], coef + (c.coef + (int64_t)my * c.bw + mx) * 64,
-                         bad);
-            continue;
+            const int q = g.mcu0 + i;
+            return coef + (c.coef + (int64_t)(q / g.mcux) * c.bw + q % g.mcux) * 64;
         }
-        for (int k = 0; k < g.ns; ++k) {
-            const JComp &c = im.c[g.comp[k]];
-            for (int v = 0; v < c.v; ++v)
-                for (int h = 0; h < c.h; ++h)
-                    decode_block(br, tabs[g.dc[k]], tabs[g.ac[k]], pred[k],
-                                 coef + (c.coef + (int64_t)(my * c.v + v) * c.bw + mx * c.h + h) * 64, bad);
+        const int m = g.mcu0 + i / per_mcu, r = i % per_mcu;
+        return coef + (c.coef + (int64_t)((m / g.mcux) * c.v + r / c.h) * c.bw + (m % g.mcux) * c.h + r % c.h) * 64;
+    };
+    int sum = 0;
+    for (int i = i0; i < i1; ++i) sum += at(i)[0];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < 256; ++t) {
+            const int v = part[t];
+            part[t] = run;
+            run += v;
         }
     }
-    if (bad) bad_out[g.image] = 1;
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int i = i0; i < i1; ++i) {
+        int16_t *b = at(i);
+        run += b[0];
+        b[0] = (int16_t)run;
+    }
 }
 
 // ------------------------------------------------------------------- IDCT
@@ -355,8 +516,27 @@ struct ScanInfo {
     int comp[4] = {0, 0, 0, 0};
     HuffTab dc[4], ac[4];
     int64_t begin = 0, end = 0;                        // entropy-coded bytes in the file
-    std::vector<std::pair<int64_t, int64_t>> segs;     // restart segments [begin, end)
+    std::vector<std::pair<int64_t, int64_t>> segs;     // restart segments [begin, end), stuffed
+    std::vector<int64_t> seg_len;                      // their unstuffed lengths
 };
+
+// Entropy-coded bytes without the 0x00 that follows each 0xFF data byte
+// (T.81 F.1.2.3); returns the length, writes to dst when it is not null.
+int64_t unstuff(const uint8_t *src, int64_t n, uint8_t *dst) {
+    int64_t i = 0, o = 0;
+    while (i < n) {
+        const void *ff = std::memchr(src + i, 0xFF, (size_t)(n - i));
+        const int64_t j = ff ? (const uint8_t *)ff - src : n;
+        if (dst) std::memcpy(dst + o, src + i, (size_t)(j - i));
+        o += j - i;
+        if (j >= n) break;
+        if (dst) dst[o] = 0xFF;
+        ++o;
+        i = j + 1;
+        if (i < n && src[i] == 0x00) ++i;
+    }
+    return o;
+}
 
 struct FileInfo {
     int status = 0;  // 0 ok, PRNU_EINVAL / PRNU_ENOTSUP: rejected
@@ -530,6 +710,7 @@ int parse_file(const uint8_t *d, int64_t n, FileInfo &f) {
             }
             sc.end = j;
             sc.segs.push_back({seg0, j});
+            for (const auto &sg : sc.segs) sc.seg_len.push_back(unstuff(d + sg.first, sg.second - sg.first, nullptr));
             f.scans.push_back(sc);
             i = j;
         }
@@ -621,10 +802,12 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
     // and planes after it on the device
     std::vector<JImage> imgs((size_t)count);
     std::vector<JSeg> segs;
+    std::vector<int> chunk_seg;
     std::vector<HuffTab> tabs;
     std::vector<uint16_t> qtab;
-    int64_t file_bytes = 0, coef_blocks = 0, plane_bytes = 0;
-    int max_blocks = 0, max_w = 0, max_h = 0;
+    std::vector<int64_t> file_data((size_t)count, -1);  // first unstuffed byte of each accepted file
+    int64_t data_bytes = 0, coef_blocks = 0, plane_bytes = 0;
+    int max_blocks = 0, max_w = 0, max_h = 0, max_chunks = 1;
     for (int k = 0; k < count; ++k) {
         FileInfo &f = info[k];
         JImage &im = imgs[k];
@@ -668,8 +851,9 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
         }
         max_w = std::max(max_w, f.W);
         max_h = std::max(max_h, f.H);
-        // segments
-        const size_t seg_first = segs.size();
+        // segments, their chunks and their place in the unstuffed data
+        const size_t seg_first = segs.size(), chunk_first = chunk_seg.size();
+        const int64_t data_first = data_bytes;
         for (const ScanInfo &sc : f.scans) {
             JSeg base;
             std::memset(&base, 0, sizeof(base));
@@ -680,9 +864,29 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
                 const JComp &jc = im.c[sc.comp[0]];
                 base.mcux = (jc.dw + 7) / 8;
                 total = base.mcux * ((jc.dh + 7) / 8);
+                base.nbu = 1;
             } else {
                 base.mcux = mcux;
                 total = mcux * mcuy;
+                int u = 0;
+                for (int q = 0; q < sc.ns && u <= kMaxBlocksPerMcu; ++q) {
+                    const JComp &jc = im.c[sc.comp[q]];
+                    for (int vv = 0; vv < jc.v; ++vv)
+                        for (int hh = 0; hh < jc.h; ++hh) {
+                            if (u < kMaxBlocksPerMcu) {
+                                base.uslot[u] = q;
+                                base.uv[u] = vv;
+                                base.uh[u] = hh;
+                            }
+                            ++u;
+                        }
+                }
+                base.nbu = u;
+            }
+            if (base.nbu > kMaxBlocksPerMcu) {
+                f.status = PRNU_EINVAL;
+                f.err = "more than 10 blocks per MCU";
+                break;
             }
             for (int q = 0; q < sc.ns; ++q) {
                 base.comp[q] = sc.comp[q];
@@ -698,57 +902,116 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
                 f.err = "restart markers do not match the restart interval";
                 break;
             }
-            for (int s = 0; s < nseg; ++s) {
+            for (int q = 0; q < nseg; ++q) {
                 JSeg g = base;
-                g.mcu0 = s * ri;
-                g.mcus = std::min(ri, total - s * ri);
-                g.begin = file_bytes + sc.segs[s].first;
-                g.end = file_bytes + sc.segs[s].second;
+                g.mcu0 = q * ri;
+                g.mcus = std::min(ri, total - q * ri);
+                g.begin = data_bytes;
+                g.end = data_bytes + sc.seg_len[q];
+                data_bytes = g.end;
+                g.chunk0 = (int)chunk_seg.size();
+                g.nchunk = (int)std::max<int64_t>(1, (sc.seg_len[q] + kChunkBytes - 1) / kChunkBytes);
+                max_chunks = std::max(max_chunks, g.nchunk);
+                for (int t = 0; t < g.nchunk; ++t) chunk_seg.push_back((int)segs.size());
                 segs.push_back(g);
             }
         }
         if (f.status != PRNU_OK) {
             segs.resize(seg_first);
+            chunk_seg.resize(chunk_first);
+            data_bytes = data_first;
             im.W = im.H = 0;
             continue;
         }
-        file_bytes += align256((size_t)sizes[k]);
+        file_data[k] = data_first;
     }
+    const int nseg = (int)segs.size(), nchunk = (int)chunk_seg.size();
     const size_t o_img = 0, o_seg = align256(sizeof(JImage) * count);
-    const size_t o_tab = o_seg + align256(sizeof(JSeg) * std::max<size_t>(1, segs.size()));
+    const size_t o_tab = o_seg + align256(sizeof(JSeg) * std::max(1, nseg));
     const size_t o_q = o_tab + align256(sizeof(HuffTab) * std::max<size_t>(1, tabs.size()));
-    const size_t o_file = o_q + align256(sizeof(uint16_t) * std::max<size_t>(64, qtab.size()));
-    const size_t staged = o_file + (size_t)file_bytes;
-    const size_t o_coef = align256(staged);
+    const size_t o_cseg = o_q + align256(sizeof(uint16_t) * std::max<size_t>(64, qtab.size()));
+    const size_t o_data = o_cseg + align256(sizeof(int) * std::max(1, nchunk));
+    const size_t staged = o_data + align256((size_t)data_bytes + 8);
+    // device only: chunk states, coefficients, planes
+    const size_t o_entry = staged;
+    const size_t o_exit = o_entry + align256(sizeof(int64_t) * std::max(1, nchunk));
+    const size_t o_started = o_exit + align256(sizeof(int64_t) * std::max(1, nchunk));
+    const size_t o_first = o_started + align256(sizeof(int) * std::max(1, nchunk));
+    const size_t o_dirty = o_first + align256(sizeof(int) * std::max(1, nchunk));
+    const size_t o_coef = o_dirty + align256(sizeof(int) * std::max(1, nchunk));
     const size_t o_plane = o_coef + align256((size_t)coef_blocks * 128);
     const size_t total = o_plane + align256((size_t)plane_bytes);
     PRNU_TRY(dec->staging.ensure(staged));
     PRNU_TRY(dec->dev.ensure(total));
-    PRNU_TRY(dec->bad.ensure(sizeof(int) * (size_t)count));
+    PRNU_TRY(dec->bad.ensure(sizeof(int) * ((size_t)count + 1)));
     uint8_t *hs = static_cast<uint8_t *>(dec->staging.p);
     std::memcpy(hs + o_img, imgs.data(), sizeof(JImage) * count);
-    if (!segs.empty()) std::memcpy(hs + o_seg, segs.data(), sizeof(JSeg) * segs.size());
+    if (nseg) std::memcpy(hs + o_seg, segs.data(), sizeof(JSeg) * nseg);
     if (!tabs.empty()) std::memcpy(hs + o_tab, tabs.data(), sizeof(HuffTab) * tabs.size());
     if (!qtab.empty()) std::memcpy(hs + o_q, qtab.data(), sizeof(uint16_t) * qtab.size());
+    if (nchunk) std::memcpy(hs + o_cseg, chunk_seg.data(), sizeof(int) * nchunk);
     {
-        int64_t off = 0;
-        for (int k = 0; k < count; ++k) {
-            if (imgs[k].W == 0) continue;
-            std::memcpy(hs + o_file + off, files[k], (size_t)sizes[k]);
-            off += align256((size_t)sizes[k]);
-        }
+        // unstuffed entropy-coded data, file by file in parallel
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int nt = std::max(1, std::min(count, (int)std::min(16u, hw)));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; ++t)
+            pool.emplace_back([&, t]() {
+                for (int k = t; k < count; k += nt) {
+                    if (file_data[k] < 0) continue;
+                    int64_t off = file_data[k];
+                    for (const ScanInfo &sc : info[k].scans)
+                        for (size_t q = 0; q < sc.segs.size(); ++q)
+                            off += unstuff(files[k] + sc.segs[q].first, sc.segs[q].second - sc.segs[q].first,
+                                           hs + o_data + off);
+                }
+            });
+        for (auto &th : pool) th.join();
     }
     uint8_t *dv = static_cast<uint8_t *>(dec->dev.ptr);
     int *d_bad = static_cast<int *>(dec->bad.ptr);
+    int *d_changed = d_bad + count;
     PRNU_CHECK_HIP(hipMemcpyAsync(dv, hs, staged, hipMemcpyHostToDevice, st));
     PRNU_CHECK_HIP(hipMemsetAsync(d_bad, 0, sizeof(int) * (size_t)count, st));
     const JImage *d_img = reinterpret_cast<const JImage *>(dv + o_img);
+    const JSeg *d_seg = reinterpret_cast<const JSeg *>(dv + o_seg);
+    const HuffTab *d_tab = reinterpret_cast<const HuffTab *>(dv + o_tab);
+    const int *d_cseg = reinterpret_cast<const int *>(dv + o_cseg);
+    const uint8_t *d_data = dv + o_data;
+    int64_t *d_entry = reinterpret_cast<int64_t *>(dv + o_entry);
+    int64_t *d_exit = reinterpret_cast<int64_t *>(dv + o_exit);
+    int *d_started = reinterpret_cast<int *>(dv + o_started);
+    int *d_first = reinterpret_cast<int *>(dv + o_first);
+    int *d_dirty = reinterpret_cast<int *>(dv + o_dirty);
     int16_t *d_coef = reinterpret_cast<int16_t *>(dv + o_coef);
     uint8_t *d_plane = dv + o_plane;
-    if (!segs.empty()) {
-        hipLaunchKernelGGL(jpeg_huff_kernel, dim3((unsigned)((segs.size() + 63) / 64)), dim3(64), 0, st,
-                           reinterpret_cast<const JSeg *>(dv + o_seg), (int)segs.size(), d_img,
-                           reinterpret_cast<const HuffTab *>(dv + o_tab), dv + o_file, d_coef, d_bad);
+    if (nseg) {
+        PRNU_CHECK_HIP(hipMemsetAsync(d_coef, 0, (size_t)coef_blocks * 128, st));
+        const dim3 g256((unsigned)((nchunk + 255) / 256)), g64((unsigned)((nchunk + 63) / 64));
+        hipLaunchKernelGGL(jpeg_chunk_init_kernel, g256, dim3(256), 0, st, d_seg, d_cseg, nchunk, d_entry, d_dirty);
+        PRNU_CHECK_LAUNCH();
+        // entry states settle: after round r the first r + 1 chunks of every
+        // segment are exact, so max_chunks + 1 rounds always suffice
+        for (int round = 0; round <= max_chunks; ++round) {
+            hipLaunchKernelGGL(jpeg_chunk_sync_kernel, g64, dim3(64), 0, st, d_seg, d_cseg, nchunk, d_img, d_tab,
+                               d_data, d_entry, d_exit, d_started, d_dirty);
+            PRNU_CHECK_LAUNCH();
+            PRNU_CHECK_HIP(hipMemsetAsync(d_changed, 0, sizeof(int), st));
+            hipLaunchKernelGGL(jpeg_chunk_link_kernel, g256, dim3(256), 0, st, d_seg, d_cseg, nchunk, d_exit,
+                               d_entry, d_dirty, d_changed);
+            PRNU_CHECK_LAUNCH();
+            int changed = 0;
+            PRNU_CHECK_HIP(hipMemcpyAsync(&changed, d_changed, sizeof(int), hipMemcpyDeviceToHost, st));
+            PRNU_CHECK_HIP(hipStreamSynchronize(st));
+            if (!changed) break;
+        }
+        hipLaunchKernelGGL(jpeg_chunk_scan_kernel, dim3((unsigned)((nseg + 63) / 64)), dim3(64), 0, st, d_seg, nseg,
+                           d_started, d_first);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(jpeg_chunk_write_kernel, g64, dim3(64), 0, st, d_seg, d_cseg, nchunk, d_img, d_tab, d_data,
+                           d_entry, d_first, d_coef, d_bad);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(jpeg_dc_scan_kernel, dim3((unsigned)nseg, 4), dim3(256), 0, st, d_seg, d_img, d_coef);
         PRNU_CHECK_LAUNCH();
         hipLaunchKernelGGL(jpeg_idct_kernel, dim3((unsigned)((max_blocks + 255) / 256), 3, (unsigned)count),
                            dim3(256), 0, st, d_img, reinterpret_cast<const uint16_t *>(dv + o_q), d_coef, d_plane);

@@ -98,6 +98,42 @@ def test_bench_frame_and_same_size_batch():
     dec.close()
 
 
+@pytest.mark.parametrize("sub", [0, 2])
+def test_noise_frames_bitwise(sub):
+    """Pixel noise gives long codes and the slowest resynchronisation of
+    chunk decoders guessed mid-stream; the result must not depend on it."""
+    rng = np.random.default_rng(11 + sub)
+    sizes = [(480, 640), (97, 333)]
+    files = [encode(rng.integers(0, 256, (h, w, 3), dtype=np.uint8), quality=q, subsampling=sub)
+             for (h, w), q in zip(sizes, (97, 60))]
+    dec = jpeg.JpegDecoder()
+    got, st = decode_mixed(dec, files, sizes)
+    assert st == [0, 0]
+    for f, g in zip(files, got):
+        assert np.array_equal(g, pil_rgb(f))
+    dec.close()
+
+
+def test_corrupt_entropy_data_keeps_the_batch():
+    """Bytes changed inside the entropy-coded data: the file still decodes
+    (flagged when a code does not exist), the other files stay exact."""
+    good = encode(frame(200, 150, 3), quality=90, subsampling=2)
+    bad = bytearray(good)
+    sos = bad.index(b"\xff\xda")
+    start = sos + 2 + int.from_bytes(bad[sos + 2:sos + 4], "big")
+    rng = np.random.default_rng(2)
+    for i in rng.integers(start + 50, len(bad) - 50, 40):
+        if bad[i] != 0xFF and bad[i - 1] != 0xFF and (bad[i] ^ 0x5A) != 0xFF:
+            bad[i] ^= 0x5A
+    files = [good, bytes(bad), good]
+    dec = jpeg.JpegDecoder()
+    out, st = dec.decode(files, strict=False)
+    assert st[0] == 0 and st[2] == 0 and st[1] in (0, _lib.JPEG_CORRUPT_DATA)
+    want = pil_rgb(good)
+    assert np.array_equal(out[0].cpu().numpy(), want) and np.array_equal(out[2].cpu().numpy(), want)
+    dec.close()
+
+
 def test_rejected_files_leave_the_rest_decoded():
     good = [encode(frame(40, 30, 1)), encode(frame(40, 30, 2), subsampling=1)]
     prog = encode(frame(40, 30, 3), progressive=True)
