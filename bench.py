@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
-    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks"],
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg"],
                     help="run one secondary leg alone and print its result (for profiling)")
     ap.add_argument("--extract-shape", default=None, metavar="HxW",
                     help="extraction leg: one frame shape only (default: 1536x2048 and 512x512)")
@@ -450,6 +450,59 @@ def extraction_bench(torch, batch=32, shape=None, only_mode=None):
     return res
 
 
+def jpeg_bench(torch, batch=32, reps=3):
+    """JPEG files -> RGB8 frames on the GPU (prnu_jpeg_decode_batch) for
+    `batch` synthetic 2048x1536 camera frames encoded 4:2:0 at quality 92,
+    from host bytes (parse, unstuff and the copy included), then the same
+    followed by NCC extraction (files -> patterns).  Beside it: libjpeg-turbo
+    through Pillow, one thread, on 8 of the files."""
+    import io
+    from PIL import Image
+    from rocket_amd import _lib, csi, jpeg
+    g = torch.Generator(device="cuda").manual_seed(1)
+    cams = torch.randn((4, HEIGHT * WIDTH), generator=g, device="cuda")
+    gray = synth_images(torch, cams, [i % 4 for i in range(batch)], seed=5, h=HEIGHT, w=WIDTH).cpu().numpy()
+    files = []
+    for im in gray:
+        im = im.reshape(HEIGHT, WIDTH)
+        buf = io.BytesIO()
+        Image.fromarray(np.stack([im, np.roll(im, 7, axis=1), 255 - im], -1)).save(buf, "JPEG", quality=92,
+                                                                                   subsampling=2)
+        files.append(buf.getvalue())
+    dec = jpeg.JpegDecoder()
+    rgb = torch.empty((batch, HEIGHT, WIDTH, 3), dtype=torch.uint8, device="cuda")
+    dec.decode(files, out=rgb)
+    exact = bool(np.array_equal(rgb[1].cpu().numpy(), np.asarray(Image.open(io.BytesIO(files[1])).convert("RGB"))))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        dec.decode(files, out=rgb)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t1) / reps
+    f = csi.PRNUFilter(HEIGHT, WIDTH)
+    pat = torch.empty((batch, f.get_output_size() // 4), device="cuda")
+    f.apply_gpu_batch(rgb.view(batch, -1), pat, _lib.INPUT_RGB8)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        dec.decode(files, out=rgb)
+        f.apply_gpu_batch(rgb.view(batch, -1), pat, _lib.INPUT_RGB8, sync=False)
+    torch.cuda.synchronize()
+    dt_pipe = (time.perf_counter() - t1) / reps
+    f.cleanup()
+    dec.close()
+    t1 = time.perf_counter()
+    for b in files[:8]:
+        np.asarray(Image.open(io.BytesIO(b)).convert("RGB"))
+    cpu = (time.perf_counter() - t1) / 8
+    mb = sum(map(len, files)) / batch / 1e6
+    return {"workload": f"jpeg{batch}_{WIDTH}x{HEIGHT}_420_q92", "mb_per_file": mb,
+            "decode_images_per_s": batch / dt, "decode_ms_per_batch": 1000.0 * dt,
+            "decode_compressed_MBps": batch * mb / dt, "bitexact_vs_libjpeg": exact,
+            "files_to_ncc_patterns_images_per_s": batch / dt_pipe,
+            "cpu_libjpeg_turbo_pillow_images_per_s_1_thread": 1.0 / cpu}
+
+
 PCE_H, PCE_W, PCE_COUNT, PCE_CAMERAS = 1024, 1024, 256, 8
 
 
@@ -531,7 +584,8 @@ def main():
         leg = {"pce": lambda: pce_bench(torch), "extract": lambda: extraction_bench(torch, shape=args.extract_shape, only_mode=args.extract_mode),
                "groups": lambda: groups_bench(torch),
                "serving": lambda: serving_bench(torch, args.serving_seconds),
-               "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch)}[args.only]
+               "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch),
+               "jpeg": lambda: jpeg_bench(torch)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
     prec = args.precision
@@ -650,7 +704,7 @@ def main():
             f"{groups['eager_ms_per_step']:.2f} ms eager, {groups['separate_launches_ms_per_step']:.2f} ms as "
             f"separate launches), |dNCC| {groups['max_abs_dncc_vs_fp64_sample']:.2e}")
 
-    extract = pce = None
+    extract = pce = jpg = None
     if not args.no_extract:
         log("extraction ...")
         extract = extraction_bench(torch)
@@ -660,6 +714,11 @@ def main():
         pce = pce_bench(torch)
         log(f"PCE: {pce['reference_pairs_per_s']:.1f} pairs/s (reference), "
             f"{pce['textbook_pairs_per_s']:.1f} pairs/s (textbook)")
+        log("JPEG ...")
+        jpg = jpeg_bench(torch)
+        log(f"JPEG: {jpg['decode_images_per_s']:.0f} images/s decoded, "
+            f"{jpg['files_to_ncc_patterns_images_per_s']:.0f} images/s files -> NCC patterns, "
+            f"libjpeg-turbo one thread {jpg['cpu_libjpeg_turbo_pillow_images_per_s_1_thread']:.0f} images/s")
 
     log("c0 accuracy (64 frames of 512x512, all pairs) ...")
     c0 = c0_bench(torch)
@@ -700,6 +759,7 @@ def main():
         "serving": serve,
         "extraction": extract,
         "pce": pce,
+        "jpeg": jpg,
         "groups": groups,
         "c0": c0,
         "measured_peaks": peaks,

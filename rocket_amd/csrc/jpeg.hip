@@ -21,18 +21,20 @@
 // restart-interval segments of each scan and copies their entropy-coded
 // bytes, unstuffed, to the device.  Huffman decoding is parallel inside a
 // segment (a whole scan when the file has no restart markers): segments are
-// cut into chunks of kChunkBytes, and every chunk is decoded from a guessed
-// state (bit position, block of the MCU, coefficient index); the exit state
-// of chunk t becomes the entry state of chunk t+1, and chunks whose entry
-// changed are decoded again until no entry changes.  Huffman codes
-// resynchronise within a few symbols, so a guess that is wrong at the start
-// of a chunk gives the right state at its end and two rounds usually settle
-// every chunk; the result does not depend on that speed, only on the exact
-// fixed point.  Then a prefix sum over each segment's chunks gives each
-// chunk its first block, a write pass stores the coefficients (DC as
-// differences), and a per-component scan turns the differences into DC
-// values (the prediction chain).  jpeg_idct_kernel runs one block per
-// thread, jpeg_color_kernel one output pixel per thread.
+// cut into chunks of kChunkBytes, one lane each, and every chunk is first
+// decoded from a guessed state (its first bit, block 0 of an MCU, a DC symbol
+// next).  Linking rounds then give chunk t the exit state (bit position,
+// block of the MCU, coefficient index) of chunk t-1 and decode again the
+// chunks whose entry changed, until a round changes nothing.  Huffman codes
+// resynchronise within some hundreds of bits on photographic data, so a
+// guessed chunk usually ends in the exact state and a few rounds settle a
+// batch; the result does not depend on that speed, only on the exact fixed
+// point (rounds are bounded by the chunks per segment).  The rounds also
+// count the blocks each chunk starts and sum its DC differences per
+// component; a prefix over each segment's chunks then gives every chunk its
+// first block and DC predictors, and a write pass stores the coefficients
+// with final DC values.  jpeg_idct_kernel runs one block per thread,
+// jpeg_color_kernel four output pixels per thread.
 #include "prnu_internal.hpp"
 
 #include <algorithm>
@@ -83,6 +85,8 @@ struct JSeg {
     int mcu0, mcus;             // first MCU (block, for ns == 1) and count
     int mcux;                   // MCU (block, for ns == 1) columns of the scan
     int chunk0, nchunk;         // this segment's chunks
+    int tab_off, tab_len;       // the scan's two-level tables (uint16 entries) in the batch table array
+    int trel[3];                // per scan slot: DC table offset | AC table offset << 16, in the scan's tables
     int64_t begin, end;         // unstuffed entropy-coded bytes in the batch buffer
 };
 
@@ -93,264 +97,394 @@ __constant__ uint8_t kZigzag[80] = {
     6,  7,  14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31,
     39, 46, 53, 60, 61, 54, 47, 55, 62, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63};
 
-// ----------------------------------------------------------- bit reading
-// MSB-first bits of an unstuffed segment; past its end the stream continues
-// with zero bits (libjpeg does the same at a premature marker).  pos() is
-// the absolute bit index of the next unread bit.
-struct BitReader {
-    const uint8_t *base, *p, *end;
+// ---------------------------------------------------------- chunk decoding
+// One lane decodes one chunk.  A workgroup is one wave of kLanes chunk
+// decoders of one scan (the host pads every scan's chunk list to a multiple
+// of kLanes).  Before decoding it stages in LDS
+//   * each working lane's window: the chunk's bytes from the 4-byte boundary
+//     at or before its first byte, as big-endian dwords, zero from the
+//     segment's end on (past the end of a segment the stream continues with
+//     zero bits, as libjpeg reads a premature marker);
+//   * the scan's two-level decoding tables (build_lut2).
+// A lane's reads stay inside its window: it enters at most 30 bits past its
+// chunk's first bit (the exit of the chunk before it, whose last symbol
+// started before that chunk's end), stops before the first symbol that
+// starts at or after its own end, and a symbol (code and extra bits) spans at
+// most 31 bits; with 64 bits in the accumulator and the next word preloaded
+// that is at most 3 + kChunkBytes + 12 bytes of the 4 * kWinWords.
+constexpr int kLanes = 64;
+constexpr int kWinWords = ((kChunkBytes + 24) / 4) | 1;  // odd, which spreads the lanes' windows over the LDS banks
+constexpr int kScanTabWords = 8192;  // a scan's tables: first levels (512 entries each) and subtables
+
+struct ChunkLds {
+    uint32_t win[kLanes * kWinWords];
+    uint32_t tab[kScanTabWords / 2];  // uint16 entries
+    int64_t wbase[kLanes];            // per compact working slot: first window byte
+    int64_t wend[kLanes];             // and the end of its segment (bytes)
+};
+
+// MSB-first bits of one lane's window: acc holds the next n bits, nxt the
+// window word after them.
+struct WinBits {
+    const uint32_t *w;
     uint64_t acc;
-    int n;
-    __device__ void init(const uint8_t *data, int64_t bit, const uint8_t *seg_end) {
-        base = data;
-        p = data + (bit >> 3);
-        end = seg_end;
-        acc = 0;
-        n = 0;
-        fill();
-        const int sk = (int)(bit & 7);
-        acc <<= sk;
-        n -= sk;
+    uint32_t nxt;
+    int n, wi;  // valid bits in acc; window index of nxt
+    __device__ uint32_t word(int i) const { return i < kWinWords ? w[i] : 0u; }
+    __device__ void init(const uint32_t *win, int rel) {  // rel: first bit, relative to the window
+        w = win;
+        const int i = rel >> 5, sh = rel & 31;
+        acc = (((uint64_t)word(i) << 32) | word(i + 1)) << sh;
+        n = 64 - sh;
+        wi = i + 2;
+        nxt = word(wi);
     }
-    __device__ int64_t pos() const { return (int64_t)(p - base) * 8 - n; }
-    __device__ void fill() {
-        while (n <= 56) {
-            const uint64_t b = p < end ? *p : 0;
-            ++p;
-            acc |= b << (56 - n);
-            n += 8;
+    __device__ void refill() {  // n >= 33 afterwards
+        if (n <= 32) {
+            acc |= (uint64_t)nxt << (32 - n);
+            n += 32;
+            nxt = word(++wi);
         }
     }
-    __device__ int get(int k) {  // k in [0, 16]
-        if (k == 0) return 0;
-        fill();
-        const int v = (int)(acc >> (64 - k));
-        acc <<= k;
-        n -= k;
+    __device__ int pos() const { return wi * 32 - n; }  // relative to the window
+    __device__ int take(int s) {                         // s in [0, 15]
+        if (s == 0) return 0;
+        const int v = (int)(acc >> (64 - s));
+        acc <<= s;
+        n -= s;
         return v;
-    }
-    __device__ int huff(const HuffTab &t, int &bad) {
-        fill();
-        const int e = t.look[acc >> (64 - kLook)];
-        if (e) {
-            const int len = e >> 8;
-            acc <<= len;
-            n -= len;
-            return e & 0xFF;
-        }
-        for (int l = kLook + 1; l <= 16; ++l) {
-            const int code = (int)(acc >> (64 - l));
-            if (code <= t.maxcode[l]) {
-                acc <<= l;
-                n -= l;
-                return t.vals[(t.valoff[l] + code) & 0xFF];
-            }
-        }
-        bad = 1;  // no such code: libjpeg warns and takes symbol 0
-        acc <<= 16;
-        n -= 16;
-        return 0;
     }
 };
 
 __device__ __forceinline__ int extend(int v, int s) { return (s && v < (1 << (s - 1))) ? v - (1 << s) + 1 : v; }
 
+// Entry of the code at the head of acc in a two-level table T: (length << 8)
+// | symbol, 0 for no such code, -1 when the canonical walk must decide.
+__device__ __forceinline__ int lut_entry(const uint16_t *T, uint64_t acc) {
+    int e = T[(int)(acc >> 55)];
+    if (e & 0x8000) {
+        const int off = e & 0xFFF, nb = (e >> 12) & 7;
+        e = off == 0xFFF ? -1 : (int)T[512 + off + ((int)(acc >> (55 - nb)) & ((1 << nb) - 1))];
+    }
+    return e;
+}
+
+// Canonical decoding (T.81 F.2.2.3) of the code at the head of acc.
+__device__ int huff_walk(const HuffTab &t, uint64_t acc) {
+    for (int l = 1; l <= 16; ++l) {
+        const int code = (int)(acc >> (64 - l));
+        if (code <= t.maxcode[l]) return (l << 8) | t.vals[(t.valoff[l] + code) & 0xFF];
+    }
+    return 0;
+}
+
 // Decoder state between two symbols: bit position, block of the MCU, next
 // coefficient index (0: a DC symbol is next).
 __device__ __forceinline__ int64_t pack_state(int64_t pos, int u, int k) { return (pos << 16) | (u << 8) | k; }
 
-// Coefficient block of the segment's block number b (decode order).
-__device__ __forceinline__ int16_t *block_at(const JSeg &g, const JImage &im, int16_t *coef, int b) {
-    if (g.ns == 1) {
-        const JComp &c = im.c[g.comp[0]];
-        const int q = g.mcu0 + b;
-        return coef + (c.coef + (int64_t)(q / g.mcux) * c.bw + q % g.mcux) * 64;
-    }
-    const int m = g.mcu0 + b / g.nbu, u = b % g.nbu;
-    const JComp &c = im.c[g.comp[g.uslot[u]]];
-    return coef + (c.coef + (int64_t)((m / g.mcux) * c.v + g.uv[u]) * c.bw + (m % g.mcux) * c.h + g.uh[u]) * 64;
+template <class T>
+__device__ __forceinline__ T sel3(int s, T a, T b, T c) {
+    return s == 0 ? a : s == 1 ? b : c;
 }
 
-// Decode symbols from `state` while the position is before `stop`.  WRITE:
-// store coefficients (DC as the difference) of blocks [0, total) into
-// `coef`; the block in progress at the entry is number first - 1, the next
-// one `first`.  Returns the exit state; *started = DC symbols decoded.
+// What a lane knows of its segment: per block of the MCU (6 bits each) the
+// scan slot, block row and block column inside the MCU; per scan slot the
+// offsets of its DC and AC tables in the scan's two-level tables (low / high
+// half) and, for the canonical walk, the global index of slot 0's DC table
+// (DC and AC alternate per slot).
+struct Lane {
+    uint64_t umap;
+    int nbu, gt0, tr0, tr1, tr2;
+};
+
+__device__ __forceinline__ Lane lane_of(const JSeg &g) {
+    Lane ln;
+    ln.nbu = g.nbu;
+    ln.gt0 = g.dc[0];
+    ln.tr0 = g.trel[0];
+    ln.tr1 = g.trel[g.ns > 1 ? 1 : 0];
+    ln.tr2 = g.trel[g.ns > 2 ? 2 : 0];
+    uint64_t m = 0;
+    for (int u = 0; u < g.nbu && u < kMaxBlocksPerMcu; ++u)
+        m |= (uint64_t)(g.uslot[u] | (g.uv[u] << 2) | (g.uh[u] << 4)) << (6 * u);
+    ln.umap = m;
+    return ln;
+}
+
+// Where the write pass puts coefficient blocks: per scan slot the
+// component's first block, block columns and blocks per MCU across / down
+// (1 x 1 in a single-component scan); the MCU of the current block.
+struct Cursor {
+    int16_t *coef, *blk;
+    int64_t cb0, cb1, cb2;
+    int bw0, bw1, bw2, h0, h1, h2, v0, v1, v2;
+    int cur, total, mcux, mrow, mcol;
+    __device__ int16_t *addr(uint64_t umap, int u) const {
+        if (cur < 0 || cur >= total) return nullptr;
+        const int f = (int)(umap >> (6 * u)) & 63, s = f & 3, uv = (f >> 2) & 3, uh = f >> 4;
+        const int64_t b = sel3(s, cb0, cb1, cb2) + (int64_t)(mrow * sel3(s, v0, v1, v2) + uv) * sel3(s, bw0, bw1, bw2) +
+                          mcol * sel3(s, h0, h1, h2) + uh;
+        return coef + b * 64;
+    }
+};
+
+// Decode symbols from (bit position, block u, coefficient k) while the
+// position is before `stop` (relative to the lane's window).  d0..d2: per
+// scan slot, running sums of the DC differences (the DC predictors in the
+// write pass, which also stores the coefficients and stops after the
+// segment's last block).  A code that does not exist: libjpeg warns, takes
+// symbol 0 after 17 bits; flagged in `bad`.  Returns the exit position; u
+// and k are updated.
 template <bool WRITE>
-__device__ int64_t run_chunk(const JSeg &g, const JImage &im, const HuffTab *__restrict__ tabs,
-                             const uint8_t *__restrict__ data, int64_t state, int64_t stop, int *started, int first,
-                             int16_t *__restrict__ coef, int &bad) {
-    BitReader br;
-    br.init(data, state >> 16, data + g.end);
-    int u = (int)((state >> 8) & 0xFF), k = (int)(state & 0xFF);
-    const int total = g.mcus * g.nbu;
-    int cur = first - 1;  // the block in progress (k > 0) or the one before the next DC symbol
+__device__ int decode_chunk(WinBits &br, int stop, int &u, int &k, const Lane &ln, const uint16_t *T,
+                            const HuffTab *__restrict__ tabs, int &started, int &d0, int &d1, int &d2, Cursor &cz,
+                            int &bad) {
+    int slot = (int)(ln.umap >> (6 * u)) & 3;
+    int trel = sel3(slot, ln.tr0, ln.tr1, ln.tr2);
     int n = 0;
-    int16_t *blk = (WRITE && cur >= 0 && cur < total) ? block_at(g, im, coef, cur) : nullptr;
     while (br.pos() < stop) {
-        const int slot = g.uslot[u];
-        if (k == 0) {
-            const int s = br.huff(tabs[g.dc[slot]], bad) & 15;
-            const int diff = extend(br.get(s), s);
-            ++cur;
+        br.refill();
+        int e = lut_entry(T + (k == 0 ? (trel & 0xFFFF) : (trel >> 16)), br.acc);
+        if (e < 0) e = huff_walk(tabs[ln.gt0 + 2 * slot + (k == 0 ? 0 : 1)], br.acc);
+        int len = e >> 8;
+        if (e == 0) {
+            bad = 1;
+            len = 17;
+        }
+        br.acc <<= len;
+        br.n -= len;
+        const int sym = e & 0xFF, s = sym & 15, r = sym >> 4;
+        const int v = extend(br.take(s), s);
+        if (k == 0) {  // DC: the difference category s (<= 15, checked on the host)
             ++n;
+            const int dv = sel3(slot, d0, d1, d2) + v;
+            d0 = slot == 0 ? dv : d0;
+            d1 = slot == 1 ? dv : d1;
+            d2 = slot == 2 ? dv : d2;
             if (WRITE) {
-                blk = cur < total ? block_at(g, im, coef, cur) : nullptr;
-                if (blk) blk[0] = (int16_t)diff;
+                ++cz.cur;
+                cz.blk = cz.addr(ln.umap, u);
+                if (cz.blk) cz.blk[0] = (int16_t)dv;
             }
             k = 1;
+        } else if (s) {
+            k += r;
+            if (WRITE && cz.blk) cz.blk[kZigzag[min(k, 79)]] = (int16_t)v;
+            ++k;
         } else {
-            const int rs = br.huff(tabs[g.ac[slot]], bad);
-            const int r = rs >> 4, sz = rs & 15;
-            if (sz) {
-                k += r;
-                const int v = extend(br.get(sz), sz);
-                if (WRITE && blk) blk[kZigzag[min(k, 79)]] = (int16_t)v;
-                ++k;
-            } else {
-                k = (r == 15) ? k + 16 : 64;  // ZRL or end of block
-            }
+            k = (r == 15) ? k + 16 : 64;  // ZRL or end of block
         }
         if (k >= 64) {
             k = 0;
-            u = (u + 1 == g.nbu) ? 0 : u + 1;
+            if (++u == ln.nbu) {
+                u = 0;
+                if (WRITE && ++cz.mcol == cz.mcux) {
+                    cz.mcol = 0;
+                    ++cz.mrow;
+                }
+            }
+            slot = (int)(ln.umap >> (6 * u)) & 3;
+            trel = sel3(slot, ln.tr0, ln.tr1, ln.tr2);
             // the segment's last block is complete: what follows is the
             // 1-bit padding before the marker, never decoded (by libjpeg
             // either), so it cannot raise the corrupt-data flag
-            if (WRITE && cur + 1 >= total) break;
+            if (WRITE && cz.cur + 1 >= cz.total) break;
         }
     }
-    *started = n;
-    return pack_state(br.pos(), u, k);
+    started = n;
+    return br.pos();
 }
 
-__device__ __forceinline__ int64_t chunk_start_bit(const JSeg &g, int ch) {
-    return (g.begin + (int64_t)(ch - g.chunk0) * kChunkBytes) * 8;
-}
-__device__ __forceinline__ int64_t chunk_stop_bit(const JSeg &g, int ch) {
-    return min(g.end, g.begin + (int64_t)(ch - g.chunk0 + 1) * kChunkBytes) * 8;
+__device__ __forceinline__ int64_t chunk_start_byte(const JSeg &g, int ch) {
+    return g.begin + (int64_t)(ch - g.chunk0) * kChunkBytes;
 }
 
-// Every chunk starts from its own first bit, at the start of an MCU (exact
-// for each segment's first chunk, a guess for the others).
-__global__ __launch_bounds__(256) void jpeg_chunk_init_kernel(const JSeg *__restrict__ segs,
-                                                              const int *__restrict__ chunk_seg, int nchunk,
-                                                              int64_t *__restrict__ entry, int *__restrict__ dirty) {
-    const int ch = blockIdx.x * 256 + threadIdx.x;
-    if (ch >= nchunk) return;
-    entry[ch] = pack_state(chunk_start_bit(segs[chunk_seg[ch]], ch), 0, 0);
-    dirty[ch] = 1;
+// Stage the working lanes' windows and the scan's tables (lane 0's, the
+// same for every lane of the workgroup).  Returns this lane's compact slot
+// (-1 when it does not work).
+__device__ int stage_chunks(ChunkLds &S, const uint8_t *__restrict__ data, const uint16_t *__restrict__ lut,
+                            bool work, int64_t start_byte, int64_t seg_end, int tab_off, int tab_len) {
+    const int t = threadIdx.x;
+    const uint64_t mask = __ballot(work);
+    const int nwork = __popcll(mask);
+    const int slot = work ? __popcll(mask & ((1ull << t) - 1)) : -1;
+    if (work) {
+        S.wbase[slot] = start_byte & ~(int64_t)3;
+        S.wend[slot] = seg_end;
+    }
+    const int toff = __shfl(tab_off, 0), twords = __shfl(tab_len, 0) / 2;
+    __syncthreads();
+    for (int L = t; L < nwork * kWinWords; L += kLanes) {
+        const int s = L / kWinWords, w = L - s * kWinWords;
+        const int64_t byte = S.wbase[s] + 4 * w, end = S.wend[s];
+        uint32_t v = 0;
+        if (byte < end) {
+            v = __builtin_bswap32(*reinterpret_cast<const uint32_t *>(data + byte));
+            if (byte + 4 > end) v &= ~0u << (8 * (int)(byte + 4 - end));
+        }
+        S.win[s * kWinWords + w] = v;
+    }
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(lut + toff);
+    for (int L = t; L < twords; L += kLanes) S.tab[L] = src[L];
+    __syncthreads();
+    return slot;
 }
 
-// Decode the chunks whose entry state changed; record exit states.
-__global__ __launch_bounds__(64) void jpeg_chunk_sync_kernel(const JSeg *__restrict__ segs,
-                                                             const int *__restrict__ chunk_seg, int nchunk,
-                                                             const JImage *__restrict__ imgs,
-                                                             const HuffTab *__restrict__ tabs,
-                                                             const uint8_t *__restrict__ data,
-                                                             const int64_t *__restrict__ entry,
-                                                             int64_t *__restrict__ exit_state,
-                                                             int *__restrict__ started, int *__restrict__ dirty) {
-    const int ch = blockIdx.x * 64 + threadIdx.x;
-    if (ch >= nchunk || !dirty[ch]) return;
-    const JSeg &g = segs[chunk_seg[ch]];
-    int bad = 0, n = 0;
-    exit_state[ch] = run_chunk<false>(g, imgs[g.image], tabs, data, entry[ch], chunk_stop_bit(g, ch), &n, 0,
-                                      nullptr, bad);
+// One linking round.  Round 0 decodes every chunk from its own first byte
+// at the start of an MCU (exact for a segment's first chunk, a guess for the
+// others).  Round r > 0 gives chunk t the exit state of chunk t-1 from round
+// r-1 and decodes it again if that differs from its entry; exits go to
+// exit_out (exit_in is the previous round's).  After round r the first r + 1
+// chunks of every segment are exact; changed[r] is set when any chunk was
+// decoded again, and a round after one that changed nothing returns at once.
+// Per chunk: blocks started (DC symbols) and per scan slot the sum of the DC
+// differences.  chunk_seg[ch] < 0: padding.
+__global__ __launch_bounds__(kLanes) void jpeg_chunk_round_kernel(
+    const JSeg *__restrict__ segs, const int *__restrict__ chunk_seg, int nchunk, const HuffTab *__restrict__ tabs,
+    const uint16_t *__restrict__ lut, const uint8_t *__restrict__ data, int round, int64_t *__restrict__ entry,
+    const int64_t *__restrict__ exit_in, int64_t *__restrict__ exit_out, int *__restrict__ started,
+    int *__restrict__ dcsum, int *__restrict__ changed) {
+    __shared__ ChunkLds S;
+    if (round > 0 && changed[round - 1] == 0) return;
+    const int ch = blockIdx.x * kLanes + threadIdx.x;
+    const int sg = ch < nchunk ? chunk_seg[ch] : -1;
+    const bool valid = sg >= 0;
+    const JSeg &g = segs[valid ? sg : chunk_seg[blockIdx.x * kLanes]];
+    const int64_t start = chunk_start_byte(g, valid ? ch : g.chunk0);
+    bool work = false;
+    int64_t e = 0;
+    if (valid) {
+        if (round == 0) {
+            e = pack_state(start * 8, 0, 0);
+            entry[ch] = e;
+            work = true;
+        } else if (ch == g.chunk0) {
+            exit_out[ch] = exit_in[ch];
+        } else {
+            e = exit_in[ch - 1];
+            if (e != entry[ch]) {
+                entry[ch] = e;
+                work = true;
+            } else {
+                exit_out[ch] = exit_in[ch];
+            }
+        }
+    }
+    if (__ballot(work) == 0) return;
+    const int slot = stage_chunks(S, data, lut, work, start, g.end, g.tab_off, g.tab_len);
+    if (!work) return;
+    const Lane ln = lane_of(g);
+    WinBits br;
+    const int64_t wbit = (start & ~(int64_t)3) * 8;
+    br.init(S.win + slot * kWinWords, (int)((e >> 16) - wbit));
+    const int stop = (int)(min(g.end, start + kChunkBytes) * 8 - wbit);
+    int u = (int)((e >> 8) & 0xFF), k = (int)(e & 0xFF), n = 0, d0 = 0, d1 = 0, d2 = 0, bad = 0;
+    Cursor cz;
+    const int p = decode_chunk<false>(br, stop, u, k, ln, reinterpret_cast<const uint16_t *>(S.tab), tabs, n, d0, d1,
+                                      d2, cz, bad);
+    exit_out[ch] = pack_state(wbit + p, u, k);
     started[ch] = n;
-    dirty[ch] = 0;
+    dcsum[3 * ch] = d0;
+    dcsum[3 * ch + 1] = d1;
+    dcsum[3 * ch + 2] = d2;
+    changed[round] = 1;
 }
 
-// entry[t] = exit[t-1] inside a segment; flags what changed.
-__global__ __launch_bounds__(256) void jpeg_chunk_link_kernel(const JSeg *__restrict__ segs,
-                                                              const int *__restrict__ chunk_seg, int nchunk,
-                                                              const int64_t *__restrict__ exit_state,
-                                                              int64_t *__restrict__ entry, int *__restrict__ dirty,
-                                                              int *__restrict__ changed) {
-    const int ch = blockIdx.x * 256 + threadIdx.x;
-    if (ch >= nchunk || ch == segs[chunk_seg[ch]].chunk0) return;
-    const int64_t e = exit_state[ch - 1];
-    if (e != entry[ch]) {
-        entry[ch] = e;
-        dirty[ch] = 1;
-        *changed = 1;
-    }
-}
-
-// First block of each chunk: prefix sum of the blocks started before it in
-// its segment.  One thread per segment.
-__global__ __launch_bounds__(64) void jpeg_chunk_scan_kernel(const JSeg *__restrict__ segs, int nseg,
-                                                             const int *__restrict__ started,
-                                                             int *__restrict__ first) {
-    const int si = blockIdx.x * 64 + threadIdx.x;
-    if (si >= nseg) return;
-    const JSeg &g = segs[si];
-    int run = 0;
-    for (int ch = g.chunk0; ch < g.chunk0 + g.nchunk; ++ch) {
-        first[ch] = run;
-        run += started[ch];
-    }
-}
-
-__global__ __launch_bounds__(64) void jpeg_chunk_write_kernel(const JSeg *__restrict__ segs,
-                                                              const int *__restrict__ chunk_seg, int nchunk,
-                                                              const JImage *__restrict__ imgs,
-                                                              const HuffTab *__restrict__ tabs,
-                                                              const uint8_t *__restrict__ data,
-                                                              const int64_t *__restrict__ entry,
-                                                              const int *__restrict__ first,
-                                                              int16_t *__restrict__ coef, int *__restrict__ bad_out) {
-    const int ch = blockIdx.x * 64 + threadIdx.x;
-    if (ch >= nchunk) return;
-    const JSeg &g = segs[chunk_seg[ch]];
-    int bad = 0, n = 0;
-    (void)run_chunk<true>(g, imgs[g.image], tabs, data, entry[ch], chunk_stop_bit(g, ch), &n, first[ch], coef, bad);
-    if (bad) bad_out[g.image] = 1;
-}
-
-// DC prediction: per (segment, scan slot), the component's blocks in decode
-// order hold differences; replace them by running sums (reset at each
-// segment, i.e. restart interval).  grid (segments, 4), 256 threads: each
-// thread sums a contiguous run, then a block-wide exclusive scan.
-__global__ __launch_bounds__(256) void jpeg_dc_scan_kernel(const JSeg *__restrict__ segs,
-                                                           const JImage *__restrict__ imgs,
-                                                           int16_t *__restrict__ coef) {
-    __shared__ int part[256];
+// Per segment (one workgroup each): exclusive prefix over its chunks of the
+// blocks started and the per-slot DC difference sums, i.e. each chunk's
+// first block and its DC predictors (prediction restarts at each segment).
+__global__ __launch_bounds__(256) void jpeg_chunk_scan_kernel(const JSeg *__restrict__ segs,
+                                                              const int *__restrict__ started,
+                                                              const int *__restrict__ dcsum, int *__restrict__ first,
+                                                              int *__restrict__ pred) {
+    __shared__ int4 part[256];
     const JSeg &g = segs[blockIdx.x];
-    const int slot = blockIdx.y;
-    if (slot >= g.ns) return;
+    const int t = threadIdx.x, span = (g.nchunk + 255) / 256;
+    const int c0 = g.chunk0 + min(g.nchunk, t * span), c1 = g.chunk0 + min(g.nchunk, (t + 1) * span);
+    int4 s = make_int4(0, 0, 0, 0);
+    for (int ch = c0; ch < c1; ++ch) {
+        s.x += started[ch];
+        s.y += dcsum[3 * ch];
+        s.z += dcsum[3 * ch + 1];
+        s.w += dcsum[3 * ch + 2];
+    }
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {  // inclusive Hillis-Steele scan
+        const int4 v = t >= off ? part[t - off] : make_int4(0, 0, 0, 0);
+        __syncthreads();
+        part[t] = make_int4(part[t].x + v.x, part[t].y + v.y, part[t].z + v.z, part[t].w + v.w);
+        __syncthreads();
+    }
+    int4 run = t > 0 ? part[t - 1] : make_int4(0, 0, 0, 0);
+    for (int ch = c0; ch < c1; ++ch) {
+        first[ch] = run.x;
+        pred[3 * ch] = run.y;
+        pred[3 * ch + 1] = run.z;
+        pred[3 * ch + 2] = run.w;
+        run.x += started[ch];
+        run.y += dcsum[3 * ch];
+        run.z += dcsum[3 * ch + 1];
+        run.w += dcsum[3 * ch + 2];
+    }
+}
+
+// Every chunk from its exact entry state: coefficients into the zeroed
+// coefficient blocks, DC as the predicted value.
+__global__ __launch_bounds__(kLanes) void jpeg_chunk_write_kernel(
+    const JSeg *__restrict__ segs, const int *__restrict__ chunk_seg, int nchunk, const JImage *__restrict__ imgs,
+    const HuffTab *__restrict__ tabs, const uint16_t *__restrict__ lut, const uint8_t *__restrict__ data,
+    const int64_t *__restrict__ entry, const int *__restrict__ first, const int *__restrict__ pred,
+    int16_t *__restrict__ coef, int *__restrict__ bad_out) {
+    __shared__ ChunkLds S;
+    const int ch = blockIdx.x * kLanes + threadIdx.x;
+    const int sg = ch < nchunk ? chunk_seg[ch] : -1;
+    const bool valid = sg >= 0;
+    const JSeg &g = segs[valid ? sg : chunk_seg[blockIdx.x * kLanes]];
+    const int64_t start = chunk_start_byte(g, valid ? ch : g.chunk0);
+    const int slot = stage_chunks(S, data, lut, valid, start, g.end, g.tab_off, g.tab_len);
+    if (!valid) return;
+    const Lane ln = lane_of(g);
+    const int64_t e = entry[ch];
+    WinBits br;
+    const int64_t wbit = (start & ~(int64_t)3) * 8;
+    br.init(S.win + slot * kWinWords, (int)((e >> 16) - wbit));
+    const int stop = (int)(min(g.end, start + kChunkBytes) * 8 - wbit);
+    int u = (int)((e >> 8) & 0xFF), k = (int)(e & 0xFF), n = 0, bad = 0;
+    int d0 = pred[3 * ch], d1 = pred[3 * ch + 1], d2 = pred[3 * ch + 2];
     const JImage &im = imgs[g.image];
-    const JComp &c = im.c[g.comp[slot]];
-    const int per_mcu = g.ns == 1 ? 1 : c.v * c.h;
-    const int count = g.mcus * per_mcu;
-    const int span = (count + 255) / 256;
-    const int i0 = min(count, (int)threadIdx.x * span), i1 = min(count, i0 + span);
-    auto at = [&](int i) -> int16_t * {
-        if (g.ns == 1) {
-            const int q = g.mcu0 + i;
-            return coef + (c.coef + (int64_t)(q / g.mcux) * c.bw + q % g.mcux) * 64;
-        }
-        const int m = g.mcu0 + i / per_mcu, r = i % per_mcu;
-        return coef + (c.coef + (int64_t)((m / g.mcux) * c.v + r / c.h) * c.bw + (m % g.mcux) * c.h + r % c.h) * 64;
-    };
-    int sum = 0;
-    for (int i = i0; i < i1; ++i) sum += at(i)[0];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int t = 0; t < 256; ++t) {
-            const int v = part[t];
-            part[t] = run;
-            run += v;
-        }
+    Cursor cz;
+    cz.coef = coef;
+    const bool single = g.ns == 1;
+    {
+        const JComp &a = im.c[g.comp[0]];
+        const JComp &b = im.c[g.comp[g.ns > 1 ? 1 : 0]];
+        const JComp &c = im.c[g.comp[g.ns > 2 ? 2 : 0]];
+        cz.cb0 = a.coef;
+        cz.cb1 = b.coef;
+        cz.cb2 = c.coef;
+        cz.bw0 = a.bw;
+        cz.bw1 = b.bw;
+        cz.bw2 = c.bw;
+        cz.h0 = single ? 1 : a.h;
+        cz.h1 = b.h;
+        cz.h2 = c.h;
+        cz.v0 = single ? 1 : a.v;
+        cz.v1 = b.v;
+        cz.v2 = c.v;
     }
-    __syncthreads();
-    int run = part[threadIdx.x];
-    for (int i = i0; i < i1; ++i) {
-        int16_t *b = at(i);
-        run += b[0];
-        b[0] = (int16_t)run;
-    }
+    const int f = first[ch];
+    cz.total = g.mcus * g.nbu;
+    cz.cur = f - 1;  // the block in progress (k > 0) or the last one completed
+    const int bpos = max(0, k > 0 ? f - 1 : f);
+    const int m = g.mcu0 + bpos / g.nbu;
+    cz.mcux = g.mcux;
+    cz.mrow = m / g.mcux;
+    cz.mcol = m - cz.mrow * g.mcux;
+    cz.blk = k > 0 ? cz.addr(ln.umap, u) : nullptr;
+    (void)decode_chunk<true>(br, stop, u, k, ln, reinterpret_cast<const uint16_t *>(S.tab), tabs, n, d0, d1, d2, cz,
+                             bad);
+    if (bad) bad_out[g.image] = 1;
 }
 
 // ------------------------------------------------------------------- IDCT
@@ -484,24 +618,108 @@ __device__ __forceinline__ int sample_at(const JComp &c, const uint8_t *__restri
 
 __device__ __forceinline__ uint8_t clamp255(int v) { return (uint8_t)min(255, max(0, v)); }
 
-// grid (ceil(max W / 256), max H, images)
+// Chroma of four output pixels x0..x0+3 (x0 a multiple of 4) of row y from
+// a plane upsampled by the triangle filter, 2x1 (VF 1) or 2x2 (VF 2): the
+// four samples around columns x0/2 and x0/2 + 1, clamped to the component's
+// width, which is libjpeg's edge rule.
+template <int VF>
+__device__ __forceinline__ void chroma_fancy4(const JComp &c, const uint8_t *__restrict__ planes, int x0, int y,
+                                              int out[4]) {
+    const uint8_t *P = planes + c.plane;
+    const int cx = x0 >> 1;
+    int col[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) col[j] = min(max(cx - 1 + j, 0), c.dw - 1);
+    if (VF == 1) {
+        const uint8_t *row = P + (int64_t)y * c.pitch;
+        int p[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[j] = row[col[j]];
+        out[0] = (3 * p[1] + p[0] + 1) >> 2;
+        out[1] = (3 * p[1] + p[2] + 2) >> 2;
+        out[2] = (3 * p[2] + p[1] + 1) >> 2;
+        out[3] = (3 * p[2] + p[3] + 2) >> 2;
+    } else {
+        const int cy = y >> 1;
+        const int r2 = (y & 1) == 0 ? max(cy - 1, 0) : min(cy + 1, c.dh - 1);
+        const uint8_t *row = P + (int64_t)cy * c.pitch, *row2 = P + (int64_t)r2 * c.pitch;
+        int cs[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cs[j] = 3 * row[col[j]] + row2[col[j]];
+        out[0] = (3 * cs[1] + cs[0] + 8) >> 4;
+        out[1] = (3 * cs[1] + cs[2] + 7) >> 4;
+        out[2] = (3 * cs[2] + cs[1] + 8) >> 4;
+        out[3] = (3 * cs[2] + cs[3] + 7) >> 4;
+    }
+}
+
+// Four output pixels per thread (12 bytes, three dword stores when aligned).
+// Full-resolution luma with both chroma planes at 1x1, or both triangle-
+// filtered 2x1 or 2x2 (the common files), take the direct path; every other
+// combination goes through sample_at.
+// grid (ceil(max W / 1024), max H, images)
 __global__ __launch_bounds__(256) void jpeg_color_kernel(const JImage *__restrict__ imgs,
                                                          const uint8_t *__restrict__ planes,
                                                          uint8_t *__restrict__ rgb) {
     const JImage &im = imgs[blockIdx.z];
-    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
-    if (x >= im.W || y >= im.H) return;
-    const int Y = sample_at(im.c[0], planes, x, y);
-    const int cb = sample_at(im.c[1], planes, x, y) - 128;
-    const int cr = sample_at(im.c[2], planes, x, y) - 128;
-    // libjpeg's tables: FIX(x) = x * 2^16 rounded; R and B rounded by 2^15
-    const int r = Y + ((91881 * cr + 32768) >> 16);
-    const int g = Y + ((-22554 * cb + 32768 - 46802 * cr) >> 16);
-    const int b = Y + ((116130 * cb + 32768) >> 16);
-    uint8_t *o = rgb + im.rgb + ((int64_t)y * im.W + x) * 3;
-    o[0] = clamp255(r);
-    o[1] = clamp255(g);
-    o[2] = clamp255(b);
+    const int x0 = (blockIdx.x * 256 + threadIdx.x) * 4, y = blockIdx.y;
+    if (x0 >= im.W || y >= im.H) return;
+    const int np = min(4, im.W - x0);
+    const JComp &c0 = im.c[0], &c1 = im.c[1], &c2 = im.c[2];
+    int Y[4], Cb[4], Cr[4];
+    const bool full_y = c0.hf == 1 && c0.vf == 1;
+    const bool same_c = c1.hf == c2.hf && c1.vf == c2.vf && c1.dw == c2.dw && c1.dh == c2.dh;
+    const bool fancy = same_c && c1.hf == 2 && c1.vf <= 2 && c1.dw > 2;
+    if (full_y && (fancy || (same_c && c1.hf == 1 && c1.vf == 1))) {
+        const uint32_t yw = *reinterpret_cast<const uint32_t *>(planes + c0.plane + (int64_t)y * c0.pitch + x0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Y[i] = (int)((yw >> (8 * i)) & 0xFF);
+        if (!fancy) {
+            const uint32_t bw = *reinterpret_cast<const uint32_t *>(planes + c1.plane + (int64_t)y * c1.pitch + x0);
+            const uint32_t rw = *reinterpret_cast<const uint32_t *>(planes + c2.plane + (int64_t)y * c2.pitch + x0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                Cb[i] = (int)((bw >> (8 * i)) & 0xFF);
+                Cr[i] = (int)((rw >> (8 * i)) & 0xFF);
+            }
+        } else if (c1.vf == 1) {
+            chroma_fancy4<1>(c1, planes, x0, y, Cb);
+            chroma_fancy4<1>(c2, planes, x0, y, Cr);
+        } else {
+            chroma_fancy4<2>(c1, planes, x0, y, Cb);
+            chroma_fancy4<2>(c2, planes, x0, y, Cr);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = min(x0 + i, im.W - 1);
+            Y[i] = sample_at(c0, planes, x, y);
+            Cb[i] = sample_at(c1, planes, x, y);
+            Cr[i] = sample_at(c2, planes, x, y);
+        }
+    }
+    uint32_t px[3] = {0, 0, 0};  // 12 bytes, little-endian
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int cb = Cb[i] - 128, cr = Cr[i] - 128;
+        // libjpeg's tables: FIX(x) = x * 2^16 rounded; R and B rounded by 2^15
+        const uint32_t r = clamp255(Y[i] + ((91881 * cr + 32768) >> 16));
+        const uint32_t g = clamp255(Y[i] + ((-22554 * cb + 32768 - 46802 * cr) >> 16));
+        const uint32_t b = clamp255(Y[i] + ((116130 * cb + 32768) >> 16));
+        const int o = 3 * i;  // byte offsets o, o + 1, o + 2 of the 12
+        px[o >> 2] |= r << (8 * (o & 3));
+        px[(o + 1) >> 2] |= g << (8 * ((o + 1) & 3));
+        px[(o + 2) >> 2] |= b << (8 * ((o + 2) & 3));
+    }
+    uint8_t *dst = rgb + im.rgb + ((int64_t)y * im.W + x0) * 3;
+    if (np == 4 && (reinterpret_cast<uintptr_t>(dst) & 3) == 0) {
+        uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+        d[0] = px[0];
+        d[1] = px[1];
+        d[2] = px[2];
+    } else {
+        for (int j = 0; j < 3 * np; ++j) dst[j] = (uint8_t)(px[j >> 2] >> (8 * (j & 3)));
+    }
 }
 
 // ------------------------------------------------------------------- host
@@ -514,7 +732,9 @@ struct RawHuff {
 struct ScanInfo {
     int ns = 0;
     int comp[4] = {0, 0, 0, 0};
+    int td[4] = {0, 0, 0, 0}, ta[4] = {0, 0, 0, 0};  // DC / AC table ids per slot
     HuffTab dc[4], ac[4];
+    RawHuff rdc[4], rac[4];
     int64_t begin = 0, end = 0;                        // entropy-coded bytes in the file
     std::vector<std::pair<int64_t, int64_t>> segs;     // restart segments [begin, end), stuffed
     std::vector<int64_t> seg_len;                      // their unstuffed lengths
@@ -579,6 +799,56 @@ bool build_huff(const RawHuff &r, HuffTab &t) {
     }
     t.maxcode[17] = INT_MAX;
     return true;
+}
+
+// Two-level decoding table of one Huffman table, appended to `out` (uint16
+// entries): kLook2 first-level entries indexed by the next 9 bits, then the
+// second-level subtables.  An entry is (length << 8) | symbol for a code of
+// at most 9 bits (first level) or of any length (second level), 0 for no
+// code, or, in the first level, 0x8000 | (nb << 12) | off for the 9-bit
+// prefix of longer codes: their entry is at second-level index off + the nb
+// bits after the prefix (nb = longest such code - 9).  off == 0xFFF: the
+// subtable did not fit in `budget` entries; decoded by the canonical walk.
+// The table has passed build_huff (no over-subscribed length).
+void build_lut2(const RawHuff &r, std::vector<uint16_t> &out, int budget) {
+    std::vector<uint16_t> L(512, 0), sub;
+    int maxlen[512] = {0};
+    struct LongCode {
+        int l, code, sym;
+    };
+    std::vector<LongCode> longc;
+    int code = 0, k = 0;
+    for (int l = 1; l <= 16; ++l) {
+        for (int i = 0; i < r.counts[l - 1]; ++i, ++code, ++k) {
+            if (l <= 9) {
+                for (int j = 0; j < (1 << (9 - l)); ++j) L[(code << (9 - l)) + j] = (uint16_t)((l << 8) | r.vals[k]);
+            } else {
+                const int p = code >> (l - 9);
+                maxlen[p] = std::max(maxlen[p], l);
+                longc.push_back({l, code, r.vals[k]});
+            }
+        }
+        code <<= 1;
+    }
+    for (int p = 0; p < 512; ++p) {
+        if (!maxlen[p]) continue;
+        const int nb = maxlen[p] - 9, size = 1 << nb;
+        if ((int)sub.size() + size <= budget && (int)sub.size() < 0xFFF) {
+            L[p] = (uint16_t)(0x8000 | (nb << 12) | (int)sub.size());
+            sub.resize(sub.size() + size, 0);
+        } else {
+            L[p] = (uint16_t)(0x8000 | 0xFFF);
+        }
+    }
+    for (const LongCode &c : longc) {
+        const int p = c.code >> (c.l - 9);
+        if ((L[p] & 0xFFF) == 0xFFF) continue;
+        const int nb = (L[p] >> 12) & 7, off = L[p] & 0xFFF, extra = c.l - 9;
+        const int rest = c.code & ((1 << extra) - 1), span = 1 << (nb - extra);
+        for (int j = 0; j < span; ++j) sub[off + (rest << (nb - extra)) + j] = (uint16_t)((c.l << 8) | c.sym);
+    }
+    out.insert(out.end(), L.begin(), L.end());
+    out.insert(out.end(), sub.begin(), sub.end());
 }
 
 int fail(FileInfo &f, int status, const std::string &msg) {
@@ -674,8 +944,19 @@ int parse_file(const uint8_t *d, int64_t n, FileInfo &f) {
                 const int td = s[2 + 2 * k] >> 4, ta = s[2 + 2 * k] & 15;
                 if (ci < 0 || td > 3 || ta > 3 || !dc[td].set || !ac[ta].set)
                     return fail(f, PRNU_EINVAL, "scan references an unknown component or table");
+                for (int j = 0; j < k; ++j)
+                    if (sc.comp[j] == ci) return fail(f, PRNU_EINVAL, "a component appears twice in a scan");
                 if (!build_huff(dc[td], sc.dc[k]) || !build_huff(ac[ta], sc.ac[k]))
                     return fail(f, PRNU_EINVAL, "invalid Huffman table");
+                // DC symbols are difference categories (libjpeg rejects > 15)
+                int ndc = 0;
+                for (int l = 0; l < 16; ++l) ndc += dc[td].counts[l];
+                for (int j = 0; j < ndc; ++j)
+                    if (dc[td].vals[j] > 15) return fail(f, PRNU_EINVAL, "invalid Huffman table");
+                sc.td[k] = td;
+                sc.ta[k] = ta;
+                sc.rdc[k] = dc[td];
+                sc.rac[k] = ac[ta];
                 sc.comp[k] = ci;
                 if (!f.latched[ci]) {
                     if (!have_q[f.tq[ci]]) return fail(f, PRNU_EINVAL, "missing quantization table");
@@ -804,6 +1085,7 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
     std::vector<JSeg> segs;
     std::vector<int> chunk_seg;
     std::vector<HuffTab> tabs;
+    std::vector<uint16_t> lut;  // two-level tables, scan by scan
     std::vector<uint16_t> qtab;
     std::vector<int64_t> file_data((size_t)count, -1);  // first unstuffed byte of each accepted file
     int64_t data_bytes = 0, coef_blocks = 0, plane_bytes = 0;
@@ -852,7 +1134,8 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
         max_w = std::max(max_w, f.W);
         max_h = std::max(max_h, f.H);
         // segments, their chunks and their place in the unstuffed data
-        const size_t seg_first = segs.size(), chunk_first = chunk_seg.size();
+        const size_t seg_first = segs.size(), chunk_first = chunk_seg.size(), tab_first = tabs.size(),
+                     lut_first = lut.size();
         const int64_t data_first = data_bytes;
         for (const ScanInfo &sc : f.scans) {
             JSeg base;
@@ -895,6 +1178,32 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
                 base.ac[q] = (int)tabs.size();
                 tabs.push_back(sc.ac[q]);
             }
+            // the scan's distinct tables, two-level, within kScanTabWords
+            {
+                if (lut.size() & 1) lut.push_back(0);  // blocks start and end on 4-byte boundaries
+                base.tab_off = (int)lut.size();
+                int key[8], rel[8], nd = 0;
+                for (int q = 0; q < sc.ns; ++q) {
+                    int pr[2];
+                    for (int cls = 0; cls < 2; ++cls) {
+                        const int id = cls * 4 + (cls ? sc.ta[q] : sc.td[q]);
+                        int j = 0;
+                        while (j < nd && key[j] != id) ++j;
+                        if (j == nd) {
+                            key[nd] = id;
+                            rel[nd] = (int)lut.size() - base.tab_off;
+                            ++nd;
+                            // level-2 room: what is left after the first levels still to come
+                            const int left = kScanTabWords - rel[j] - 512 * (2 * sc.ns - (2 * q + cls));
+                            build_lut2(cls ? sc.rac[q] : sc.rdc[q], lut, std::max(0, left));
+                        }
+                        pr[cls] = rel[j];
+                    }
+                    base.trel[q] = pr[0] | (pr[1] << 16);
+                }
+                if (lut.size() & 1) lut.push_back(0);
+                base.tab_len = (int)lut.size() - base.tab_off;
+            }
             const int ri = f.ri > 0 ? f.ri : total;
             const int nseg = (total + ri - 1) / ri;
             if ((int)sc.segs.size() != nseg) {
@@ -915,10 +1224,14 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
                 for (int t = 0; t < g.nchunk; ++t) chunk_seg.push_back((int)segs.size());
                 segs.push_back(g);
             }
+            // a workgroup decodes chunks of one scan only (its tables in LDS)
+            while (chunk_seg.size() % kLanes) chunk_seg.push_back(-1);
         }
         if (f.status != PRNU_OK) {
             segs.resize(seg_first);
             chunk_seg.resize(chunk_first);
+            tabs.resize(tab_first);
+            lut.resize(lut_first);
             data_bytes = data_first;
             im.W = im.H = 0;
             continue;
@@ -930,26 +1243,28 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
     const size_t o_tab = o_seg + align256(sizeof(JSeg) * std::max(1, nseg));
     const size_t o_q = o_tab + align256(sizeof(HuffTab) * std::max<size_t>(1, tabs.size()));
     const size_t o_cseg = o_q + align256(sizeof(uint16_t) * std::max<size_t>(64, qtab.size()));
-    const size_t o_data = o_cseg + align256(sizeof(int) * std::max(1, nchunk));
+    const size_t o_lut = o_cseg + align256(sizeof(int) * std::max(1, nchunk));
+    const size_t o_data = o_lut + align256(sizeof(uint16_t) * std::max<size_t>(2, lut.size()));
     const size_t staged = o_data + align256((size_t)data_bytes + 8);
-    // device only: chunk states, coefficients, planes
-    const size_t o_entry = staged;
-    const size_t o_exit = o_entry + align256(sizeof(int64_t) * std::max(1, nchunk));
-    const size_t o_started = o_exit + align256(sizeof(int64_t) * std::max(1, nchunk));
-    const size_t o_first = o_started + align256(sizeof(int) * std::max(1, nchunk));
-    const size_t o_dirty = o_first + align256(sizeof(int) * std::max(1, nchunk));
-    const size_t o_coef = o_dirty + align256(sizeof(int) * std::max(1, nchunk));
+    // device only: chunk states, per-chunk sums and prefixes, round flags,
+    // coefficients, planes
+    const size_t n8 = align256(sizeof(int64_t) * std::max(1, nchunk)), n4 = align256(sizeof(int) * std::max(1, nchunk));
+    const size_t o_entry = staged, o_exit0 = o_entry + n8, o_exit1 = o_exit0 + n8;
+    const size_t o_started = o_exit1 + n8, o_dcsum = o_started + n4, o_first = o_dcsum + 3 * n4;
+    const size_t o_pred = o_first + n4, o_changed = o_pred + 3 * n4;
+    const size_t o_coef = o_changed + align256(sizeof(int) * ((size_t)max_chunks + 2));
     const size_t o_plane = o_coef + align256((size_t)coef_blocks * 128);
     const size_t total = o_plane + align256((size_t)plane_bytes);
     PRNU_TRY(dec->staging.ensure(staged));
     PRNU_TRY(dec->dev.ensure(total));
-    PRNU_TRY(dec->bad.ensure(sizeof(int) * ((size_t)count + 1)));
+    PRNU_TRY(dec->bad.ensure(sizeof(int) * (size_t)count));
     uint8_t *hs = static_cast<uint8_t *>(dec->staging.p);
     std::memcpy(hs + o_img, imgs.data(), sizeof(JImage) * count);
     if (nseg) std::memcpy(hs + o_seg, segs.data(), sizeof(JSeg) * nseg);
     if (!tabs.empty()) std::memcpy(hs + o_tab, tabs.data(), sizeof(HuffTab) * tabs.size());
     if (!qtab.empty()) std::memcpy(hs + o_q, qtab.data(), sizeof(uint16_t) * qtab.size());
     if (nchunk) std::memcpy(hs + o_cseg, chunk_seg.data(), sizeof(int) * nchunk);
+    if (!lut.empty()) std::memcpy(hs + o_lut, lut.data(), sizeof(uint16_t) * lut.size());
     {
         // unstuffed entropy-coded data, file by file in parallel
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -970,53 +1285,55 @@ int decode_batch(JpegDecoder *dec, const uint8_t *const *files, const int64_t *s
     }
     uint8_t *dv = static_cast<uint8_t *>(dec->dev.ptr);
     int *d_bad = static_cast<int *>(dec->bad.ptr);
-    int *d_changed = d_bad + count;
     PRNU_CHECK_HIP(hipMemcpyAsync(dv, hs, staged, hipMemcpyHostToDevice, st));
     PRNU_CHECK_HIP(hipMemsetAsync(d_bad, 0, sizeof(int) * (size_t)count, st));
     const JImage *d_img = reinterpret_cast<const JImage *>(dv + o_img);
     const JSeg *d_seg = reinterpret_cast<const JSeg *>(dv + o_seg);
     const HuffTab *d_tab = reinterpret_cast<const HuffTab *>(dv + o_tab);
     const int *d_cseg = reinterpret_cast<const int *>(dv + o_cseg);
+    const uint16_t *d_lut = reinterpret_cast<const uint16_t *>(dv + o_lut);
     const uint8_t *d_data = dv + o_data;
     int64_t *d_entry = reinterpret_cast<int64_t *>(dv + o_entry);
-    int64_t *d_exit = reinterpret_cast<int64_t *>(dv + o_exit);
+    int64_t *d_exit[2] = {reinterpret_cast<int64_t *>(dv + o_exit0), reinterpret_cast<int64_t *>(dv + o_exit1)};
     int *d_started = reinterpret_cast<int *>(dv + o_started);
+    int *d_dcsum = reinterpret_cast<int *>(dv + o_dcsum);
     int *d_first = reinterpret_cast<int *>(dv + o_first);
-    int *d_dirty = reinterpret_cast<int *>(dv + o_dirty);
+    int *d_pred = reinterpret_cast<int *>(dv + o_pred);
+    int *d_changed = reinterpret_cast<int *>(dv + o_changed);
     int16_t *d_coef = reinterpret_cast<int16_t *>(dv + o_coef);
     uint8_t *d_plane = dv + o_plane;
     if (nseg) {
         PRNU_CHECK_HIP(hipMemsetAsync(d_coef, 0, (size_t)coef_blocks * 128, st));
-        const dim3 g256((unsigned)((nchunk + 255) / 256)), g64((unsigned)((nchunk + 63) / 64));
-        hipLaunchKernelGGL(jpeg_chunk_init_kernel, g256, dim3(256), 0, st, d_seg, d_cseg, nchunk, d_entry, d_dirty);
-        PRNU_CHECK_LAUNCH();
-        // entry states settle: after round r the first r + 1 chunks of every
-        // segment are exact, so max_chunks + 1 rounds always suffice
-        for (int round = 0; round <= max_chunks; ++round) {
-            hipLaunchKernelGGL(jpeg_chunk_sync_kernel, g64, dim3(64), 0, st, d_seg, d_cseg, nchunk, d_img, d_tab,
-                               d_data, d_entry, d_exit, d_started, d_dirty);
-            PRNU_CHECK_LAUNCH();
-            PRNU_CHECK_HIP(hipMemsetAsync(d_changed, 0, sizeof(int), st));
-            hipLaunchKernelGGL(jpeg_chunk_link_kernel, g256, dim3(256), 0, st, d_seg, d_cseg, nchunk, d_exit,
-                               d_entry, d_dirty, d_changed);
-            PRNU_CHECK_LAUNCH();
+        PRNU_CHECK_HIP(hipMemsetAsync(d_changed, 0, sizeof(int) * ((size_t)max_chunks + 2), st));
+        const dim3 gch((unsigned)((nchunk + kLanes - 1) / kLanes));
+        // linking rounds until one changes nothing; after round r the first
+        // r + 1 chunks of every segment are exact, so rounds 0..max_chunks
+        // always suffice.  The host looks at the flag once per group of
+        // rounds; a round after the fixed point returns at once.
+        constexpr int kRoundsPerCheck = 4;
+        for (int r = 0; r <= max_chunks;) {
+            const int r_end = std::min(max_chunks + 1, r + kRoundsPerCheck);
+            for (; r < r_end; ++r) {
+                hipLaunchKernelGGL(jpeg_chunk_round_kernel, gch, dim3(kLanes), 0, st, d_seg, d_cseg, nchunk, d_tab,
+                                   d_lut, d_data, r, d_entry, d_exit[(r + 1) & 1], d_exit[r & 1], d_started, d_dcsum,
+                                   d_changed);
+                PRNU_CHECK_LAUNCH();
+            }
             int changed = 0;
-            PRNU_CHECK_HIP(hipMemcpyAsync(&changed, d_changed, sizeof(int), hipMemcpyDeviceToHost, st));
+            PRNU_CHECK_HIP(hipMemcpyAsync(&changed, d_changed + (r - 1), sizeof(int), hipMemcpyDeviceToHost, st));
             PRNU_CHECK_HIP(hipStreamSynchronize(st));
             if (!changed) break;
         }
-        hipLaunchKernelGGL(jpeg_chunk_scan_kernel, dim3((unsigned)((nseg + 63) / 64)), dim3(64), 0, st, d_seg, nseg,
-                           d_started, d_first);
+        hipLaunchKernelGGL(jpeg_chunk_scan_kernel, dim3((unsigned)nseg), dim3(256), 0, st, d_seg, d_started, d_dcsum,
+                           d_first, d_pred);
         PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(jpeg_chunk_write_kernel, g64, dim3(64), 0, st, d_seg, d_cseg, nchunk, d_img, d_tab, d_data,
-                           d_entry, d_first, d_coef, d_bad);
-        PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(jpeg_dc_scan_kernel, dim3((unsigned)nseg, 4), dim3(256), 0, st, d_seg, d_img, d_coef);
+        hipLaunchKernelGGL(jpeg_chunk_write_kernel, gch, dim3(kLanes), 0, st, d_seg, d_cseg, nchunk, d_img, d_tab,
+                           d_lut, d_data, d_entry, d_first, d_pred, d_coef, d_bad);
         PRNU_CHECK_LAUNCH();
         hipLaunchKernelGGL(jpeg_idct_kernel, dim3((unsigned)((max_blocks + 255) / 256), 3, (unsigned)count),
                            dim3(256), 0, st, d_img, reinterpret_cast<const uint16_t *>(dv + o_q), d_coef, d_plane);
         PRNU_CHECK_LAUNCH();
-        hipLaunchKernelGGL(jpeg_color_kernel, dim3((unsigned)((max_w + 255) / 256), (unsigned)max_h, (unsigned)count),
+        hipLaunchKernelGGL(jpeg_color_kernel, dim3((unsigned)((max_w + 1023) / 1024), (unsigned)max_h, (unsigned)count),
                            dim3(256), 0, st, d_img, d_plane, d_rgb);
         PRNU_CHECK_LAUNCH();
     }

@@ -56,8 +56,8 @@ class JpegDecoder:
             out = torch.empty((n, hh, ww, 3), dtype=torch.uint8, device="cuda")
         if not out.is_cuda or out.dtype != torch.uint8 or not out.is_contiguous():
             raise ValueError("out must be a contiguous uint8 CUDA tensor")
-        bufs = [ctypes.create_string_buffer(f, len(f)) for f in files]
-        ptrs = (ctypes.c_void_p * n)(*[ctypes.addressof(b) for b in bufs])
+        # the files' own buffers (read only; `files` keeps them alive)
+        ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(ctypes.c_char_p(f), ctypes.c_void_p).value for f in files])
         lens = (ctypes.c_int64 * n)(*[len(f) for f in files])
         ws, hs, st = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
         stride = out.numel() // n
