@@ -403,6 +403,15 @@ PRNU_EXPORT int prnu_jpeg_info(const void *h_file, int64_t size, int32_t *width,
 PRNU_EXPORT int prnu_jpeg_decode_batch(prnu_jpeg_decoder *dec, const void *const *h_files, const int64_t *h_sizes,
                                        int count, uint8_t *d_rgb, int64_t rgb_stride, int64_t rgb_capacity,
                                        int32_t *h_width, int32_t *h_height, int32_t *h_status, prnu_stream stream);
+/* readJPEG itself (readjpeg/src/main/cpp/native.cpp:59-134, bound by JNA in
+ * CSI/ReadJPEG.java:12-23): the same name, arguments and result, so the
+ * binding can load this library in place of libreadjpeg.  One file from host
+ * memory into the host buffer `output` (packed RGB8 rows); returns 0, or 1
+ * with the reason on stderr (not 3-component YCbCr, unsupported kind,
+ * malformed, or output_capacity < 3 * width * height).  Decoded on the
+ * current device; calls are serialised. */
+PRNU_EXPORT int readJPEG(const char *input, int32_t input_size, int32_t *output_width, int32_t *output_height,
+                         char *output, int32_t output_capacity);
 
 /* --------------------------------------------------------------- timing
  * No reference counterpart (the reference times whole tasks on the host,

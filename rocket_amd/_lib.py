@@ -114,6 +114,8 @@ _SIGNATURES = {
     "prnu_jpeg_info": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "prnu_jpeg_decode_batch": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64, _c_i64,
                                         _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "readJPEG": (_c_int, [_c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                          _c_void_p, ctypes.c_int32]),
     "prnu_ctx_set_timing": (_c_int, [_c_void_p, _c_int]),
     "prnu_ctx_last_kernel_ms": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "prnu_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,
@@ -126,7 +128,7 @@ _lib = None
 def header_symbols() -> list[str]:
     """Every function name include/prnu_api.h declares."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"PRNU_EXPORT\s+[\w\s\*]*?\b(prnu_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"PRNU_EXPORT\s+[\w\s\*]*?\b(prnu_\w+|readJPEG)\s*\(", text)))
 
 
 def load() -> ctypes.CDLL:

@@ -7,13 +7,35 @@ scores fill a symmetric matrix with a zero diagonal, and the linkage,
 clustering, edge list and matrix files are written.  The reference spreads
 the pairs over Rocket's distributed scheduler; here extraction runs in
 batches, NCC as one triangular Gram over a device-resident pattern store and
-PCE as batched pair lists.
+PCE as batched pair lists.  ``common_source_identification_dir`` takes a
+folder of JPEG files as ``Main`` does; they are decoded on the GPU (libjpeg's
+pixels, bit for bit) into the buffer the extraction reads.
 """
 from __future__ import annotations
+
+import os
+import re
 
 import numpy as np
 
 from . import _lib, csi, csi_io
+
+
+def _compare(pats, n: int, h: int, w: int, filt, pce: bool, precision: str, pce_variant: int):
+    """All-pairs scores of n device-resident patterns (or spectra)."""
+    if pce:
+        cmp = csi.PeakToCorrelationEnergy(h, w, pce_variant, _ctx=filt._ctx)
+        pairs = np.array([(i, j) for i in range(n) for j in range(i + 1, n)], dtype=np.int32)
+        scores = cmp.pairs(pats, pairs).cpu().numpy() if len(pairs) else np.zeros(0)
+        return csi_io.correlation_matrix(n, pairs, scores)
+    gram = csi.GramNCC(h, w)
+    store = csi.PatternStore(h * w, n, precision)
+    store.put(pats)
+    corrs = store.gram(gram, n).cpu().numpy()
+    np.fill_diagonal(corrs, 0.0)               # Main.java:121-126 never sets the diagonal
+    store.close()
+    gram.close()
+    return corrs
 
 
 def correlation_matrix_gpu(images, strategy: str = "NCC", precision: str = "split_bf16",
@@ -30,19 +52,7 @@ def correlation_matrix_gpu(images, strategy: str = "NCC", precision: str = "spli
     for b in range(0, n, batch):
         chunk = imgs[b:b + batch].reshape(min(batch, n - b), -1).cuda()
         filt.apply_gpu_batch(chunk, pats[b:b + batch], fmt)
-    if pce:
-        cmp = csi.PeakToCorrelationEnergy(h, w, pce_variant, _ctx=filt._ctx)
-        pairs = np.array([(i, j) for i in range(n) for j in range(i + 1, n)], dtype=np.int32)
-        scores = cmp.pairs(pats, pairs).cpu().numpy() if len(pairs) else np.zeros(0)
-        corrs = csi_io.correlation_matrix(n, pairs, scores)
-    else:
-        gram = csi.GramNCC(h, w)
-        store = csi.PatternStore(h * w, n, precision)
-        store.put(pats)
-        corrs = store.gram(gram, n).cpu().numpy()
-        np.fill_diagonal(corrs, 0.0)               # Main.java:121-126 never sets the diagonal
-        store.close()
-        gram.close()
+    corrs = _compare(pats, n, h, w, filt, pce, precision, pce_variant)
     filt.cleanup()
     return corrs
 
@@ -55,3 +65,73 @@ def common_source_identification(images, filenames, testcase: str, outdir: str =
     paths = csi_io.write_all(testcase, corrs, list(filenames), outdir)
     linkage = csi_io.hierarchical_clustering(corrs, list(filenames))
     return corrs, linkage, paths
+
+
+def scan_jpeg_directory(folder: str) -> list[str]:
+    """Names of the entries of `folder` ending in ".jpg" (any case), sorted by
+    name: ``Util.scanDirectory`` with the filter of ``Main.java:88``
+    (``rocket/util/Util.java:91-106``)."""
+    if not os.path.isdir(folder):
+        raise FileNotFoundError(f"folderpath does not exist or is not a directory: {folder}")
+    return sorted(f for f in os.listdir(folder) if f.lower().endswith(".jpg"))
+
+
+def correlation_matrix_jpeg(paths, strategy: str = "NCC", precision: str = "split_bf16",
+                            pce_variant: int = _lib.PCE_REFERENCE, batch: int = 32):
+    """All-pairs scores of the JPEG files `paths`, decoded on the GPU
+    (``prnu_jpeg_decode_batch``, libjpeg's result bit for bit) straight into
+    the device buffer the extraction reads.  The frame size is the first
+    file's (``Main.java:97-104``); a file of another size raises ValueError
+    ("image has incorrect dimensions", ``CommonSourceIdentificationContext.java:96-99``),
+    one that does not decode raises RuntimeError ("error while reading",
+    ``:91-95``)."""
+    import torch
+    from . import jpeg
+    n = len(paths)
+    if n == 0:
+        raise ValueError("no JPEG files")
+    data = [open(p, "rb").read() for p in paths]
+    try:
+        h, w = jpeg.jpeg_size(data[0])
+    except _lib.PrnuError as e:
+        raise RuntimeError(f"error while reading: {paths[0]}") from e
+    for p, d in zip(paths, data):
+        try:
+            size = jpeg.jpeg_size(d)
+        except _lib.PrnuError as e:
+            raise RuntimeError(f"error while reading: {p}") from e
+        if size != (h, w):
+            raise ValueError(f"image has incorrect dimensions: {p}")
+    pce = strategy.upper() == "PCE"
+    filt = csi.PRNUFilter(h, w, apply_spectral_filter=pce)
+    pats = torch.empty((n, filt.get_output_size() // 4), device="cuda")
+    dec = jpeg.JpegDecoder()
+    rgb = torch.empty((min(batch, n), h, w, 3), dtype=torch.uint8, device="cuda")
+    try:
+        for b in range(0, n, batch):
+            m = min(batch, n - b)
+            out, status = dec.decode(data[b:b + m], out=rgb[:m], strict=False)
+            for k, st in enumerate(status):
+                if st not in (_lib.OK, _lib.JPEG_CORRUPT_DATA):   # libjpeg only warns on corrupt data
+                    raise RuntimeError(f"error while reading: {paths[b + k]}")
+            filt.apply_gpu_batch(out.view(m, -1), pats[b:b + m], _lib.INPUT_RGB8)
+        corrs = _compare(pats, n, h, w, filt, pce, precision, pce_variant)
+    finally:
+        dec.close()
+        filt.cleanup()
+    return corrs
+
+
+def common_source_identification_dir(testcase: str, folder: str, outdir: str = ".", strategy: str = "NCC",
+                                      precision: str = "split_bf16", pce_variant: int = _lib.PCE_REFERENCE):
+    """``Main``'s flow from a folder of JPEG files (``Main.java:69-139``): the
+    test-case name check, the sorted ``.jpg`` listing, GPU decode and
+    extraction, all pairs, clustering and the five result files named after
+    the test case.  Returns (corrs, linkage, paths, filenames)."""
+    if not re.fullmatch(r"[a-zA-Z0-9_.-]+", testcase):
+        raise ValueError("testcase will be used for file names, please do not use special characters")
+    names = scan_jpeg_directory(folder)
+    corrs = correlation_matrix_jpeg([os.path.join(folder, f) for f in names], strategy, precision, pce_variant)
+    paths = csi_io.write_all(testcase, corrs, names, outdir)
+    linkage = csi_io.hierarchical_clustering(corrs, names)
+    return corrs, linkage, paths, names

@@ -39,7 +39,9 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -1417,6 +1419,53 @@ int prnu_jpeg_decode_batch(prnu_jpeg_decoder *dec, const void *const *h_files, c
     return decode_batch(reinterpret_cast<JpegDecoder *>(dec), reinterpret_cast<const uint8_t *const *>(h_files),
                         h_sizes, count, d_rgb, rgb_stride, rgb_capacity, h_width, h_height, h_status,
                         (hipStream_t)stream);
+}
+
+// readJPEG (readjpeg/src/main/cpp/native.cpp:59-134) under its own name and
+// signature, so that ReadJPEG.Bindings (CSI/ReadJPEG.java:12-23) can load
+// this library in place of libreadjpeg: one file into a host buffer, 0 on
+// success, 1 on failure with the reason on stderr, width and height set only
+// on success.  It decodes on the calling thread's current device through one
+// process-wide decoder and device buffer (calls are serialised) and copies
+// the frame back to `output`.
+int readJPEG(const char *input, int32_t input_size, int32_t *output_width, int32_t *output_height, char *output,
+             int32_t output_capacity) {
+    static std::mutex mu;
+    static JpegDecoder *dec = nullptr;
+    static Workspace frame;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!input || input_size <= 0 || !output_width || !output_height || !output) {
+        std::fprintf(stderr, "error: readJPEG: null pointer or empty input\n");
+        return 1;
+    }
+    FileInfo f;
+    if (parse_file(reinterpret_cast<const uint8_t *>(input), input_size, f) != PRNU_OK) {
+        std::fprintf(stderr, "error: %s\n", f.err.c_str());
+        return 1;
+    }
+    const int64_t bytes = (int64_t)f.W * f.H * 3;
+    if (bytes > output_capacity) {
+        std::fprintf(stderr, "error: insufficient buffer space provided for image\n");
+        return 1;
+    }
+    if (!dec) dec = new (std::nothrow) JpegDecoder();
+    if (!dec || frame.ensure((size_t)bytes) != PRNU_OK) {
+        std::fprintf(stderr, "error: readJPEG: out of memory\n");
+        return 1;
+    }
+    const uint8_t *file = reinterpret_cast<const uint8_t *>(input);
+    const int64_t size = input_size;
+    int32_t w = 0, h = 0, st = 0;
+    if (decode_batch(dec, &file, &size, 1, static_cast<uint8_t *>(frame.ptr), bytes, bytes, &w, &h, &st, nullptr) !=
+            PRNU_OK ||
+        hipMemcpy(output, frame.ptr, (size_t)bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        std::fprintf(stderr, "error: %s\n", prnu_last_error());
+        return 1;
+    }
+    *output_width = w;
+    *output_height = h;
+    return 0;
 }
 
 }  // extern "C"

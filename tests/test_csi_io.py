@@ -141,3 +141,16 @@ def test_result_files_round_trip(tmp_path):
     assert rows[0] == "".join(io.java_fixed6(v) + ", " for v in c[0]) and rows[n] == "" and rows[n + 1] == ""
     link = open(paths["linkage"]).read().splitlines()
     assert len(link) == n and link[-1] == ""          # n-1 links, then an empty line
+
+
+def test_jpeg_directory_listing_and_testcase_name(tmp_path):
+    """Main.java:69-73, 88: the test-case name check and the sorted,
+    case-insensitive ".jpg" listing (Util.java:91-106), before any GPU work."""
+    from rocket_amd import app
+    for nm in ("b.jpg", "A.JPG", "c.jpeg", "a.Jpg", "notes.txt", "z.jpg"):
+        (tmp_path / nm).write_bytes(b"")
+    assert app.scan_jpeg_directory(str(tmp_path)) == ["A.JPG", "a.Jpg", "b.jpg", "z.jpg"]
+    with pytest.raises(ValueError, match="special characters"):
+        app.common_source_identification_dir("bad name!", str(tmp_path))
+    with pytest.raises(FileNotFoundError):
+        app.scan_jpeg_directory(str(tmp_path / "missing"))

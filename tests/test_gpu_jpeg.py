@@ -152,3 +152,21 @@ def test_rejected_files_leave_the_rest_decoded():
     _, st = dec.decode(good[:1], out=small, strict=False)
     assert st == [_lib.EINVAL]
     dec.close()
+
+
+def test_readjpeg_drop_in_matches_libjpeg():
+    """The exported readJPEG (native.cpp:59-134's signature): one file into a
+    host buffer, sizes out, 0 on success; then a rejected file gives 1."""
+    import ctypes
+    lib = _lib.load()
+    for sub in (0, 2):
+        data = encode(frame(77, 45, sub), quality=88, subsampling=sub)
+        cap = 77 * 45 * 3 + 5
+        w, h = ctypes.c_int32(), ctypes.c_int32()
+        out = ctypes.create_string_buffer(cap)
+        assert lib.readJPEG(data, len(data), ctypes.byref(w), ctypes.byref(h), out, cap) == 0
+        assert (w.value, h.value) == (77, 45)
+        got = np.frombuffer(out.raw[: 77 * 45 * 3], dtype=np.uint8).reshape(45, 77, 3)
+        assert np.array_equal(got, pil_rgb(data))
+    prog = encode(frame(30, 20, 1), progressive=True)
+    assert lib.readJPEG(prog, len(prog), ctypes.byref(w), ctypes.byref(h), out, cap) == 1

@@ -3,6 +3,7 @@ sizes of the committed fixtures, and the files the decoder rejects as the
 reference's readJPEG does (readjpeg/src/main/cpp/native.cpp:99-103: not
 3-component RGB; :26-29: input ends before libjpeg is done) or does not
 implement (progressive)."""
+import ctypes
 import glob
 import io
 import os
@@ -46,3 +47,22 @@ def test_info_rejects_what_readjpeg_rejects():
     with pytest.raises(_lib.PrnuError):
         jpeg.jpeg_size(b"GIF89a not a jpeg")
     assert jpeg.jpeg_size(good) == (16, 24)
+
+
+def _read_jpeg(data, capacity):
+    lib = _lib.load()
+    w, h = ctypes.c_int32(-1), ctypes.c_int32(-1)
+    out = ctypes.create_string_buffer(max(1, capacity))
+    rc = lib.readJPEG(data, len(data), ctypes.byref(w), ctypes.byref(h), out, capacity)
+    return rc, w.value, h.value, out
+
+
+def test_readjpeg_drop_in_rejects_before_the_device():
+    """readJPEG's own failures (native.cpp:99-108): 1, sizes untouched, for a
+    gray file, a non-JPEG and an output buffer smaller than 3 * w * h; all
+    decided on the host."""
+    gray = _encode(np.zeros((12, 20), dtype=np.uint8))
+    rgb = _encode(np.zeros((12, 20, 3), dtype=np.uint8))
+    for data, cap in ((gray, 10000), (b"\xff\xd8 not a jpeg", 10000), (rgb, 12 * 20 * 3 - 1)):
+        rc, w, h, _ = _read_jpeg(data, cap)
+        assert (rc, w, h) == (1, -1, -1)
