@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
-    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg"],
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill"],
                     help="run one secondary leg alone and print its result (for profiling)")
     ap.add_argument("--extract-shape", default=None, metavar="HxW",
                     help="extraction leg: one frame shape only (default: 1536x2048 and 512x512)")
@@ -503,6 +503,56 @@ def jpeg_bench(torch, batch=32, reps=3):
             "cpu_libjpeg_turbo_pillow_images_per_s_1_thread": 1.0 / cpu}
 
 
+SPILL_N, SPILL_COUNT, SPILL_BLOCK, SPILL_DEVICE_BLOCKS = 1024 * 1024, 4096, 512, 4
+
+
+def spill_bench(torch):
+    """All-pairs NCC of 4096 patterns of 1024x1024 held in the host spill tier
+    (FP16 slot images in pinned host memory, 8 blocks of 512) and streamed
+    through a device store of 4 blocks (half the set: 2 resident row blocks,
+    2 streaming column regions), against the same set resident in one FP16
+    store."""
+    from rocket_amd import csi, spill
+    g = torch.Generator(device="cuda").manual_seed(99)
+    cams = torch.randn((8, SPILL_N), generator=g, device="cuda")
+    tier = spill.HostPatternTier(SPILL_N, SPILL_COUNT, SPILL_BLOCK, "fp16")
+    resident = csi.PatternStore(SPILL_N, SPILL_COUNT, "fp16")
+    for b in range(tier.nblocks):
+        r0, r1 = tier.block_range(b)
+        p = torch.randn((r1 - r0, SPILL_N), generator=g, device="cuda")
+        p += 0.02 * cams[torch.arange(r0, r1, device="cuda") % 8]
+        p -= p.mean(dim=1, keepdim=True)
+        tier.put_block(b, p)
+        resident.put(p, r0)
+        del p
+    gram = csi.GramNCC(1024, 1024)
+    info = {}
+    tier.ncc(gram, device_blocks=SPILL_DEVICE_BLOCKS)        # warm-up (plans, pinned buffers)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    got = tier.ncc(gram, device_blocks=SPILL_DEVICE_BLOCKS, stats=info)
+    dt = time.perf_counter() - t1
+    ref = resident.gram(gram, SPILL_COUNT).cpu().numpy()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    resident.gram(gram, SPILL_COUNT)
+    torch.cuda.synchronize()
+    dt_res = time.perf_counter() - t1
+    iu = np.triu_indices(SPILL_COUNT, 1)
+    pairs = SPILL_COUNT * (SPILL_COUNT - 1) // 2
+    res = {"workload": f"spill{SPILL_COUNT}_1024x1024_fp16_blocks{SPILL_BLOCK}", "pairs": pairs,
+           "host_tier_GB": tier.host_bytes() / 1e9, "pairs_per_s": pairs / dt, "ms": 1000.0 * dt,
+           "bytes_to_device_GB": info["bytes_to_device"] / 1e9,
+           "effective_h2d_GBps": info["bytes_to_device"] / dt / 1e9, "block_pairs": info["block_pairs"],
+           "block_loads": info["block_loads"], "device_blocks": SPILL_DEVICE_BLOCKS,
+           "resident_pairs_per_s": pairs / dt_res, "resident_ms": 1000.0 * dt_res,
+           "max_abs_diff_vs_resident": float(np.abs(got[iu] - ref[iu]).max())}
+    tier.close()
+    resident.close()
+    gram.close()
+    return res
+
+
 PCE_H, PCE_W, PCE_COUNT, PCE_CAMERAS = 1024, 1024, 256, 8
 
 
@@ -585,7 +635,7 @@ def main():
                "groups": lambda: groups_bench(torch),
                "serving": lambda: serving_bench(torch, args.serving_seconds),
                "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch),
-               "jpeg": lambda: jpeg_bench(torch)}[args.only]
+               "jpeg": lambda: jpeg_bench(torch), "spill": lambda: spill_bench(torch)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
     prec = args.precision
@@ -704,7 +754,7 @@ def main():
             f"{groups['eager_ms_per_step']:.2f} ms eager, {groups['separate_launches_ms_per_step']:.2f} ms as "
             f"separate launches), |dNCC| {groups['max_abs_dncc_vs_fp64_sample']:.2e}")
 
-    extract = pce = jpg = None
+    extract = pce = jpg = spl = None
     if not args.no_extract:
         log("extraction ...")
         extract = extraction_bench(torch)
@@ -714,6 +764,11 @@ def main():
         pce = pce_bench(torch)
         log(f"PCE: {pce['reference_pairs_per_s']:.1f} pairs/s (reference), "
             f"{pce['textbook_pairs_per_s']:.1f} pairs/s (textbook)")
+        log("spill tier ...")
+        spl = spill_bench(torch)
+        log(f"spill tier: {spl['pairs_per_s']:.4g} pairs/s streamed from {spl['host_tier_GB']:.1f} GB of host "
+            f"slot images ({spl['effective_h2d_GBps']:.1f} GB/s to the device), resident "
+            f"{spl['resident_pairs_per_s']:.4g} pairs/s")
         log("JPEG ...")
         jpg = jpeg_bench(torch)
         log(f"JPEG: {jpg['decode_images_per_s']:.0f} images/s decoded, "
@@ -760,6 +815,7 @@ def main():
         "extraction": extract,
         "pce": pce,
         "jpeg": jpg,
+        "spill": spl,
         "groups": groups,
         "c0": c0,
         "measured_peaks": peaks,

@@ -221,6 +221,22 @@ PRNU_EXPORT int prnu_store_put(prnu_store *store, const float *const *d_patterns
 /* Device arrays (capacity entries) of the statistics recorded by put. */
 PRNU_EXPORT int prnu_store_stats(const prnu_store *store, const double **d_sum, const double **d_sumsq,
                                  const float **d_absmax);
+/* Spill tier (R/cache/HostCache.java:12-67, R/cache/FileCache.java:18-130:
+ * the reference keeps what does not fit on the device in pinned host memory
+ * and files).  A "slot image" is `count` slots of a store in the store's own
+ * layout: a 64-byte header (magic "PRNUSLOT", values per pattern, format,
+ * count, K blocks), the K blocks, and the per-slot statistics, so that a
+ * pattern set larger than HBM can wait in host memory or a file and come back
+ * without extraction or conversion.  prnu_store_image_bytes: the size of an
+ * image of `count` slots (-1 for a null store).  prnu_store_save writes slots
+ * [first_slot, first_slot + count) into h_image; prnu_store_load copies an
+ * image into slots [first_slot, first_slot + its count) of a store of the
+ * same length and format (PRNU_EINVAL otherwise).  Both are asynchronous on
+ * `stream` (pinned host memory lets them overlap kernels; with stream 0 they
+ * return when done). */
+PRNU_EXPORT int64_t prnu_store_image_bytes(const prnu_store *store, int count);
+PRNU_EXPORT int prnu_store_save(const prnu_store *store, int first_slot, int count, void *h_image, prnu_stream stream);
+PRNU_EXPORT int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, prnu_stream stream);
 /* All-pairs NCC of slots [first_slot, first_slot + count) in the store's
  * format, same epilogue and output as prnu_gram_ncc.  `ctx` supplies the
  * workspace, the launch shape (its prnu_gram_config of the store's format)

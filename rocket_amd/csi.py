@@ -451,6 +451,26 @@ class PatternStore:
         _sync(sync)
         return out
 
+    def image_bytes(self, count: int) -> int:
+        """Bytes of a slot image of `count` slots (``prnu_store_image_bytes``)."""
+        return int(_lib.load().prnu_store_image_bytes(self.handle, int(count)))
+
+    def save(self, first_slot: int, count: int, image, sync: bool = True) -> None:
+        """Slots [first_slot, first_slot + count) into `image`, a CPU uint8
+        tensor of image_bytes(count) bytes (pinned for an asynchronous copy)."""
+        if image.is_cuda or image.dtype != torch.uint8 or image.numel() < self.image_bytes(count):
+            raise ValueError("image must be a CPU uint8 tensor of image_bytes(count) bytes")
+        _lib.call("prnu_store_save", self.handle, int(first_slot), int(count), image.data_ptr(), _stream())
+        _sync(sync)
+
+    def load(self, first_slot: int, image, sync: bool = True) -> None:
+        """A slot image (written by save, same length and format) into slots
+        from first_slot on."""
+        if image.is_cuda or image.dtype != torch.uint8:
+            raise ValueError("image must be a CPU uint8 tensor")
+        _lib.call("prnu_store_load", self.handle, int(first_slot), image.data_ptr(), _stream())
+        _sync(sync)
+
     def close(self) -> None:
         if self.handle:
             _lib.load().prnu_store_destroy(self.handle)

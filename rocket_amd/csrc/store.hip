@@ -18,6 +18,8 @@
 // value, and the per-pattern FP64 statistics are taken in the same pass.
 #include "prnu_internal.hpp"
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -1532,6 +1534,93 @@ int prnu_store_put(prnu_store *store, const float *const *d_patterns, int count,
     PRNU_REQUIRE(store && d_patterns, "store_put: null pointer");
     Store *s = reinterpret_cast<Store *>(store);
     PRNU_TRY(store_put(s, d_patterns, count, first_slot, (hipStream_t)stream));
+    if (!stream) PRNU_CHECK_HIP(hipStreamSynchronize(nullptr));
+    return PRNU_OK;
+}
+
+// Slot images (the spill tier): a header, then the slots' K blocks, block by
+// block (count x 128 bytes each), then per slot the FP64 sum and sum of
+// squares and the FP32 max|x| and scale (0 for formats without a scale).
+struct SlotImageHeader {
+    char magic[8];  // "PRNUSLOT"
+    int64_t n;      // values per pattern
+    int32_t format, count;
+    int64_t nkb;    // K blocks per pattern
+    int64_t reserved[4];
+};
+static_assert(sizeof(SlotImageHeader) == 64, "slot image header is 64 bytes");
+
+static int64_t slot_image_bytes(const Store *s, int count) {
+    return (int64_t)sizeof(SlotImageHeader) +
+           (int64_t)count * (s->nkb * kBlkBytes + 2 * (int64_t)sizeof(double) + 2 * (int64_t)sizeof(float));
+}
+
+int64_t prnu_store_image_bytes(const prnu_store *store, int count) {
+    if (!store || count < 0) return -1;
+    return slot_image_bytes(reinterpret_cast<const Store *>(store), count);
+}
+
+int prnu_store_save(const prnu_store *store, int first_slot, int count, void *h_image, prnu_stream stream) {
+    PRNU_REQUIRE(store && h_image, "store_save: null pointer");
+    const Store *s = reinterpret_cast<const Store *>(store);
+    PRNU_REQUIRE(count >= 1 && first_slot >= 0 && first_slot + count <= s->capacity,
+                 "store_save: slots [%d, %d) outside the capacity %d", first_slot, first_slot + count, s->capacity);
+    hipStream_t st = (hipStream_t)stream;
+    uint8_t *h = static_cast<uint8_t *>(h_image);
+    SlotImageHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    std::memcpy(hd.magic, "PRNUSLOT", 8);
+    hd.n = s->n;
+    hd.format = s->format;
+    hd.count = count;
+    hd.nkb = s->nkb;
+    std::memcpy(h, &hd, sizeof(hd));
+    uint8_t *data = h + sizeof(hd);
+    const size_t row = (size_t)count * kBlkBytes;
+    PRNU_CHECK_HIP(hipMemcpy2DAsync(data, row, s->blocks + (size_t)first_slot * kBlkBytes,
+                                    (size_t)s->cap_pad * kBlkBytes, row, (size_t)s->nkb, hipMemcpyDeviceToHost, st));
+    uint8_t *stats = data + row * (size_t)s->nkb;
+    PRNU_CHECK_HIP(hipMemcpyAsync(stats, s->sum + first_slot, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+    stats += sizeof(double) * count;
+    PRNU_CHECK_HIP(hipMemcpyAsync(stats, s->sumsq + first_slot, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+    stats += sizeof(double) * count;
+    PRNU_CHECK_HIP(hipMemcpyAsync(stats, s->absmax + first_slot, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+    stats += sizeof(float) * count;
+    if (s->scale)
+        PRNU_CHECK_HIP(hipMemcpyAsync(stats, s->scale + first_slot, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+    else
+        std::memset(stats, 0, sizeof(float) * count);
+    if (!stream) PRNU_CHECK_HIP(hipStreamSynchronize(nullptr));
+    return PRNU_OK;
+}
+
+int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, prnu_stream stream) {
+    PRNU_REQUIRE(store && h_image, "store_load: null pointer");
+    Store *s = reinterpret_cast<Store *>(store);
+    const uint8_t *h = static_cast<const uint8_t *>(h_image);
+    SlotImageHeader hd;
+    std::memcpy(&hd, h, sizeof(hd));
+    PRNU_REQUIRE(std::memcmp(hd.magic, "PRNUSLOT", 8) == 0, "store_load: not a slot image");
+    PRNU_REQUIRE(hd.n == s->n && hd.format == s->format && hd.nkb == s->nkb,
+                 "store_load: image of %lld values in format %d, store of %lld in format %d", (long long)hd.n,
+                 hd.format, (long long)s->n, s->format);
+    const int count = hd.count;
+    PRNU_REQUIRE(count >= 1 && first_slot >= 0 && first_slot + count <= s->capacity,
+                 "store_load: slots [%d, %d) outside the capacity %d", first_slot, first_slot + count, s->capacity);
+    hipStream_t st = (hipStream_t)stream;
+    const uint8_t *data = h + sizeof(hd);
+    const size_t row = (size_t)count * kBlkBytes;
+    PRNU_CHECK_HIP(hipMemcpy2DAsync(s->blocks + (size_t)first_slot * kBlkBytes, (size_t)s->cap_pad * kBlkBytes, data,
+                                    row, row, (size_t)s->nkb, hipMemcpyHostToDevice, st));
+    const uint8_t *stats = data + row * (size_t)s->nkb;
+    PRNU_CHECK_HIP(hipMemcpyAsync(s->sum + first_slot, stats, sizeof(double) * count, hipMemcpyHostToDevice, st));
+    stats += sizeof(double) * count;
+    PRNU_CHECK_HIP(hipMemcpyAsync(s->sumsq + first_slot, stats, sizeof(double) * count, hipMemcpyHostToDevice, st));
+    stats += sizeof(double) * count;
+    PRNU_CHECK_HIP(hipMemcpyAsync(s->absmax + first_slot, stats, sizeof(float) * count, hipMemcpyHostToDevice, st));
+    stats += sizeof(float) * count;
+    if (s->scale)
+        PRNU_CHECK_HIP(hipMemcpyAsync(s->scale + first_slot, stats, sizeof(float) * count, hipMemcpyHostToDevice, st));
     if (!stream) PRNU_CHECK_HIP(hipStreamSynchronize(nullptr));
     return PRNU_OK;
 }
