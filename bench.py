@@ -241,6 +241,21 @@ def cpu_baseline_aux(seconds=6.0):
                       "FP64 numpy restatement, one thread"}
 
 
+def cpu_check_pce(check):
+    """Part of the CPU baseline leg: the FP64 numpy restatement scores the
+    sampled pairs of the PCE leg from the same FP32 patterns (both variants);
+    max relative error of the GPU scores against it."""
+    sys.path.insert(0, ROOT)
+    from oracle import prnu_oracle as O
+    out = {}
+    for name, fn in (("reference", O.pce_reference), ("textbook", O.pce_textbook)):
+        want = np.array([fn(check["patterns"][i].astype(np.float64), check["patterns"][j].astype(np.float64))
+                         for i, j in check["pairs"]])
+        out[f"{name}_max_rel_err_vs_fp64"] = float(np.max(np.abs(check[name] - want) / np.abs(want)))
+    out["check_pairs"] = len(check["pairs"])
+    return out
+
+
 def synth_images(torch, fingerprints, owners, seed, h=None, w=None):
     """u8 (len(owners), h, w) images on the GPU: I = clamp(round(S*(1+0.02*K) + N(0, 2^2)), 0, 255)
     with S a sum of 8 random-phase cosines (frequencies <= 4 per frame) rescaled
@@ -569,9 +584,24 @@ def pce_bench(torch):
     for b in range(0, PCE_COUNT, 64):
         imgs = synth_images(torch, sensors, owners[b:b + 64], seed=b, h=PCE_H, w=PCE_W)
         f.apply_gpu_batch(imgs.view(len(imgs), n), spec[b:b + 64], _lib.INPUT_GRAY8)
+    # FP32 patterns of a few images (same chain, NCC mode) for the accuracy
+    # check of the CPU baseline leg: 4 same-camera and 4 cross-camera pairs
+    check_pairs = [(0, 8), (1, 9), (2, 10), (3, 11), (0, 1), (2, 3), (4, 5), (6, 7)]
+    check_imgs = sorted({i for pr in check_pairs for i in pr})
+    fn = csi.PRNUFilter(PCE_H, PCE_W)
+    imgs = synth_images(torch, sensors, owners[:64], seed=0, h=PCE_H, w=PCE_W)
+    check_pats = torch.empty((len(check_imgs), n), device="cuda")
+    fn.apply_gpu_batch(imgs[check_imgs].reshape(len(check_imgs), n), check_pats, _lib.INPUT_GRAY8)
+    fn.cleanup()
+    check_spec = torch.empty((len(check_imgs), f.get_output_size() // 4), device="cuda")
+    for q in range(len(check_imgs)):
+        f.get_spectral_filter().apply_gpu(check_pats[q], check_spec[q])
+    slot = {img: q for q, img in enumerate(check_imgs)}
+    cp = np.array([(slot[i], slot[j]) for i, j in check_pairs], dtype=np.int32)
     pairs = np.array([(i, j) for i in range(PCE_COUNT) for j in range(i + 1, PCE_COUNT)], dtype=np.int32)
     same = np.array([owners[i] == owners[j] for i, j in pairs])
     res = {"workload": f"pce{PCE_COUNT}_{PCE_W}x{PCE_H}", "pairs": int(len(pairs))}
+    check = {}
     for variant, name in ((_lib.PCE_REFERENCE, "reference"), (_lib.PCE_TEXTBOOK, "textbook")):
         cmp = csi.PeakToCorrelationEnergy(PCE_H, PCE_W, variant, _ctx=f._ctx)
         out = cmp.pairs(spec, pairs[:256])                         # warm-up
@@ -584,7 +614,11 @@ def pce_bench(torch):
         res[f"{name}_pairs_per_s"] = len(pairs) / dt
         res[f"{name}_same_camera_min"] = float(v[same].min())
         res[f"{name}_other_camera_max"] = float(v[~same].max())
+        check[name] = cmp.pairs(check_spec, cp).cpu().numpy()
     f.cleanup()
+    check["pairs"] = check_pairs
+    check["patterns"] = {img: check_pats[q].cpu().numpy().reshape(PCE_H, PCE_W) for img, q in slot.items()}
+    res["_check"] = check
     return res
 
 
@@ -636,7 +670,10 @@ def main():
                "serving": lambda: serving_bench(torch, args.serving_seconds),
                "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch),
                "jpeg": lambda: jpeg_bench(torch), "spill": lambda: spill_bench(torch)}[args.only]
-        print(json.dumps({args.only: leg()}), flush=True)
+        out = leg()
+        if isinstance(out, dict):
+            out.pop("_check", None)
+        print(json.dumps({args.only: out}), flush=True)
         return
     prec = args.precision
     t0 = time.time()
@@ -762,6 +799,13 @@ def main():
             f"{extract['pce_images_per_s']:.1f} images/s (PCE spectra) at {WIDTH}x{HEIGHT}")
         log("PCE ...")
         pce = pce_bench(torch)
+        check = pce.pop("_check")
+        if not args.no_cpu:
+            pce.update(cpu_check_pce(check))
+            log(f"PCE vs FP64 restatement ({pce['check_pairs']} pairs): max relative error "
+                f"{pce['reference_max_rel_err_vs_fp64']:.2e} (reference), {pce['textbook_max_rel_err_vs_fp64']:.2e} "
+                f"(textbook)")
+        del check
         log(f"PCE: {pce['reference_pairs_per_s']:.1f} pairs/s (reference), "
             f"{pce['textbook_pairs_per_s']:.1f} pairs/s (textbook)")
         log("spill tier ...")
