@@ -352,38 +352,47 @@ def serving_bench(torch, duration: float):
 GROUPS_C3 = [(2000, 1024, 1024), (1500, 1536, 2048), (596, 3000, 4000)]
 
 
+GROUP_FORMATS = ("split_bf16", "fp32", "fp16", "fp8")
+
+
 def groups_bench(torch, steps=5):
     """All-pairs NCC of three image-size groups (2000 @ 1024x1024, 1500 @
     2048x1536, 596 @ 4000x3000; 3,300,560 pairs) from split-BF16 stores: one
     grouped launch (autotuned slab, replayed as a HIP graph) against one
-    store Gram launch per group; FP64 check on 24 patterns of each group."""
+    store Gram launch per group; FP64 check on 24 patterns of each group.
+    The same patterns in FP32, FP16 and FP8 stores: one autotuned grouped
+    launch each."""
     from rocket_amd import csi
     g = torch.Generator(device="cuda").manual_seed(31337)
-    stores, checks, spec = [], [], []
+    stores = {fmt: [] for fmt in GROUP_FORMATS}
+    checks = []
     cam0 = 0
     t0 = time.time()
     for count, h, w in GROUPS_C3:
         n = h * w
         cams = {1024: 16, 1536: 16, 3000: 8}[h]
         k = torch.randn((cams, n), generator=g, device="cuda")
-        st = csi.PatternStore(n, count, "split_bf16")
+        sts = {fmt: csi.PatternStore(n, count, fmt) for fmt in GROUP_FORMATS}
         chunk = max(24, int((1 << 30) // (4 * n)))
         for b in range(0, count, chunk):
             e = min(count, b + chunk)
             blk = torch.randn((e - b, n), generator=g, device="cuda")
             blk += 0.02 * k[torch.arange(b, e, device="cuda") % cams]
             blk -= blk.mean(dim=1, keepdim=True)
-            st.put(blk, first_slot=b)
+            for st in sts.values():
+                st.put(blk, first_slot=b)
             if b == 0:
                 checks.append(blk[:24].cpu().numpy().astype(np.float64))
             del blk
         del k
         cam0 += cams
-        stores.append(st)
-        spec.append((st, 0, count))
+        for fmt, st in sts.items():
+            stores[fmt].append(st)
     torch.cuda.synchronize()
-    log(f"groups: stores ready ({sum(c for c, _, _ in GROUPS_C3)} patterns, {cam0} cameras) in {time.time() - t0:.1f} s")
+    log(f"groups: stores ready ({sum(c for c, _, _ in GROUPS_C3)} patterns, {cam0} cameras, "
+        f"{len(GROUP_FORMATS)} formats) in {time.time() - t0:.1f} s")
     gram = csi.GramNCC(8, 8)
+    spec = [(st, 0, c) for st, (c, _, _) in zip(stores["split_bf16"], GROUPS_C3)]
     plan = csi.GramPlan(gram, spec)
     slab, tune_ms = plan.autotune()
     info = plan.info()
@@ -396,6 +405,15 @@ def groups_bench(torch, steps=5):
             fn()
         torch.cuda.synchronize()
         return 1000.0 * (time.perf_counter() - t1) / reps
+
+    def sample_dncc(outs):
+        worst = 0.0
+        for host, got in zip(checks, outs):
+            gm = host @ host.T
+            sm = host.sum(axis=1)
+            d = np.sqrt(np.einsum("ij,ij->i", host, host) - sm * sm)
+            worst = max(worst, float(np.abs(got[:24, :24] - gm / np.outer(d, d)).max()))
+        return worst
 
     ms_graph = timed(lambda: plan.run(use_graph=True, sync=False), steps)
     ms_eager = timed(lambda: plan.run(use_graph=False, sync=False), steps)
@@ -410,25 +428,31 @@ def groups_bench(torch, steps=5):
         for (st, first, c), o in zip(spec, sep_out):
             st.gram(gram, c, first, out=o, sync=False)
     ms_sep = timed(separate, steps)
-    dncc = 0.0
-    for host, got in zip(checks, outs):
-        gm = host @ host.T
-        sm = host.sum(axis=1)
-        d = np.sqrt(np.einsum("ij,ij->i", host, host) - sm * sm)
-        dncc = max(dncc, float(np.abs(got[:24, :24] - gm / np.outer(d, d)).max()))
+    dncc = sample_dncc(outs)
     pairs = sum(c * (c - 1) // 2 for c, _, _ in GROUPS_C3)
     flops = 3 * info["flop"]
     plan.close()
-    for st in stores:
-        st.close()
+    res = {"workload": "groups 2000@1024x1024 + 1500@2048x1536 + 596@4000x3000", "pairs": pairs,
+           "pairs_per_s": pairs / (ms_graph / 1000.0), "ms_per_step": ms_graph, "eager_ms_per_step": ms_eager,
+           "separate_launches_ms_per_step": ms_sep, "slab_values": slab, "work_items": info["work_items"],
+           "launch": "persistent" if tuned_persistent else "per_item",
+           ("per_item" if tuned_persistent else "persistent") + "_ms_per_step": ms_other,
+           "mfma_tflops": flops / (ms_graph / 1000.0) / 1e12, "max_abs_dncc_vs_fp64_sample": dncc,
+           "autotune_best_ms": tune_ms}
+    for fmt in GROUP_FORMATS[1:]:
+        p = csi.GramPlan(gram, [(st, 0, c) for st, (c, _, _) in zip(stores[fmt], GROUPS_C3)])
+        p.autotune()
+        ms = timed(lambda: p.run(use_graph=True, sync=False), steps)
+        res[f"{fmt}_pairs_per_s"] = pairs / (ms / 1000.0)
+        res[f"{fmt}_ms_per_step"] = ms
+        res[f"{fmt}_tflops"] = p.info()["flop"] / (ms / 1000.0) / 1e12
+        res[f"{fmt}_max_abs_dncc_vs_fp64_sample"] = sample_dncc([o.cpu().numpy() for o in p.outputs])
+        p.close()
+    for sts in stores.values():
+        for st in sts:
+            st.close()
     gram.close()
-    return {"workload": "groups 2000@1024x1024 + 1500@2048x1536 + 596@4000x3000", "pairs": pairs,
-            "pairs_per_s": pairs / (ms_graph / 1000.0), "ms_per_step": ms_graph, "eager_ms_per_step": ms_eager,
-            "separate_launches_ms_per_step": ms_sep, "slab_values": slab, "work_items": info["work_items"],
-            "launch": "persistent" if tuned_persistent else "per_item",
-            ("per_item" if tuned_persistent else "persistent") + "_ms_per_step": ms_other,
-            "mfma_tflops": flops / (ms_graph / 1000.0) / 1e12, "max_abs_dncc_vs_fp64_sample": dncc,
-            "autotune_best_ms": tune_ms}
+    return res
 
 
 def extraction_bench(torch, batch=32, shape=None, only_mode=None):
@@ -789,7 +813,8 @@ def main():
         groups = groups_bench(torch)
         log(f"groups: {groups['pairs_per_s']:.4g} pairs/s ({groups['ms_per_step']:.2f} ms graph, "
             f"{groups['eager_ms_per_step']:.2f} ms eager, {groups['separate_launches_ms_per_step']:.2f} ms as "
-            f"separate launches), |dNCC| {groups['max_abs_dncc_vs_fp64_sample']:.2e}")
+            f"separate launches), |dNCC| {groups['max_abs_dncc_vs_fp64_sample']:.2e}; "
+            + ", ".join(f"{fmt} {groups[fmt + '_pairs_per_s']:.4g} pairs/s" for fmt in GROUP_FORMATS[1:]))
 
     extract = pce = jpg = spl = None
     if not args.no_extract:
