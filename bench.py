@@ -214,31 +214,50 @@ def cpu_baseline(torch, p):
 SERVE_H, SERVE_W, SERVE_STORE, SERVE_RATE = 1024, 1024, 20000, 200.0
 
 
-def cpu_baseline_aux(seconds=6.0):
-    """The FP64 numpy restatement of the extraction chain (RGB8 512x512) and of
-    reference-variant PCE (1024x1024 spectra) timed on one host core over a
-    bounded sample (part of the CPU baseline leg)."""
-    sys.path.insert(0, ROOT)
-    from oracle import prnu_oracle as O
+def cpu_baseline_aux():
+    """The C restatement of the extraction chain (the applyCPU methods of
+    kernels/filter/: grayscale, FastNoise, ZeroMean, Wiener with its own FFT)
+    and of PCE from two spectra (both variants), one image or pair per
+    thread on all host cores, over bounded samples (part of the CPU baseline
+    leg)."""
+    lib_path = os.path.join(ROOT, "oracle", "_build", "libprnu_cpu_port.so")
+    port = ctypes.CDLL(lib_path)
+    port.port_max_threads.restype = ctypes.c_int
+    port.port_prnu_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    port.port_pce_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    threads = port.port_max_threads()
     rng = np.random.default_rng(5)
-    img = rng.integers(0, 256, size=(512, 512, 3), dtype=np.uint8)
-    t0 = time.perf_counter()
-    n_img = 0
-    while time.perf_counter() - t0 < seconds / 2 or n_img < 1:
-        O.prnu_pattern(img)
-        n_img += 1
-    ext = n_img / (time.perf_counter() - t0)
-    a = rng.standard_normal((1024, 1024))
-    b = rng.standard_normal((1024, 1024))
-    t0 = time.perf_counter()
-    n_pce = 0
-    while time.perf_counter() - t0 < seconds / 2 or n_pce < 1:
-        O.pce_reference(a, b)
-        n_pce += 1
-    pce = n_pce / (time.perf_counter() - t0)
-    return {"extraction_images_per_s_512": ext, "pce_pairs_per_s_1024": pce, "cores": 1, "kind": "port",
-            "sample": f"{n_img} RGB8 512x512 extractions and {n_pce} reference-variant PCE pairs at 1024x1024, "
-                      "FP64 numpy restatement, one thread"}
+    res = {"cores": int(threads), "kind": "port"}
+    samples = []
+    for h, w, tag in ((HEIGHT, WIDTH, ""), (512, 512, "_512")):
+        count = 2 * threads
+        imgs = rng.integers(0, 256, size=(count, h, w, 3), dtype=np.uint8)
+        pats = np.zeros((count, h, w), dtype=np.float32)
+        t0 = time.perf_counter()
+        rc = port.port_prnu_batch(imgs.ctypes.data, count, h, w, 0, pats.ctypes.data, None, threads)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError("CPU port extraction failed")
+        res[f"extraction_images_per_s{tag}"] = count / dt
+        samples.append(f"{count} RGB8 {w}x{h} extractions")
+        del imgs, pats
+    h = w = PCE_H
+    spec = (rng.standard_normal((4, h, w // 2 + 1)) + 1j * rng.standard_normal((4, h, w // 2 + 1))).astype(np.complex64)
+    count = 4 * threads
+    pairs = np.array([(q % 4, (q + 1) % 4) for q in range(count)], dtype=np.int32)
+    out = np.zeros(count)
+    for variant, name in ((0, "reference"), (1, "textbook")):
+        t0 = time.perf_counter()
+        rc = port.port_pce_pairs(spec.ctypes.data, h, w, pairs.ctypes.data, count, variant, out.ctypes.data, threads)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError("CPU port PCE failed")
+        res[f"pce_{name}_pairs_per_s_1024"] = count / dt
+    samples.append(f"{count} PCE pairs of 1024x1024 spectra per variant")
+    res["sample"] = ", ".join(samples) + f"; C port (oracle/cpu_port.c), OpenMP, {threads} threads"
+    return res
 
 
 def cpu_check_pce(check):
@@ -802,8 +821,9 @@ def main():
         if cpu:
             log(f"CPU: {cpu['value']:.4g} pairs/s on {cpu['cores']} threads")
         cpu_aux = cpu_baseline_aux()
-        log(f"CPU: extraction {cpu_aux['extraction_images_per_s_512']:.2f} images/s (512x512), "
-            f"PCE {cpu_aux['pce_pairs_per_s_1024']:.2f} pairs/s (1024x1024), one thread")
+        log(f"CPU: extraction {cpu_aux['extraction_images_per_s']:.2f} images/s ({WIDTH}x{HEIGHT}), "
+            f"{cpu_aux['extraction_images_per_s_512']:.2f} images/s (512x512), PCE "
+            f"{cpu_aux['pce_reference_pairs_per_s_1024']:.2f} pairs/s (1024x1024), {cpu_aux['cores']} threads")
 
     del p                                               # free 12.9 GB before the other legs
     torch.cuda.empty_cache()
