@@ -917,27 +917,50 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_persistent_kernel(
 __global__ void store_queue_reset_kernel(int *queue) { queue[threadIdx.x] = 0; }
 
 // FP64 sum of the S partials + NCC epilogue (normalizedcrosscorrelation.cu:121-123).
+// Four consecutive elements of a partial tile per thread: one 16-byte load
+// per slab, four FP64 sums, each in slab order (the same order as one
+// element per thread, so the results do not depend on this grouping).
+// grid (kTileS * kTileS / 1024, tiles), 256 threads.
+__device__ __forceinline__ void sum_slabs4(const float *__restrict__ p, int S, double g[4]) {
+    g[0] = g[1] = g[2] = g[3] = 0.0;
+    const float4 *q4 = reinterpret_cast<const float4 *>(p);
+    constexpr int kStride = kTileS * kTileS / 4;
+#pragma unroll 8
+    for (int q = 0; q < S; ++q) {
+        const float4 v = q4[(int64_t)q * kStride];
+        g[0] += (double)v.x;
+        g[1] += (double)v.y;
+        g[2] += (double)v.z;
+        g[3] += (double)v.w;
+    }
+}
+
 __global__ __launch_bounds__(256) void store_reduce_kernel(const float *__restrict__ partials, int S, int nt,
                                                            int count, const double *__restrict__ sum,
                                                            const double *__restrict__ sumsq,
                                                            const float *__restrict__ scale,
                                                            double *__restrict__ out, int64_t ld) {
     const int t = blockIdx.y;
-    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int e = (blockIdx.x * 256 + threadIdx.x) * 4;
     const int2 tile = tri_tile_s(t, nt);
     const int i = tile.x * kTileS + (e >> 8);
-    const int j = tile.y * kTileS + (e & 255);
-    if (i >= count || j >= count) return;
-    if (tile.x == tile.y && i > j) return;
-    const float *p = partials + (int64_t)t * S * (kTileS * kTileS) + e;
-    double g = 0.0;
-    for (int q = 0; q < S; ++q) g += (double)p[(int64_t)q * (kTileS * kTileS)];
-    if (scale) g *= (double)scale[i] * (double)scale[j];  // FP8: undo the per-pattern scales
+    const int j0 = tile.y * kTileS + (e & 255);
+    if (i >= count || j0 >= count) return;
+    if (tile.x == tile.y && i > j0 + 3) return;
+    double g[4];
+    sum_slabs4(partials + (int64_t)t * S * (kTileS * kTileS) + e, S, g);
     const double di = sqrt(sumsq[i] - sum[i] * sum[i]);
-    const double dj = sqrt(sumsq[j] - sum[j] * sum[j]);
-    const double v = g / (di * dj);
-    out[(int64_t)i * ld + j] = v;
-    out[(int64_t)j * ld + i] = v;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int j = j0 + c;
+        if (j >= count || (tile.x == tile.y && i > j)) continue;
+        double v = g[c];
+        if (scale) v *= (double)scale[i] * (double)scale[j];  // FP8 / FP16: undo the per-pattern scales
+        const double dj = sqrt(sumsq[j] - sum[j] * sum[j]);
+        v = v / (di * dj);
+        out[(int64_t)i * ld + j] = v;
+        out[(int64_t)j * ld + i] = v;
+    }
 }
 
 // Rectangle: element (i, j) of tile t = (I, J) is rows i of [first_a..) and
@@ -952,18 +975,23 @@ __global__ __launch_bounds__(256) void store_reduce_rect_kernel(const float *__r
                                                                 const float *__restrict__ scale_b,
                                                                 double *__restrict__ out, int64_t ld) {
     const int t = blockIdx.y;
-    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int e = (blockIdx.x * 256 + threadIdx.x) * 4;
     const int I = t / ntb, J = t - (t / ntb) * ntb;
     const int i = I * kTileS + (e >> 8);
-    const int j = J * kTileS + (e & 255);
-    if (i >= count_a || j >= count_b) return;
-    const float *p = partials + (int64_t)t * S * (kTileS * kTileS) + e;
-    double g = 0.0;
-    for (int q = 0; q < S; ++q) g += (double)p[(int64_t)q * (kTileS * kTileS)];
-    if (scale_a) g *= (double)scale_a[i] * (double)scale_b[j];
+    const int j0 = J * kTileS + (e & 255);
+    if (i >= count_a || j0 >= count_b) return;
+    double g[4];
+    sum_slabs4(partials + (int64_t)t * S * (kTileS * kTileS) + e, S, g);
     const double di = sqrt(sumsq_a[i] - sum_a[i] * sum_a[i]);
-    const double dj = sqrt(sumsq_b[j] - sum_b[j] * sum_b[j]);
-    out[(int64_t)i * ld + j] = g / (di * dj);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int j = j0 + c;
+        if (j >= count_b) continue;
+        double v = g[c];
+        if (scale_a) v *= (double)scale_a[i] * (double)scale_b[j];
+        const double dj = sqrt(sumsq_b[j] - sum_b[j] * sum_b[j]);
+        out[(int64_t)i * ld + j] = v / (di * dj);
+    }
 }
 
 }  // namespace
@@ -1148,7 +1176,7 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
             hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_SPLIT_BF16>, grid, dim3(kThr4), 0, st, a);
         PRNU_CHECK_LAUNCH();
         time_end(c, st);
-        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials,
+        hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 1024, tiles), dim3(256), 0, st, a.partials,
                            a.S, a.nt, count, s->sum + first, s->sumsq + first, s->scale ? s->scale + first : nullptr,
                            d_ncc, (int64_t)count);
         PRNU_CHECK_LAUNCH();
@@ -1174,7 +1202,7 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
 #undef PRNU_STORE_GRAM
     PRNU_CHECK_LAUNCH();
     time_end(c, st);
-    hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, tiles), dim3(256), 0, st, a.partials, a.S,
+    hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 1024, tiles), dim3(256), 0, st, a.partials, a.S,
                        a.nt, count, s->sum + first, s->sumsq + first, s->scale ? s->scale + first : nullptr, d_ncc,
                        (int64_t)count);
     PRNU_CHECK_LAUNCH();
@@ -1336,11 +1364,11 @@ int plan_launch(GramPlan *p, hipStream_t st) {
         const PlanBlock &g = p->groups[i];
         const Store *s = g.store;
         if (g.tri) {
-            hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 256, g.tiles()), dim3(256), 0, st,
+            hipLaunchKernelGGL(store_reduce_kernel, dim3(kTileS * kTileS / 1024, g.tiles()), dim3(256), 0, st,
                                d.partials, d.S, d.nt, d.count, s->sum + d.first, s->sumsq + d.first,
                                s->scale ? s->scale + d.first : nullptr, g.out, g.ld);
         } else {
-            hipLaunchKernelGGL(store_reduce_rect_kernel, dim3(kTileS * kTileS / 256, g.tiles()), dim3(256), 0, st,
+            hipLaunchKernelGGL(store_reduce_rect_kernel, dim3(kTileS * kTileS / 1024, g.tiles()), dim3(256), 0, st,
                                d.partials, d.S, d.ntb, d.count, d.count_b, s->sum + d.first, s->sumsq + d.first,
                                s->sum + d.first_b, s->sumsq + d.first_b, s->scale ? s->scale + d.first : nullptr,
                                s->scale ? s->scale + d.first_b : nullptr, g.out, g.ld);
