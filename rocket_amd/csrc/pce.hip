@@ -152,6 +152,10 @@ __global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h
 }
 
 constexpr int kMaxStrip = 16;
+// Values of one column-pass strip (64 KiB of LDS, two workgroups per CU): a
+// strip of 1024-row columns is 8 wide, so its rows are read in 64-byte
+// pieces; the columns are transformed kFftMultiMax values at a time.
+constexpr int kPceStripMax = 8192;
 
 // Columns [j0, j0 + sw) of couple blockIdx.y: rows of the strip are read
 // contiguously into column-major LDS, the columns are inverse-transformed
@@ -179,8 +183,10 @@ __global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_cols2_kernel(Fft
     __syncthreads();
     if constexpr (BIG)
         fft_lds<1>(buf, d);  // one column longer than kFftMultiMax (sw == 1)
-    else
-        fft_lds_multi<1>(buf, d, ncol);
+    else {
+        const int per = max(1, kFftMultiMax / h);  // columns per multi-transform call
+        for (int c0 = 0; c0 < ncol; c0 += per) fft_lds_multi<1>(buf + c0 * h, d, min(per, ncol - c0));
+    }
     const int halves = 2 * q + 1 < nb ? 2 : 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int c = 0; c < ncol; ++c) {
@@ -293,14 +299,15 @@ __global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restri
     }
 }
 
-// Columns per strip of the column pass: a power of two with the transforms
-// of one workgroup (<= kFftMultiMax values, 32 KiB) transformed together; a
-// column longer than that runs alone through fft_lds (the BIG kernel, up to
-// kFftMaxN).  The row pass runs one row per workgroup through fft_row (rows
+// Columns per strip of the column pass: a power of two with the strip's
+// values (<= kPceStripMax, 64 KiB) in one workgroup's LDS, transformed
+// kFftMultiMax values at a time; a column longer than kFftMultiMax runs alone
+// through fft_lds (the BIG kernel, up to kFftMaxN).  The row pass runs one row per workgroup through fft_row (rows
 // grouped per workgroup measured slower: 184 VGPRs, 2 waves per SIMD).
 int pce_strip(int h) {
     int sw = kMaxStrip;
-    while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
+    if (h > kFftMultiMax) return 1;
+    while (sw > 1 && (size_t)sw * h > (size_t)kPceStripMax) sw >>= 1;
     return sw;
 }
 
