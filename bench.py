@@ -550,10 +550,14 @@ def jpeg_bench(torch, batch=32, reps=3):
     torch.cuda.synchronize()
 
     reps_conc = 2 * reps
+    errors = []
 
     def work(t):
-        for _ in range(reps_conc):
-            decs[t].decode(files, out=outs[t], stream=streams[t].cuda_stream)
+        try:
+            for _ in range(reps_conc):
+                decs[t].decode(files, out=outs[t], stream=streams[t].cuda_stream)
+        except Exception as e:  # re-raised below: a failed decoder must not pass as a timing
+            errors.append(e)
     threads = [threading.Thread(target=work, args=(t,)) for t in range(ndec)]
     t1 = time.perf_counter()
     for th in threads:
@@ -561,6 +565,8 @@ def jpeg_bench(torch, batch=32, reps=3):
     for th in threads:
         th.join()
     torch.cuda.synchronize()
+    if errors:
+        raise errors[0]
     dt_conc = (time.perf_counter() - t1) / (reps_conc * ndec)
     for d in decs:
         d.close()
