@@ -581,6 +581,20 @@ def jpeg_bench(torch, batch=32, reps=3):
         f.apply_gpu_batch(rgb.view(batch, -1), pat, _lib.INPUT_RGB8, sync=False)
     torch.cuda.synchronize()
     dt_pipe = (time.perf_counter() - t1) / reps
+    # the same through a pool of three decoders: batches decode on worker
+    # streams while earlier ones are extracted on this one
+    pool = jpeg.JpegDecoderPool(workers=3)
+    nb = 4 * reps
+    for out, _ in pool.map([files] * 3, shape=(HEIGHT, WIDTH, 3)):   # warm-up: each worker's decoder
+        out.record_stream(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for out, _ in pool.map([files] * nb, shape=(HEIGHT, WIDTH, 3)):
+        out.record_stream(torch.cuda.current_stream())
+        f.apply_gpu_batch(out.view(batch, -1), pat, _lib.INPUT_RGB8, sync=False)
+    torch.cuda.synchronize()
+    dt_pool = (time.perf_counter() - t1) / nb
+    pool.close()
     f.cleanup()
     dec.close()
     t1 = time.perf_counter()
@@ -593,6 +607,7 @@ def jpeg_bench(torch, batch=32, reps=3):
             "decode_images_per_s_3_decoders": batch / dt_conc,
             "decode_compressed_MBps": batch * mb / dt, "bitexact_vs_libjpeg": exact,
             "files_to_ncc_patterns_images_per_s": batch / dt_pipe,
+            "files_to_ncc_patterns_images_per_s_decoder_pool": batch / dt_pool,
             "cpu_libjpeg_turbo_pillow_images_per_s_1_thread": 1.0 / cpu}
 
 
@@ -897,7 +912,8 @@ def main():
         jpg = jpeg_bench(torch)
         log(f"JPEG: {jpg['decode_images_per_s']:.0f} images/s decoded "
             f"({jpg['decode_images_per_s_3_decoders']:.0f} with three decoders on three streams), "
-            f"{jpg['files_to_ncc_patterns_images_per_s']:.0f} images/s files -> NCC patterns, "
+            f"{jpg['files_to_ncc_patterns_images_per_s']:.0f} images/s files -> NCC patterns "
+            f"({jpg['files_to_ncc_patterns_images_per_s_decoder_pool']:.0f} with a pool of three decoders), "
             f"libjpeg-turbo one thread {jpg['cpu_libjpeg_turbo_pillow_images_per_s_1_thread']:.0f} images/s")
 
     log("c0 accuracy (64 frames of 512x512, all pairs) ...")

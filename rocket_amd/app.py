@@ -105,19 +105,20 @@ def correlation_matrix_jpeg(paths, strategy: str = "NCC", precision: str = "spli
     pce = strategy.upper() == "PCE"
     filt = csi.PRNUFilter(h, w, apply_spectral_filter=pce)
     pats = torch.empty((n, filt.get_output_size() // 4), device="cuda")
-    dec = jpeg.JpegDecoder()
-    rgb = torch.empty((min(batch, n), h, w, 3), dtype=torch.uint8, device="cuda")
+    pool = jpeg.JpegDecoderPool(workers=3)      # batches decode while earlier ones are extracted
     try:
-        for b in range(0, n, batch):
-            m = min(batch, n - b)
-            out, status = dec.decode(data[b:b + m], out=rgb[:m], strict=False)
+        starts = list(range(0, n, batch))
+        decoded = pool.map((data[b:b + batch] for b in starts), shape=(h, w, 3), strict=False)
+        for b, (out, status) in zip(starts, decoded):
+            out.record_stream(torch.cuda.current_stream())   # decoded on a worker stream, read on this one
+            m = len(status)
             for k, st in enumerate(status):
                 if st not in (_lib.OK, _lib.JPEG_CORRUPT_DATA):   # libjpeg only warns on corrupt data
                     raise RuntimeError(f"error while reading: {paths[b + k]}")
             filt.apply_gpu_batch(out.view(m, -1), pats[b:b + m], _lib.INPUT_RGB8)
         corrs = _compare(pats, n, h, w, filt, pce, precision, pce_variant)
     finally:
-        dec.close()
+        pool.close()
         filt.cleanup()
     return corrs
 

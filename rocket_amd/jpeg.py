@@ -77,3 +77,63 @@ class JpegDecoder:
             self.close()
         except Exception:
             pass
+
+
+class JpegDecoderPool:
+    """Several decoders, each with its own stream on its own worker thread, so
+    that one call's host work (parsing, unstuffing, the copy, the checks
+    between linking rounds) overlaps the kernels of the others.  ``map``
+    decodes a sequence of batches and yields (frames, status) in order; each
+    batch is complete on the device when it is yielded (every decode call is
+    synchronous on its stream)."""
+
+    def __init__(self, workers: int = 3):
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+        if workers < 1:
+            raise ValueError("workers must be positive")
+        self._local = threading.local()
+        self._all = []
+        self._lock = threading.Lock()
+        self._pool = ThreadPoolExecutor(max_workers=workers)
+
+    def _worker_state(self):
+        import torch
+        st = getattr(self._local, "state", None)
+        if st is None:
+            st = (JpegDecoder(), torch.cuda.Stream())
+            self._local.state = st
+            with self._lock:
+                self._all.append(st[0])
+        return st
+
+    def _decode(self, files, shape, strict):
+        import torch
+        dec, stream = self._worker_state()
+        with torch.cuda.stream(stream):
+            out = None
+            if shape is not None:
+                out = torch.empty((len(files),) + tuple(shape), dtype=torch.uint8, device="cuda")
+            return dec.decode(files, out=out, stream=stream.cuda_stream, strict=strict)
+
+    def map(self, batches, shape=None, strict: bool = True, ahead: int = 3):
+        """Decode each batch (a list of file bytes) of `batches`; `shape`, when
+        given, is the (H, W, 3) of every frame (else taken from the files, as
+        JpegDecoder.decode does).  At most `ahead` batches are in flight.
+        Frames are allocated on the worker's stream: a consumer on another
+        stream calls ``frames.record_stream(its_stream)`` before using them."""
+        from collections import deque
+        pending = deque()
+        for files in batches:
+            pending.append(self._pool.submit(self._decode, list(files), shape, strict))
+            if len(pending) >= ahead:
+                yield pending.popleft().result()
+        while pending:
+            yield pending.popleft().result()
+
+    def close(self) -> None:
+        self._pool.shutdown(wait=True)
+        with self._lock:
+            for dec in self._all:
+                dec.close()
+            self._all = []

@@ -170,3 +170,20 @@ def test_readjpeg_drop_in_matches_libjpeg():
         assert np.array_equal(got, pil_rgb(data))
     prog = encode(frame(30, 20, 1), progressive=True)
     assert lib.readJPEG(prog, len(prog), ctypes.byref(w), ctypes.byref(h), out, cap) == 1
+
+
+def test_decoder_pool_matches_single_decoder():
+    """JpegDecoderPool: batches come back in order, each equal to one
+    decoder's result, with several batches in flight on worker streams."""
+    files = [encode(frame(96, 64, s), quality=85, subsampling=2) for s in range(12)]
+    single = jpeg.JpegDecoder()
+    want = [single.decode(files[b:b + 3])[0].cpu().numpy() for b in range(0, 12, 3)]
+    single.close()
+    pool = jpeg.JpegDecoderPool(workers=3)
+    got = []
+    for out, st in pool.map((files[b:b + 3] for b in range(0, 12, 3)), shape=(64, 96, 3)):
+        assert st == [0, 0, 0]
+        out.record_stream(torch.cuda.current_stream())
+        got.append(out.cpu().numpy())
+    pool.close()
+    assert len(got) == 4 and all(np.array_equal(g, w) for g, w in zip(got, want))
