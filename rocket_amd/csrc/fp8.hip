@@ -9,6 +9,8 @@
 // element; the FP32 re-score of the serving path removes it for the top hits.
 #include "prnu_internal.hpp"
 
+#include <climits>
+
 #include <algorithm>
 
 namespace prnu {
@@ -212,14 +214,16 @@ __global__ __launch_bounds__(kThreads8, 1) void fp8_tile_kernel(Fp8Args a) {
 }
 
 // Sum the slabs in FP64, undo the scales and apply the NCC epilogue
-// G / (sqrt(Sxx_i - Sx_i^2) sqrt(Sxx_j - Sx_j^2)).  grid (AR*256/256, T).
+// G / (sqrt(Sxx_i - Sx_i^2) sqrt(Sxx_j - Sx_j^2)).  Four consecutive elements
+// per thread (one 16-byte load per slab, each element summed in slab order).
+// grid (AR*256/1024, T).
 template <int AR>
 __global__ __launch_bounds__(256) void fp8_reduce_kernel(Fp8Args a, const float *scale_a, const float *scale_b,
                                                          const double *sum_a, const double *sumsq_a,
                                                          const double *sum_b, const double *sumsq_b, double *out,
                                                          int64_t ld_out) {
     const int t = blockIdx.y;
-    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int e = (blockIdx.x * 256 + threadIdx.x) * 4;
     int I, J;
     if (a.tri) {
         const int2 ij = tri_tile8(t, a.nta);
@@ -230,18 +234,29 @@ __global__ __launch_bounds__(256) void fp8_reduce_kernel(Fp8Args a, const float 
         J = t % a.ntb;
     }
     const int i = I * AR + e / kBTile;
-    const int j = J * kBTile + e % kBTile;
-    if (i >= a.ma || j >= a.nb) return;
-    if (a.tri && I == J && i > j) return;
-    const float *p = a.partials + (int64_t)t * a.S * (AR * kBTile) + e;
-    double g = 0.0;
-    for (int q = 0; q < a.S; ++q) g += (double)p[(int64_t)q * (AR * kBTile)];
-    g *= (double)scale_a[i] * (double)scale_b[j];
+    const int j0 = J * kBTile + e % kBTile;
+    if (i >= a.ma || j0 >= a.nb) return;
+    if (a.tri && I == J && i > j0 + 3) return;
+    const float4 *p = reinterpret_cast<const float4 *>(a.partials + (int64_t)t * a.S * (AR * kBTile) + e);
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int q = 0; q < a.S; ++q) {
+        const float4 v = p[(int64_t)q * (AR * kBTile / 4)];
+        g[0] += (double)v.x;
+        g[1] += (double)v.y;
+        g[2] += (double)v.z;
+        g[3] += (double)v.w;
+    }
     const double di = sqrt(sumsq_a[i] - sum_a[i] * sum_a[i]);
-    const double dj = sqrt(sumsq_b[j] - sum_b[j] * sum_b[j]);
-    const double v = g / (di * dj);
-    out[(int64_t)i * ld_out + j] = v;
-    if (a.tri) out[(int64_t)j * ld_out + i] = v;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int j = j0 + c;
+        if (j >= a.nb || (a.tri && I == J && i > j)) continue;
+        const double dj = sqrt(sumsq_b[j] - sum_b[j] * sum_b[j]);
+        const double v = g[c] * ((double)scale_a[i] * (double)scale_b[j]) / (di * dj);
+        out[(int64_t)i * ld_out + j] = v;
+        if (a.tri) out[(int64_t)j * ld_out + i] = v;
+    }
 }
 
 }  // namespace
@@ -256,6 +271,27 @@ int launch_quantize_fp8(const float *const *d_ptrs, int count, int64_t n, const 
     hipLaunchKernelGGL(quantize_fp8_kernel, grid, dim3(256), 0, st, d_ptrs, n, stride, d_absmax, d_q, d_scale);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
+}
+
+// Workgroups of fp8_tile_kernel<a_rows> the device runs at once (per device,
+// queried once).
+int64_t fp8_slots(int a_rows) {
+    static int64_t cached[16][2] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const int v = a_rows == 256 ? 0 : 1;
+    if (dev < 0 || dev >= 16) dev = 0;
+    if (!cached[dev][v]) {
+        int per_cu = 0, cus = 0;
+        if (a_rows == 256)
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp8_tile_kernel<256>, kThreads8, 0);
+        else
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp8_tile_kernel<64>, kThreads8, 0);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipGetLastError();
+        cached[dev][v] = (int64_t)std::max(1, per_cu) * std::max(1, cus);
+    }
+    return cached[dev][v];
 }
 
 // C = A B^T in FP8 with NCC epilogue.  tri: B is A and only I <= J tiles run.
@@ -280,10 +316,27 @@ int launch_fp8_ncc(Workspace &ws, int a_rows, bool tri, const uint8_t *A, int64_
     PRNU_REQUIRE(!tri || a_rows == kBTile, "fp8: the triangle needs square tiles");
     a.T = tri ? a.nta * (a.nta + 1) / 2 : a.nta * a.ntb;
     const int64_t stages = K / kKT8;
-    int64_t S = slabs > 0 ? slabs : std::max<int64_t>(1, (256 * 4 + a.T - 1) / a.T);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, stages / 8));
     const int64_t tile_bytes = (int64_t)a_rows * kBTile * sizeof(float);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t(2) << 30) / (tile_bytes * a.T)));
+    const int64_t s_max = std::max<int64_t>(
+        1, std::min<int64_t>(std::min<int64_t>(stages / 8, 1024), (int64_t(2) << 30) / (tile_bytes * a.T)));
+    int64_t S;
+    if (slabs > 0) {
+        S = std::min<int64_t>(slabs, s_max);
+    } else {
+        // the slab count minimising dispatch rounds x slab length, with the
+        // workgroups the device runs at once (a function of the tile count,
+        // K and the device only: a query's scores do not depend on its batch)
+        const int64_t slots = fp8_slots(a_rows);
+        int64_t best = INT64_MAX;
+        S = 1;
+        for (int64_t c = 1; c <= s_max; ++c) {
+            const int64_t cost = ((a.T * c + slots - 1) / slots) * ((stages + c - 1) / c);
+            if (cost < best) {
+                best = cost;
+                S = c;
+            }
+        }
+    }
     a.kchunk = ((stages + S - 1) / S) * kKT8;
     a.S = (int)((K + a.kchunk - 1) / a.kchunk);
     a.xcd_remap = 1;
@@ -297,7 +350,7 @@ int launch_fp8_ncc(Workspace &ws, int a_rows, bool tri, const uint8_t *A, int64_
         hipLaunchKernelGGL(fp8_tile_kernel<64>, grid, dim3(kThreads8), 0, st, a);
     PRNU_CHECK_LAUNCH();
     if (timing_ctx) time_end(timing_ctx, st);
-    const dim3 rgrid((unsigned)(a_rows * kBTile / 256), (unsigned)a.T);
+    const dim3 rgrid((unsigned)(a_rows * kBTile / 1024), (unsigned)a.T);
     if (a_rows == 256)
         hipLaunchKernelGGL(fp8_reduce_kernel<256>, rgrid, dim3(256), 0, st, a, scale_a, tri ? scale_a : scale_b,
                            sum_a, sumsq_a, tri ? sum_a : sum_b, tri ? sumsq_a : sumsq_b, out, ld_out);
