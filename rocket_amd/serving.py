@@ -15,7 +15,14 @@ fingerprints by NCC:
   batch (``prnu_preprocess_batch``, one launch per filter stage), then
   ``prnu_fpstore_query`` (FP8 MFMA scan, top-k, exact FP32 re-score), then
   device->host copy of the k results;
-* a completion thread waits for each batch's event and resolves its requests.
+* a completion thread waits for each batch's event and resolves its requests;
+* host-side pauses are kept off the request path: the pinned staging buffers
+  are written once when a slot is made (a first write into fresh pinned pages
+  faults every page in, 15-20 ms for a batch of 1-megapixel images), and
+  while the server runs the objects alive at ``start()`` are frozen out of
+  Python's cyclic collector (``gc.freeze``), so a full collection walks only
+  what was made since instead of the whole interpreter (a full collection of
+  a process holding torch took 34 ms).
 
 A query's result does not depend on which other queries share its batch
 (``prnu_fpstore_query`` is per-query by construction), so results are the
@@ -24,6 +31,7 @@ same alone and in any mix.
 from __future__ import annotations
 
 import ctypes
+import gc
 import queue
 import threading
 import time
@@ -124,16 +132,43 @@ class _Slot:
         self.h_images = torch.empty((max_batch, n), dtype=torch.uint8, pin_memory=True)
         self.h_idx = torch.empty((max_batch, k), dtype=torch.int32, pin_memory=True)
         self.h_score = torch.empty((max_batch, k), dtype=torch.float64, pin_memory=True)
+        for buf in (self.h_images, self.h_idx, self.h_score):
+            buf.zero_()                 # fault the pinned pages in now, not under the first batch
         self.event = torch.cuda.Event()
         self.busy = False
         self.requests: list[Request] = []
 
 
+_gc_lock = threading.Lock()
+_gc_servers = 0                 # running servers that asked for gc.freeze
+
+
+def _gc_hold() -> None:
+    global _gc_servers
+    with _gc_lock:
+        if _gc_servers == 0:
+            gc.collect()
+            gc.freeze()
+        _gc_servers += 1
+
+
+def _gc_release() -> None:
+    global _gc_servers
+    with _gc_lock:
+        _gc_servers -= 1
+        if _gc_servers == 0:
+            gc.unfreeze()
+
+
 class Server:
-    """Continuous-batching query server over a :class:`FingerprintStore`."""
+    """Continuous-batching query server over a :class:`FingerprintStore`.
+
+    ``freeze_gc``: from ``start()`` to ``stop()`` the objects alive at start
+    are moved out of the cyclic collector's reach (``gc.freeze``; shared by
+    all running servers of the process, undone when the last one stops)."""
 
     def __init__(self, store: FingerprintStore, k: int = 10, max_batch: int = 64, slots: int = 2,
-                 hbm_reserve: int = 2 << 30):
+                 hbm_reserve: int = 2 << 30, freeze_gc: bool = True):
         self.store = store
         self.k = int(k)
         self.h, self.w = store.height, store.width
@@ -157,6 +192,8 @@ class Server:
         self._sched = threading.Thread(target=self._schedule, daemon=True)
         self._done = threading.Thread(target=self._complete, daemon=True)
         self.batches: list[int] = []
+        self.freeze_gc = bool(freeze_gc)
+        self._frozen = False
 
     @staticmethod
     def mem_info() -> tuple[int, int]:
@@ -188,6 +225,9 @@ class Server:
                                                   self.max_batch, self.k, slot.idx.data_ptr(),
                                                   slot.score.data_ptr(), None, st))
             slot.stream.synchronize()
+        if self.freeze_gc:
+            _gc_hold()
+            self._frozen = True
         self._sched.start()
         self._done.start()
         return self
@@ -197,6 +237,9 @@ class Server:
         self._sched.join()
         self.inflight.put(None)
         self._done.join()
+        if self._frozen:
+            self._frozen = False
+            _gc_release()
 
     # ------------------------------------------------------------- scheduler
     def _admit(self) -> list[Request]:

@@ -7,6 +7,7 @@
 * a query's result is bitwise the same alone and inside any batch;
 * the server returns, for every request, the result of that query alone.
 """
+import gc
 import threading
 import time
 
@@ -117,6 +118,7 @@ def test_server_continuous_batching_matches_alone():
     store.put(fp)
     queries = images[:40]                          # cameras 24..31 are not enrolled
     srv = serving.Server(store, k=5, max_batch=16, slots=2).start()
+    assert gc.get_freeze_count() > 0                # serving objects kept out of full collections
     reqs = []
 
     def client():
@@ -135,6 +137,7 @@ def test_server_continuous_batching_matches_alone():
     for r in reqs:
         assert r.wait(60), "request not answered"
     srv.stop()
+    assert gc.get_freeze_count() == 0
     assert len(srv.batches) >= 2 and max(srv.batches) > 1          # really batched
     ctx = csi._Ctx(h, w, _lib.MODE_NCC)
     pat = torch.empty(h * w, device="cuda")
