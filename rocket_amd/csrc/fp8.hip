@@ -213,6 +213,153 @@ __global__ __launch_bounds__(kThreads8, 1) void fp8_tile_kernel(Fp8Args a) {
                 out[(wrow0 + rb * 16 + orow + r) * kBTile + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
 }
 
+// Query scan: 64 query rows against 256 store rows per tile, the store
+// streamed once per call.  The work per stage is a few MFMAs against 32 KiB
+// of store rows, so the kernel is bound by how many bytes it keeps in flight:
+// a ring of kScanRing LDS stages filled by direct global-to-LDS loads
+// (global_load_lds_dwordx4), kScanRing - 1 of them in flight while one is
+// multiplied.  Each stage starts with an explicit wait for this wave's loads
+// of that stage and an s_barrier in inline asm (a memory clobber, no fence):
+// a workgroup fence would wait for every load in flight, which is what the
+// ring avoids.  Rows past ma / nb
+// hold clamped duplicates; they only feed partial rows and columns the
+// reduction skips.  Same partial layout as fp8_tile_kernel<64>.
+constexpr int kScanRing = 4;
+constexpr int kScanRows = 64 + kBTile;          // LDS rows per stage
+constexpr int kScanBuf = kScanRows * kKT8;      // 40 KiB per stage
+constexpr int kScanLoads = kScanRows * 8 / kThreads8;  // 16-byte loads per thread per stage
+
+__global__ __launch_bounds__(kThreads8, 1) void fp8_scan_kernel(Fp8Args a) {
+    constexpr int RB = 4, CB = 2;  // wave tile: 64 rows x 32 columns (8 waves across the 256 columns)
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kScanRing * kScanBuf];
+
+    const int bid = blockIdx.x;
+    const int local = bid >> 3;
+    const int t = local % a.T;
+    const int s = (local / a.T) * 8 + (bid & 7);
+    if (s >= a.S) return;
+    const int I = t / a.ntb, J = t % a.ntb;
+    const int64_t k_begin = (int64_t)s * a.kchunk;
+    const int64_t k_end = min(a.K, k_begin + a.kchunk);
+    const int nst = (int)((k_end - k_begin) / kKT8);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wcol0 = wave * 32;
+
+    // load `it` of a stage fills LDS rows 64*it + 8*wave .. +8, lane l at
+    // byte 16*l: row 64*it + (tid >> 3), 16-byte slot tid & 7, which holds
+    // chunk (tid & 7) ^ (row & 7) of the row (the swizzle q_off reads)
+    const int r0 = tid >> 3;
+    const int col = ((tid & 7) ^ (r0 & 7)) << 4;
+    const uint8_t *src[kScanLoads];
+#pragma unroll
+    for (int it = 0; it < kScanLoads; ++it) {
+        const int r = r0 + 64 * it;
+        if (r < 64) {
+            const int g = min(I * 64 + r, a.ma - 1);
+            src[it] = a.A + (int64_t)g * a.lda + col + k_begin;
+        } else {
+            const int g = min(J * kBTile + (r - 64), a.nb - 1);
+            src[it] = a.B + (int64_t)g * a.ldb + col + k_begin;
+        }
+    }
+    auto issue = [&](int q) {
+        uint8_t *buf = lds + (q % kScanRing) * kScanBuf;
+        const int64_t k0 = (int64_t)q * kKT8;
+#pragma unroll
+        for (int it = 0; it < kScanLoads; ++it) {
+            typedef const __attribute__((address_space(1))) void *gptr;
+            typedef __attribute__((address_space(3))) void *lptr;
+            __builtin_amdgcn_global_load_lds((gptr)(src[it] + k0), (lptr)(buf + (64 * it + 8 * wave) * kKT8), 16, 0,
+                                             0);
+        }
+    };
+
+    floatx4 acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+    for (int d = 0; d < kScanRing - 1; ++d)
+        if (d < nst) issue(d);
+    const int frow = lane & 15, ch = (lane >> 4) * 2;
+    const uint32_t lds_base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t *)lds;
+    const uint32_t b_off_lo = (uint32_t)q_off(64 + wcol0 + frow, ch), b_off_hi = (uint32_t)q_off(64 + wcol0 + frow, ch + 1);
+    const uint32_t a_off_lo = (uint32_t)q_off(frow, ch), a_off_hi = (uint32_t)q_off(frow, ch + 1);
+    for (int q = 0; q < nst; ++q) {
+        // stages issued past q: min(nst - 1 - q, kScanRing - 2); wait until
+        // only those are outstanding, then every wave's stage q has landed
+        // and every wave is done reading the buffer stage q + 3 will fill
+        const int ahead = min(nst - 1 - q, kScanRing - 2);
+        static_assert(kScanRing == 4 && kScanLoads == 5, "the waits below count 5 loads per stage");
+        // s_waitcnt immediates: vmcnt in bits 3:0, expcnt 6:4 (7 = no wait),
+        // lgkmcnt 11:8 (0 = this wave's LDS reads of the last stage are done)
+        if (ahead >= 2)
+            __builtin_amdgcn_s_waitcnt(0x007A);  // vmcnt(10) lgkmcnt(0)
+        else if (ahead == 1)
+            __builtin_amdgcn_s_waitcnt(0x0075);  // vmcnt(5) lgkmcnt(0)
+        else
+            __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+        asm volatile("s_barrier" ::: "memory");
+        if (q + kScanRing - 1 < nst) issue(q + kScanRing - 1);
+        // this wave's fragments of the stage: B rows 64 + wcol0 + {0, 16} + frow
+        // and A rows {0, 16, 32, 48} + frow, chunks ch and ch + 1.  Rows 16
+        // apart are 2048 bytes apart under the swizzle (16 is a multiple of
+        // 8), so four addresses and immediate offsets cover the 12 reads.
+        // They are one asm block ending in lgkmcnt(0): as plain LDS loads the
+        // compiler would wait for every LDS fill in flight before them.
+        const uint32_t sb = lds_base + (uint32_t)((q % kScanRing) * kScanBuf);
+        const uint32_t b_lo = sb + b_off_lo, b_hi = sb + b_off_hi, a_lo = sb + a_off_lo, a_hi = sb + a_off_hi;
+        int4 f[12];
+        asm volatile(
+            "ds_read_b128 %0, %12\n\t"
+            "ds_read_b128 %1, %13\n\t"
+            "ds_read_b128 %2, %12 offset:2048\n\t"
+            "ds_read_b128 %3, %13 offset:2048\n\t"
+            "ds_read_b128 %4, %14\n\t"
+            "ds_read_b128 %5, %15\n\t"
+            "ds_read_b128 %6, %14 offset:2048\n\t"
+            "ds_read_b128 %7, %15 offset:2048\n\t"
+            "ds_read_b128 %8, %14 offset:4096\n\t"
+            "ds_read_b128 %9, %15 offset:4096\n\t"
+            "ds_read_b128 %10, %14 offset:6144\n\t"
+            "ds_read_b128 %11, %15 offset:6144\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(f[0]), "=&v"(f[1]), "=&v"(f[2]), "=&v"(f[3]), "=&v"(f[4]), "=&v"(f[5]), "=&v"(f[6]),
+              "=&v"(f[7]), "=&v"(f[8]), "=&v"(f[9]), "=&v"(f[10]), "=&v"(f[11])
+            : "v"(b_lo), "v"(b_hi), "v"(a_lo), "v"(a_hi)
+            : "memory");
+        intx8 bf[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            const int4 lo = f[2 * cb], hi = f[2 * cb + 1];
+            bf[cb] = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int4 lo = f[4 + 2 * rb], hi = f[5 + 2 * rb];
+            const intx8 af = intx8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af, bf[cb], acc[rb][cb], 0, 0, 0, 127,
+                                                                               0, 127);
+        }
+    }
+
+    float *out = a.partials + ((int64_t)t * a.S + s) * (64 * kBTile);
+    const int ocol = lane & 15, orow = (lane >> 4) * 4;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(rb * 16 + orow + r) * kBTile + wcol0 + cb * 16 + ocol] = acc[rb][cb][r];
+}
+
 // Sum the slabs in FP64, undo the scales and apply the NCC epilogue
 // G / (sqrt(Sxx_i - Sx_i^2) sqrt(Sxx_j - Sx_j^2)).  Four consecutive elements
 // per thread (one 16-byte load per slab, each element summed in slab order).
@@ -273,8 +420,8 @@ int launch_quantize_fp8(const float *const *d_ptrs, int count, int64_t n, const 
     return PRNU_OK;
 }
 
-// Workgroups of fp8_tile_kernel<a_rows> the device runs at once (per device,
-// queried once).
+// Workgroups of the a_rows kernel (fp8_tile_kernel<256> or fp8_scan_kernel)
+// the device runs at once (per device, queried once).
 int64_t fp8_slots(int a_rows) {
     static int64_t cached[16][2] = {};
     int dev = 0;
@@ -286,7 +433,7 @@ int64_t fp8_slots(int a_rows) {
         if (a_rows == 256)
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp8_tile_kernel<256>, kThreads8, 0);
         else
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp8_tile_kernel<64>, kThreads8, 0);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp8_scan_kernel, kThreads8, 0);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipGetLastError();
         cached[dev][v] = (int64_t)std::max(1, per_cu) * std::max(1, cus);
@@ -347,7 +494,7 @@ int launch_fp8_ncc(Workspace &ws, int a_rows, bool tri, const uint8_t *A, int64_
     if (a_rows == 256)
         hipLaunchKernelGGL(fp8_tile_kernel<256>, grid, dim3(kThreads8), 0, st, a);
     else
-        hipLaunchKernelGGL(fp8_tile_kernel<64>, grid, dim3(kThreads8), 0, st, a);
+        hipLaunchKernelGGL(fp8_scan_kernel, grid, dim3(kThreads8), 0, st, a);
     PRNU_CHECK_LAUNCH();
     if (timing_ctx) time_end(timing_ctx, st);
     const dim3 rgrid((unsigned)(a_rows * kBTile / 1024), (unsigned)a.T);

@@ -78,6 +78,38 @@ def test_query_topk_and_exact_scores(small_store):
     ctx.close()
 
 
+@pytest.mark.parametrize("h,w", [(8, 16), (16, 16), (16, 24)])
+def test_scan_with_one_to_three_k_steps(h, w):
+    """Fingerprints of 128, 256 and 384 values: one, two and three FP8 K steps
+    per row, so every slab of the scan ends before its load ring fills.  300
+    stored (a second, clamped store tile) and 70 queries (a second, ragged
+    query block); scan within the e4m3 bound of the exact NCC, owners found,
+    and each query's scores the same alone."""
+    n, cap = h * w, 300
+    fp = fingerprints(cap, n, seed=21 + n)
+    store = serving.FingerprintStore(h, w, cap)
+    store.put(fp)
+    owners = [(7 * i) % cap if i % 3 else -1 for i in range(70)]
+    q = queries_from(fp, owners, seed=31 + n, gain=2.0)
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    idx, score, scan = (t.cpu().numpy() for t in store.query(ctx, q, k=5, scan=True))
+    qh, fh = q.cpu().numpy(), fp.cpu().numpy()
+    exact = O.ncc_matrix(np.concatenate([qh, fh]))[: len(owners), len(owners):]
+    # e4m3 rounding: relative error uniform within +-2^-4 per factor (sigma
+    # 0.036); for a query made mostly of its owner's pattern the NCC error has
+    # sigma ~ sqrt(2) * 0.036 * sqrt(3 / n) (fourth moment of a Gaussian), the
+    # largest of any entry; 6 sigma
+    assert np.abs(scan - exact).max() <= 6 * np.sqrt(2) * 0.036 * np.sqrt(3.0 / n) + 2e-3
+    for r, o in enumerate(owners):
+        if o >= 0:
+            assert idx[r, 0] == o, f"query {r}"
+    for r in (0, 63, 64, 69):
+        idx1, sc1 = (t.cpu().numpy() for t in store.query(ctx, q[r:r + 1], k=5))
+        assert np.array_equal(idx1[0], idx[r]) and np.array_equal(sc1[0], score[r])
+    ctx.close()
+    store.close()
+
+
 def test_same_alone_and_in_mix(small_store):
     h, w, fp, store = small_store
     owners = list(range(0, 300, 7)) + [-1] * 5
