@@ -470,14 +470,17 @@ int launch_fp8_ncc(Workspace &ws, int a_rows, bool tri, const uint8_t *A, int64_
     if (slabs > 0) {
         S = std::min<int64_t>(slabs, s_max);
     } else {
-        // the slab count minimising dispatch rounds x slab length, with the
+        // the slab count minimising dispatch rounds x (slab length + a fixed
+        // cost per slab: filling the load pipeline, writing the partial tile
+        // and reading it back, about kSlabOverhead stages), with the
         // workgroups the device runs at once (a function of the tile count,
         // K and the device only: a query's scores do not depend on its batch)
+        constexpr int64_t kSlabOverhead = 8;
         const int64_t slots = fp8_slots(a_rows);
         int64_t best = INT64_MAX;
         S = 1;
         for (int64_t c = 1; c <= s_max; ++c) {
-            const int64_t cost = ((a.T * c + slots - 1) / slots) * ((stages + c - 1) / c);
+            const int64_t cost = ((a.T * c + slots - 1) / slots) * ((stages + c - 1) / c + kSlabOverhead);
             if (cost < best) {
                 best = cost;
                 S = c;
