@@ -120,36 +120,106 @@ __global__ __launch_bounds__(256) void topk_kernel(const double *scores, int64_t
     }
 }
 
-// Exact NCC of (query q, candidate r): grid (k, nq), block 256.
-__global__ __launch_bounds__(256) void rescore_kernel(const float *const *qptrs, const double *qsum,
-                                                      const double *qsumsq, const float *f32, int64_t n,
-                                                      const double *fsum, const double *fsumsq, const int *idx,
-                                                      int k, double *score) {
-    __shared__ double sh[4];
-    const int q = blockIdx.y, r = blockIdx.x;
+// Exact NCC of each query with its k candidates, in two passes over a fixed
+// split of the n values into slices (a function of n only, so a query's
+// scores do not depend on its batch).  Pass 1, grid (slices, nq), block 256:
+// a block reads its slice of the query once and the same slice of every
+// candidate, FP32 products summed in FP64 (16-byte loads where every row is
+// 16-byte aligned), and writes one partial per candidate.  Pass 2, grid nq:
+// the partials of each candidate summed in slice order, then the NCC.
+constexpr int64_t kRescoreSliceMin = 16384;  // values per slice, at least
+
+int64_t rescore_slices(int64_t n) { return std::min<int64_t>(256, std::max<int64_t>(1, n / kRescoreSliceMin)); }
+
+int64_t rescore_slice_len(int64_t n) {
+    const int64_t sl = rescore_slices(n);
+    return ((n + sl - 1) / sl + 3) / 4 * 4;
+}
+
+__global__ __launch_bounds__(256) void rescore_slice_kernel(const float *const *qptrs, const float *f32, int64_t n,
+                                                            int64_t slice_len, const int *idx, int k,
+                                                            double *part) {
+    __shared__ double sh[4][kMaxK];
+    const int sl = blockIdx.x, slices = gridDim.x, q = blockIdx.y;
+    const int64_t lo = (int64_t)sl * slice_len, hi = min(n, lo + slice_len);
+    const float *x = qptrs[q];
+    const bool vec = ((uintptr_t)x & 15) == 0 && (n & 3) == 0 && ((uintptr_t)f32 & 15) == 0;
+    // All kMaxK slots run, branch-free: a slot past k or an absent candidate
+    // (index < 0) re-reads the query row (cache hits) and is not written;
+    // the final pass writes -1e300 for an absent one.  Global-address-space
+    // pointers, so the loads are global_load (the tables hold generic ones).
+    typedef float fv4 __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(1))) float *gf;
+    typedef const __attribute__((address_space(1))) fv4 *gf4;
+    const gf xg = (gf)x;
+    gf y[kMaxK];
+    double s[kMaxK];
+#pragma unroll
+    for (int r = 0; r < kMaxK; ++r) {
+        const int c = r < k ? idx[q * k + r] : -1;
+        y[r] = c >= 0 ? (gf)(f32 + (int64_t)c * n) : xg;
+        s[r] = 0.0;
+    }
+    if (vec) {
+        for (int64_t i = lo + 4 * threadIdx.x; i < hi; i += 1024) {
+            const fv4 xv = *(gf4)(xg + i);
+            fv4 yv[kMaxK];
+#pragma unroll
+            for (int r = 0; r < kMaxK; ++r) yv[r] = *(gf4)(y[r] + i);
+#pragma unroll
+            for (int r = 0; r < kMaxK; ++r) {
+                const float p0 = xv.x * yv[r].x, p1 = xv.y * yv[r].y, p2 = xv.z * yv[r].z, p3 = xv.w * yv[r].w;
+                s[r] += (double)p0;
+                s[r] += (double)p1;
+                s[r] += (double)p2;
+                s[r] += (double)p3;
+            }
+        }
+    } else {
+        for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
+            const float xv = xg[i];
+            float yv[kMaxK];
+#pragma unroll
+            for (int r = 0; r < kMaxK; ++r) yv[r] = y[r][i];
+#pragma unroll
+            for (int r = 0; r < kMaxK; ++r) {
+                const float p = xv * yv[r];
+                s[r] += (double)p;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < kMaxK; ++r) {
+        double v = s[r];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sh[wave][r] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < k)
+        part[((int64_t)q * k + threadIdx.x) * slices + sl] =
+            (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+}
+
+// grid nq, block 64: thread r < k finishes candidate r of query blockIdx.x.
+__global__ __launch_bounds__(64) void rescore_final_kernel(const double *part, int slices, const double *qsum,
+                                                           const double *qsumsq, const double *fsum,
+                                                           const double *fsumsq, const int *idx, int k,
+                                                           double *score) {
+    const int q = blockIdx.x, r = threadIdx.x;
+    if (r >= k) return;
     const int c = idx[q * k + r];
     if (c < 0) {
-        if (threadIdx.x == 0) score[q * k + r] = -1e300;
+        score[q * k + r] = -1e300;
         return;
     }
-    const float *x = qptrs[q];
-    const float *y = f32 + (int64_t)c * n;
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        const float p = x[i] * y[i];  // FP32 product, FP64 sum
-        s += p;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double sxy = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-        const double dx = sqrt(qsumsq[q] - qsum[q] * qsum[q]);
-        const double dy = sqrt(fsumsq[c] - fsum[c] * fsum[c]);
-        score[q * k + r] = sxy / (dx * dy);
-    }
+    const double *p = part + ((int64_t)q * k + r) * slices;
+    double sxy = 0.0;
+    for (int i = 0; i < slices; ++i) sxy += p[i];
+    const double dx = sqrt(qsumsq[q] - qsum[q] * qsum[q]);
+    const double dy = sqrt(fsumsq[c] - fsum[c] * fsum[c]);
+    score[q * k + r] = sxy / (dx * dy);
 }
 
 // Sort each query's k (score, index) by descending exact score (ties: index).
@@ -245,7 +315,9 @@ int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, i
     const size_t b_stats = a256(sizeof(double) * nq) * 2 + a256(sizeof(float) * nq);
     const size_t b_q = a256((size_t)s->stride * nq) + a256(sizeof(float) * nq);
     const size_t b_scan = d_scan ? 0 : a256(sizeof(double) * (size_t)nq * N);
-    PRNU_TRY(c->ws_fp8.ensure(b_stats + b_q + b_scan));
+    const int64_t slices = rescore_slices(s->n);
+    const size_t b_part = a256(sizeof(double) * (size_t)nq * k * slices);
+    PRNU_TRY(c->ws_fp8.ensure(b_stats + b_q + b_scan + b_part));
     char *p = static_cast<char *>(c->ws_fp8.ptr);
     double *qsum = reinterpret_cast<double *>(p);
     double *qsumsq = reinterpret_cast<double *>(p + a256(sizeof(double) * nq));
@@ -253,6 +325,7 @@ int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, i
     uint8_t *qq = reinterpret_cast<uint8_t *>(p + b_stats);
     float *qscale = reinterpret_cast<float *>(p + b_stats + a256((size_t)s->stride * nq));
     double *scan = d_scan ? d_scan : reinterpret_cast<double *>(p + b_stats + b_q);
+    double *part = reinterpret_cast<double *>(p + b_stats + b_q + b_scan);
 
     PRNU_TRY(launch_pattern_stats(d_qptrs, nq, s->n, qsum, qsumsq, qabs, c->ws_small, st));
     PRNU_TRY(launch_quantize_fp8(d_qptrs, nq, s->n, qabs, qq, qscale, st));
@@ -266,7 +339,10 @@ int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, i
     time_end(c, st);
     hipLaunchKernelGGL(topk_kernel, dim3(nq), dim3(256), 0, st, scan, (int64_t)N, N, k, d_idx);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(rescore_kernel, dim3(k, nq), dim3(256), 0, st, d_qptrs, qsum, qsumsq, s->f32, s->n, s->sum,
+    hipLaunchKernelGGL(rescore_slice_kernel, dim3((unsigned)slices, nq), dim3(256), 0, st, d_qptrs, s->f32, s->n,
+                       rescore_slice_len(s->n), d_idx, k, part);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rescore_final_kernel, dim3(nq), dim3(64), 0, st, part, (int)slices, qsum, qsumsq, s->sum,
                        s->sumsq, d_idx, k, d_score);
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(sort_k_kernel, dim3((nq + 63) / 64), dim3(64), 0, st, d_score, d_idx, nq, k);
