@@ -633,10 +633,12 @@ int col_strip(int h) {
     while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
     return sw;
 }
-// Column transforms by strips when a strip holds at least 4 columns (32-byte
-// row pieces); longer columns go through transposes (measured faster at
-// h = 1536, where a strip would hold 2 columns).
-bool strips_for(int h) { return col_strip(h) >= 4; }
+// Column transforms by strips when a strip holds at least 8 columns (64-byte
+// row pieces, h <= 512); longer columns go through transposes.  Measured at
+// batches of 32: h = 1024 (4-column strips) runs 26.1k -> 37.5k images/s
+// (NCC, 1024 x 1024) through the transposes; 512 x 512 and below are as fast
+// or faster by strips (PCE mode 7-30 % faster).
+bool strips_for(int h) { return col_strip(h) >= 8; }
 
 // Per-image buffers, each an array over the batch with the given stride.
 struct ExtractWs {

@@ -68,7 +68,7 @@ def test_zeromean(h, w, rng):
     f.cleanup()
 
 
-@pytest.mark.parametrize("h,w", SIZES + [(1536, 2048)])
+@pytest.mark.parametrize("h,w", SIZES + [(1536, 2048), (1024, 768)])
 def test_wiener(h, w, rng):
     x = O.zeromean(rng.normal(size=(h, w))).astype(np.float32)
     f = csi.PRNUFilter(h, w)
@@ -99,7 +99,8 @@ def test_spectral(h, w, rng):
     f.cleanup()
 
 
-@pytest.mark.parametrize("h,w,rgb", [(80, 96, True), (96, 80, False), (512, 512, False), (45, 60, True)])
+@pytest.mark.parametrize("h,w,rgb", [(80, 96, True), (96, 80, False), (512, 512, False), (45, 60, True),
+                                   (1024, 1024, False)])
 def test_preprocess_chain(h, w, rgb, rng):
     if rgb:
         img = rng.integers(0, 256, size=(h, w, 3), dtype=np.uint8)
