@@ -363,6 +363,27 @@ def serving_bench(torch, duration: float):
            "mean_batch": float(np.mean(srv.batches)), "max_batch": int(max(srv.batches)),
            "top1_known_camera": float(np.mean(hit)) if hit else None,
            "store": f"{SERVE_STORE} x {SERVE_W}x{SERVE_H} FP8-e4m3 + FP32 originals", "k": 10}
+    # the store scan of one batch of 64 queries, timed by the context's HIP
+    # events (fp8_scan_kernel and its slab reduction); algorithmic bytes: the
+    # store's FP8 rows (n bytes each, n a multiple of 128) and the 64 query rows
+    lib = _lib.load()
+    ctx = csi._Ctx(SERVE_H, SERVE_W, _lib.MODE_NCC)
+    _lib.check(lib.prnu_ctx_set_timing(ctx.handle, 1))
+    qs = torch.randn((64, n), generator=g, device="cuda")
+    qs -= qs.mean(dim=1, keepdim=True)
+    ms, times = ctypes.c_float(), []
+    for _ in range(5):
+        store.query(ctx, qs, k=10)
+        _lib.check(lib.prnu_ctx_last_kernel_ms(ctx.handle, ctypes.byref(ms)))
+        times.append(ms.value)
+    scan_ms = min(times)
+    scan_bytes = (SERVE_STORE + 64) * n
+    res["scan_ms_per_batch_of_64"] = scan_ms
+    res["scan_roofline"] = {"bound": "hbm", "kernel": "fp8_scan_kernel", "achieved": scan_bytes / scan_ms / 1e6,
+                            "peak": 8000.0, "unit": "GB/s", "frac": scan_bytes / scan_ms / 1e6 / 8000.0,
+                            "algorithmic_bytes": scan_bytes}
+    ctx.close()
+    del qs
     store.close()
     return res
 
@@ -929,7 +950,9 @@ def main():
         torch.cuda.empty_cache()
         log("serving ...")
         serve = serving_bench(torch, args.serving_seconds)
-        log(f"serving: p50 {serve['p50_ms']:.1f} ms, p99 {serve['p99_ms']:.1f} ms, {serve['qps']:.0f} q/s")
+        log(f"serving: p50 {serve['p50_ms']:.1f} ms, p99 {serve['p99_ms']:.1f} ms, {serve['qps']:.0f} q/s; "
+            f"store scan of 64 queries {serve['scan_ms_per_batch_of_64']:.2f} ms "
+            f"({serve['scan_roofline']['achieved']:.0f} GB/s, {100 * serve['scan_roofline']['frac']:.1f} % of HBM peak)")
 
     line = {
         "metric": "NCC pairs/s, split-BF16 MFMA vs FP32 at |\u0394NCC|\u22641e-5; query p50/p99 latency",
