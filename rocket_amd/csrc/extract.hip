@@ -633,12 +633,16 @@ int col_strip(int h) {
     while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
     return sw;
 }
-// Column transforms by strips when a strip holds at least 8 columns (64-byte
-// row pieces, h <= 512); longer columns go through transposes.  Measured at
-// batches of 32: h = 1024 (4-column strips) runs 26.1k -> 37.5k images/s
-// (NCC, 1024 x 1024) through the transposes; 512 x 512 and below are as fast
-// or faster by strips (PCE mode 7-30 % faster).
-bool strips_for(int h) { return col_strip(h) >= 8; }
+// Column transforms by strips or through transposes, chosen per pass from
+// batches of 32 measured both ways.  The Wiener pass goes through the
+// transposes above 128 rows: 26.1k -> 37.5k images/s at 1024 x 1024 (4-column
+// strips, 32-byte row pieces), 100k -> 121k at 512 x 512.  The spectrum
+// (PCE mode) keeps strips while a strip holds at least 8 columns (h <= 512):
+// through the transposes its batched output, whose rows are not the plane
+// stride apart, takes one transpose launch per image.
+bool wiener_strips(int h) { return h <= 128; }
+bool spectrum_strips(int h) { return col_strip(h) >= 8; }
+bool needs_transposed(int h) { return !wiener_strips(h) || !spectrum_strips(h); }
 
 // Per-image buffers, each an array over the batch with the given stride.
 struct ExtractWs {
@@ -656,7 +660,7 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 size_t per_image_bytes(int h, int w) {
     const int64_t n = (int64_t)h * w;
     const int tx = (w + kFnTX - 1) / kFnTX, ty = (h + kFnTY - 1) / kFnTY;
-    return (strips_for(h) ? 16 : 24) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
+    return (needs_transposed(h) ? 24 : 16) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
            11 * 256;
 }
 
@@ -670,7 +674,7 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
     e.cp_stride = 2 * (int64_t)w * e.tiles_y;
     e.rp_stride = 2 * (int64_t)h * e.tiles_x;
     const size_t sz[] = {align256(sizeof(float2) * n * nb),
-                         strips_for(h) ? 0 : align256(sizeof(float2) * n * nb),
+                         needs_transposed(h) ? align256(sizeof(float2) * n * nb) : 0,
                          align256(sizeof(float) * n * nb),    align256(sizeof(float) * n * nb),
                          align256(sizeof(double) * h * nb),   align256(sizeof(double) * nb),
                          align256(sizeof(double) * e.cp_stride * nb), align256(sizeof(double) * e.rp_stride * nb),
@@ -767,7 +771,7 @@ int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int nco
 // w/2+1 columns, the variance stencil on the full grid (mirrored reads of
 // the half power spectrum), the symmetrized scale + inverse column FFTs,
 // real inverse row FFTs two rows at a time; columns by strips or through
-// transposes (strips_for).
+// transposes (wiener_strips).
 int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float *out,
                   int64_t out_stride, bool zm, hipStream_t st) {
     const int h = c->h, w = c->w, wh = w / 2 + 1;
@@ -780,7 +784,7 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
     PRNU_CHECK_LAUNCH();
-    if (strips_for(h)) {
+    if (wiener_strips(h)) {
         PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, wh, wh, nb, nullptr, e.P, st)));
         PRNU_TRY(launch_varest(e.P, e.S, h, w, false, nb, e.var, st));
         PRNU_TRY((launch_cols_fft<1, 2>(dh, e.A, n, wh, wh, nb, e.S, nullptr, st, w)));
@@ -809,7 +813,7 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     const int h = c->h, w = c->w, wh = w / 2 + 1;
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    if (strips_for(h)) {
+    if (spectrum_strips(h)) {
         PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, out, out_stride, h);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
