@@ -633,16 +633,11 @@ int col_strip(int h) {
     while (sw > 1 && (size_t)sw * h > (size_t)kFftMultiMax) sw >>= 1;
     return sw;
 }
-// Column transforms by strips or through transposes, chosen per pass from
-// batches of 32 measured both ways.  The Wiener pass goes through the
-// transposes above 128 rows: 26.1k -> 37.5k images/s at 1024 x 1024 (4-column
-// strips, 32-byte row pieces), 100k -> 121k at 512 x 512.  The spectrum
-// (PCE mode) keeps strips while a strip holds at least 8 columns (h <= 512):
-// through the transposes its batched output, whose rows are not the plane
-// stride apart, takes one transpose launch per image.
-bool wiener_strips(int h) { return h <= 128; }
-bool spectrum_strips(int h) { return col_strip(h) >= 8; }
-bool needs_transposed(int h) { return !wiener_strips(h) || !spectrum_strips(h); }
+// Column transforms by strips up to 128 rows, through transposes above
+// (batches of 32 measured both ways, NCC / PCE images/s): 1024 x 1024
+// 26.1k / 20.8k by 4-column strips (32-byte row pieces) -> 37.5k / 29.2k;
+// 512 x 512 99k / 78k by 8-column strips -> 123k / 98k.
+bool strips_for(int h) { return h <= 128; }
 
 // Per-image buffers, each an array over the batch with the given stride.
 struct ExtractWs {
@@ -660,7 +655,7 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 size_t per_image_bytes(int h, int w) {
     const int64_t n = (int64_t)h * w;
     const int tx = (w + kFnTX - 1) / kFnTX, ty = (h + kFnTY - 1) / kFnTY;
-    return (needs_transposed(h) ? 24 : 16) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
+    return (strips_for(h) ? 16 : 24) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
            11 * 256;
 }
 
@@ -674,7 +669,7 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
     e.cp_stride = 2 * (int64_t)w * e.tiles_y;
     e.rp_stride = 2 * (int64_t)h * e.tiles_x;
     const size_t sz[] = {align256(sizeof(float2) * n * nb),
-                         needs_transposed(h) ? align256(sizeof(float2) * n * nb) : 0,
+                         strips_for(h) ? 0 : align256(sizeof(float2) * n * nb),
                          align256(sizeof(float) * n * nb),    align256(sizeof(float) * n * nb),
                          align256(sizeof(double) * h * nb),   align256(sizeof(double) * nb),
                          align256(sizeof(double) * e.cp_stride * nb), align256(sizeof(double) * e.rp_stride * nb),
@@ -771,7 +766,7 @@ int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int nco
 // w/2+1 columns, the variance stencil on the full grid (mirrored reads of
 // the half power spectrum), the symmetrized scale + inverse column FFTs,
 // real inverse row FFTs two rows at a time; columns by strips or through
-// transposes (wiener_strips).
+// transposes (strips_for).
 int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t img_stride, float *out,
                   int64_t out_stride, bool zm, hipStream_t st) {
     const int h = c->h, w = c->w, wh = w / 2 + 1;
@@ -784,7 +779,7 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
     PRNU_CHECK_LAUNCH();
-    if (wiener_strips(h)) {
+    if (strips_for(h)) {
         PRNU_TRY((launch_cols_fft<-1, 1>(dh, e.A, n, wh, wh, nb, nullptr, e.P, st)));
         PRNU_TRY(launch_varest(e.P, e.S, h, w, false, nb, e.var, st));
         PRNU_TRY((launch_cols_fft<1, 2>(dh, e.A, n, wh, wh, nb, e.S, nullptr, st, w)));
@@ -813,7 +808,7 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     const int h = c->h, w = c->w, wh = w / 2 + 1;
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    if (spectrum_strips(h)) {
+    if (strips_for(h)) {
         PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, out, out_stride, h);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
@@ -825,12 +820,7 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
     PRNU_LAUNCH_ROW(rows_fft_fwd_inplace_kernel, h, dim3(wh, nb), st, dh, e.B, n);
     PRNU_CHECK_LAUNCH();
-    if (nb == 1 || out_stride == n) {
-        PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st));
-    } else {
-        for (int b = 0; b < nb; ++b)
-            PRNU_TRY(launch_transpose_c(e.B + (int64_t)b * n, out + (int64_t)b * out_stride, wh, h, 1, 0, st));
-    }
+    PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st, out_stride));
     return PRNU_OK;
 }
 

@@ -96,9 +96,9 @@ namespace {
 constexpr int kTT = 32;
 
 __global__ __launch_bounds__(256) void transpose_c_kernel(const float2 *__restrict__ in, float2 *__restrict__ out,
-                                                          int rows, int cols, int64_t batch_stride) {
+                                                          int rows, int cols, int64_t in_stride, int64_t out_stride) {
     __shared__ float2 tile[kTT][kTT + 1];
-    const int64_t off = (int64_t)blockIdx.z * batch_stride;
+    const int64_t off = (int64_t)blockIdx.z * in_stride, ooff = (int64_t)blockIdx.z * out_stride;
     const int c0 = blockIdx.x * kTT, r0 = blockIdx.y * kTT;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
 #pragma unroll
@@ -110,15 +110,16 @@ __global__ __launch_bounds__(256) void transpose_c_kernel(const float2 *__restri
 #pragma unroll
     for (int i = 0; i < kTT; i += 8) {
         const int c = c0 + ty + i, r = r0 + tx;
-        if (r < rows && c < cols) out[off + (int64_t)c * rows + r] = tile[tx][ty + i];
+        if (r < rows && c < cols) out[ooff + (int64_t)c * rows + r] = tile[tx][ty + i];
     }
 }
 }  // namespace
 
 int launch_transpose_c(const float2 *in, float2 *out, int rows, int cols, int batch, int64_t batch_stride,
-                       hipStream_t st) {
+                       hipStream_t st, int64_t out_batch_stride) {
     dim3 grid((cols + kTT - 1) / kTT, (rows + kTT - 1) / kTT, batch);
-    hipLaunchKernelGGL(transpose_c_kernel, grid, dim3(256), 0, st, in, out, rows, cols, batch_stride);
+    hipLaunchKernelGGL(transpose_c_kernel, grid, dim3(256), 0, st, in, out, rows, cols, batch_stride,
+                       out_batch_stride < 0 ? batch_stride : out_batch_stride);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
