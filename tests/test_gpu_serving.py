@@ -78,13 +78,14 @@ def test_query_topk_and_exact_scores(small_store):
     ctx.close()
 
 
-@pytest.mark.parametrize("h,w", [(8, 16), (16, 16), (16, 24)])
+@pytest.mark.parametrize("h,w", [(8, 16), (16, 16), (16, 24), (9, 15)])
 def test_scan_with_one_to_three_k_steps(h, w):
     """Fingerprints of 128, 256 and 384 values: one, two and three FP8 K steps
-    per row, so every slab of the scan ends before its load ring fills.  300
-    stored (a second, clamped store tile) and 70 queries (a second, ragged
-    query block); scan within the e4m3 bound of the exact NCC, owners found,
-    and each query's scores the same alone."""
+    per row, so every slab of the scan ends before its load ring fills; and
+    135 values (two K steps, rows not a multiple of 16 bytes, so the re-score
+    takes its scalar loads).  300 stored (a second, clamped store tile) and 70
+    queries (a second, ragged query block); scan within the e4m3 bound of the
+    exact NCC, owners found, and each query's scores the same alone."""
     n, cap = h * w, 300
     fp = fingerprints(cap, n, seed=21 + n)
     store = serving.FingerprintStore(h, w, cap)
