@@ -41,18 +41,27 @@ __global__ __launch_bounds__(256) void copy_rows_kernel(const float *const *ptrs
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) d[i] = src[i];
 }
 
-// Top-k of one row of `scores` (length N), k <= kMaxK: one 256-thread block
-// per row.  Each thread keeps the best kMaxK of its strided share, sorted, so
-// the union of the heads contains the row's top k; then k rounds of a block
-// arg-max over the list heads.  Ties go to the smaller index (deterministic).
+// Top-k of a range of one row of `scores`, k <= kMaxK: one 256-thread block
+// per range.  Each thread keeps the best kMaxK of its strided share, sorted,
+// so the union of the heads contains the range's top k; then k rounds of a
+// block arg-max over the list heads.  Ties go to the smaller index
+// (deterministic).
 __device__ __forceinline__ bool better(double a, int ia, double b, int ib) { return a > b || (a == b && ia < ib); }
 
-__global__ __launch_bounds__(256) void topk_kernel(const double *scores, int64_t ld, int N, int k, int *out_idx) {
+// grid (rows, kTopkChunks): block (q, c) takes the top `k` of columns
+// [c * chunk_len, (c + 1) * chunk_len) of row q into out_idx[q * out_ld + c * k
+// ...] (-1 past the chunk's end).  With k = kMaxK per chunk the chunk lists
+// hold every member of the row's top k, which topk_merge_kernel picks out.
+constexpr int kTopkChunks = 8;
+
+__global__ __launch_bounds__(256) void topk_kernel(const double *scores, int64_t ld, int N, int k, int *out_idx,
+                                                   int chunk_len, int64_t out_ld) {
     __shared__ double sv[256 * kMaxK];
     __shared__ int si[256 * kMaxK];
     __shared__ double wv[4];
     __shared__ int wi[4], wt[4];
     const double *row = scores + (int64_t)blockIdx.x * ld;
+    const int j0 = blockIdx.y * chunk_len, j1 = min(N, j0 + chunk_len);
     double v[kMaxK];
     int ix[kMaxK];
 #pragma unroll
@@ -60,7 +69,7 @@ __global__ __launch_bounds__(256) void topk_kernel(const double *scores, int64_t
         v[q] = -1e300;
         ix[q] = 0x7fffffff;
     }
-    for (int j = threadIdx.x; j < N; j += 256) {
+    for (int j = j0 + threadIdx.x; j < j1; j += 256) {
         double cv = row[j];
         int ci = j;
         if (better(cv, ci, v[kMaxK - 1], ix[kMaxK - 1])) {
@@ -114,9 +123,58 @@ __global__ __launch_bounds__(256) void topk_kernel(const double *scores, int64_t
                 bi = wi[q];
                 bt = wt[q];
             }
-        if (threadIdx.x == 0) out_idx[(int64_t)blockIdx.x * k + r] = bi < N ? bi : -1;
+        if (threadIdx.x == 0) out_idx[(int64_t)blockIdx.x * out_ld + (int64_t)blockIdx.y * k + r] = bi < N ? bi : -1;
         if (threadIdx.x == bt) ++head;
         __syncthreads();
+    }
+}
+
+// grid rows, block 64: the top k of row q among its kTopkChunks * kMaxK
+// chunk candidates cand[q * kTopkChunks * kMaxK ...] (-1: none), ties to the
+// smaller index, into out_idx[q * k ...].
+__global__ __launch_bounds__(64) void topk_merge_kernel(const double *scores, int64_t ld, const int *cand, int k,
+                                                        int *out_idx) {
+    constexpr int kC = kTopkChunks * kMaxK / 64;  // candidates per lane
+    const int q = blockIdx.x, lane = threadIdx.x;
+    const double *row = scores + (int64_t)q * ld;
+    double v[kC];
+    int ix[kC];
+#pragma unroll
+    for (int e = 0; e < kC; ++e) {
+        const int c = cand[(int64_t)q * kTopkChunks * kMaxK + e * 64 + lane];
+        v[e] = c >= 0 ? row[c] : -1e300;
+        ix[e] = c >= 0 ? c : 0x7fffffff;
+    }
+    for (int r = 0; r < k; ++r) {
+        double bv = v[0];
+        int bi = ix[0], be = 0;
+#pragma unroll
+        for (int e = 1; e < kC; ++e)
+            if (better(v[e], ix[e], bv, bi)) {
+                bv = v[e];
+                bi = ix[e];
+                be = e;
+            }
+        double wv = bv;
+        int wi = bi;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(wv, off, 64);
+            const int oi = __shfl_xor(wi, off, 64);
+            if (better(ov, oi, wv, wi)) {
+                wv = ov;
+                wi = oi;
+            }
+        }
+        if (lane == 0) out_idx[(int64_t)q * k + r] = wi == 0x7fffffff ? -1 : wi;
+        if (bi == wi && wi != 0x7fffffff) {  // indices are distinct: one lane holds the winner
+#pragma unroll
+            for (int e = 0; e < kC; ++e)
+                if (e == be) {
+                    v[e] = -1e300;
+                    ix[e] = 0x7fffffff;
+                }
+        }
     }
 }
 
@@ -317,7 +375,8 @@ int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, i
     const size_t b_scan = d_scan ? 0 : a256(sizeof(double) * (size_t)nq * N);
     const int64_t slices = rescore_slices(s->n);
     const size_t b_part = a256(sizeof(double) * (size_t)nq * k * slices);
-    PRNU_TRY(c->ws_fp8.ensure(b_stats + b_q + b_scan + b_part));
+    const size_t b_cand = a256(sizeof(int) * (size_t)nq * kTopkChunks * kMaxK);
+    PRNU_TRY(c->ws_fp8.ensure(b_stats + b_q + b_scan + b_part + b_cand));
     char *p = static_cast<char *>(c->ws_fp8.ptr);
     double *qsum = reinterpret_cast<double *>(p);
     double *qsumsq = reinterpret_cast<double *>(p + a256(sizeof(double) * nq));
@@ -326,6 +385,7 @@ int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, i
     float *qscale = reinterpret_cast<float *>(p + b_stats + a256((size_t)s->stride * nq));
     double *scan = d_scan ? d_scan : reinterpret_cast<double *>(p + b_stats + b_q);
     double *part = reinterpret_cast<double *>(p + b_stats + b_q + b_scan);
+    int *cand = reinterpret_cast<int *>(p + b_stats + b_q + b_scan + b_part);
 
     PRNU_TRY(launch_pattern_stats(d_qptrs, nq, s->n, qsum, qsumsq, qabs, c->ws_small, st));
     PRNU_TRY(launch_quantize_fp8(d_qptrs, nq, s->n, qabs, qq, qscale, st));
@@ -337,7 +397,11 @@ int fpstore_query(Context *c, FpStore *s, const float *const *d_qptrs, int nq, i
                                 s->stride, scan + (size_t)q0 * N, N, c->gram_cfg[PRNU_PREC_FP8].slabs, st, nullptr));
     }
     time_end(c, st);
-    hipLaunchKernelGGL(topk_kernel, dim3(nq), dim3(256), 0, st, scan, (int64_t)N, N, k, d_idx);
+    const int chunk_len = (N + kTopkChunks - 1) / kTopkChunks;
+    hipLaunchKernelGGL(topk_kernel, dim3(nq, kTopkChunks), dim3(256), 0, st, scan, (int64_t)N, N, kMaxK, cand,
+                       chunk_len, (int64_t)kTopkChunks * kMaxK);
+    PRNU_CHECK_LAUNCH();
+    hipLaunchKernelGGL(topk_merge_kernel, dim3(nq), dim3(64), 0, st, scan, (int64_t)N, cand, k, d_idx);
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(rescore_slice_kernel, dim3((unsigned)slices, nq), dim3(256), 0, st, d_qptrs, s->f32, s->n,
                        rescore_slice_len(s->n), d_idx, k, part);
