@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """NCC all-pairs benchmark on one MI355X.
 
-Workload: 1024 synthetic patterns of 2048x1536 (16 cameras), all 523,776
-pairs as a triangular Gram on MFMA.  One step = per-pattern FP64 statistics +
-the split-BF16 Gram + the FP64 NCC epilogue over every pair, with the patterns
-already resident in HBM.  Prints one JSON line.
+Workload: 1024 synthetic 8-bit frames of 2048x1536 from 16 cameras, extracted
+on the GPU into PRNU patterns (gray, FastNoise, zero-mean, Wiener) and
+inserted into a K-blocked pattern store; then all 523,776 pairs as a
+triangular Gram on MFMA.  Insertion computes the per-pattern FP64 statistics
+and the split-BF16 hi/lo planes once (reported as store_insert_ms); one
+timed step = the split-BF16 Gram + the FP64 slab reduction and NCC epilogue
+over every pair, with the store resident in HBM.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--precision split_bf16|fp32]
 """
@@ -54,18 +57,23 @@ def log(msg):
 
 
 def make_patterns(torch):
-    """Zero-mean FP32 sensor-noise patterns: 0.02 * K_c + N(0, 1), seeded."""
+    """PRNU patterns of 1024 synthetic 2048x1536 frames from 16 cameras:
+    frame i of camera i % 16 is I = clamp(round(S (1 + 0.02 K_c) + N(0, 2^2)),
+    0, 255) (S a smooth random scene, K_c ~ N(0, 1), seeded), extracted on the
+    GPU by the batched PRNUFilter chain in chunks of 64 frames."""
+    from rocket_amd import _lib, csi
     g = torch.Generator(device="cuda").manual_seed(20261202)
     n = HEIGHT * WIDTH
     k = torch.randn((CAMERAS, n), generator=g, device="cuda")
     p = torch.empty((COUNT, n), dtype=torch.float32, device="cuda")
+    filt = csi.PRNUFilter(HEIGHT, WIDTH)
     chunk = 64
     for b in range(0, COUNT, chunk):
         e = min(COUNT, b + chunk)
-        blk = torch.randn((e - b, n), generator=g, device="cuda")
-        blk += 0.02 * k[torch.arange(b, e, device="cuda") % CAMERAS]
-        blk -= blk.mean(dim=1, keepdim=True)
-        p[b:e] = blk
+        imgs = synth_images(torch, k, [i % CAMERAS for i in range(b, e)], seed=1000 + b, h=HEIGHT, w=WIDTH)
+        filt.apply_gpu_batch(imgs.view(e - b, n), p[b:e], _lib.INPUT_GRAY8)
+        del imgs
+    filt.cleanup()
     del k
     return p
 
@@ -176,13 +184,49 @@ def load_traffic(kernel: str):
         return None
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the machine, and CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "nproc": len(os.sched_getaffinity(0))}
+
+
+_PORT = {}
+
+
+def cpu_port():
+    """The C port of the reference's CPU path, built with -march=native for
+    this host (always rebuilt here, so a library built on another machine is
+    never timed); the portable x86-64-v3 build if that fails.
+    Returns (ctypes library, build description)."""
+    if "lib" not in _PORT:
+        import subprocess
+        out = os.path.join(ROOT, "oracle", "_build")
+        native = os.path.join(out, "libprnu_cpu_port_native.so")
+        r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"],
+                           capture_output=True, text=True, timeout=300)
+        if r.returncode == 0 and os.path.exists(native):
+            _PORT["lib"], _PORT["build"] = ctypes.CDLL(native), "gcc -O3 -march=native -fopenmp"
+        else:
+            _PORT["lib"] = ctypes.CDLL(os.path.join(out, "libprnu_cpu_port.so"))
+            _PORT["build"] = "gcc -O3 -march=x86-64-v3 -fopenmp (native build failed)"
+    return _PORT["lib"], _PORT["build"]
+
+
 def cpu_baseline(torch, p):
     """The C restatement of NormalizedCrossCorrelation.applyCPU, all host cores,
     on all pairs of a 48-pattern sample of the same set."""
     lib_path = os.path.join(ROOT, "oracle", "_build", "libprnu_cpu_port.so")
     if not os.path.exists(lib_path):
         return None
-    port = ctypes.CDLL(lib_path)
+    port, build = cpu_port()
     port.port_ncc_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]
     port.port_sumsq_many.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]
@@ -208,7 +252,8 @@ def cpu_baseline(torch, p):
     dt = time.perf_counter() - t0
     return {"value": count / dt, "unit": "pairs/s", "cores": int(threads), "kind": "port",
             "sample": f"{count} pair evaluations over the {len(uniq)} pairs of {sample} of the 1024 patterns "
-                      f"(2048x1536), C port of NormalizedCrossCorrelation.applyCPU, OpenMP, {dt:.1f} s"}
+                      f"(2048x1536), C port of NormalizedCrossCorrelation.applyCPU, OpenMP, {dt:.1f} s",
+            "build": build, **host_cpu()}
 
 
 SERVE_H, SERVE_W, SERVE_STORE, SERVE_RATE = 1024, 1024, 20000, 200.0
@@ -220,8 +265,7 @@ def cpu_baseline_aux():
     and of PCE from two spectra (both variants), one image or pair per
     thread on all host cores, over bounded samples (part of the CPU baseline
     leg)."""
-    lib_path = os.path.join(ROOT, "oracle", "_build", "libprnu_cpu_port.so")
-    port = ctypes.CDLL(lib_path)
+    port, build = cpu_port()
     port.port_max_threads.restype = ctypes.c_int
     port.port_prnu_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
@@ -229,7 +273,7 @@ def cpu_baseline_aux():
                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
     threads = port.port_max_threads()
     rng = np.random.default_rng(5)
-    res = {"cores": int(threads), "kind": "port"}
+    res = {"cores": int(threads), "kind": "port", "build": build, **host_cpu()}
     samples = []
     for h, w, tag in ((HEIGHT, WIDTH, ""), (512, 512, "_512")):
         count = 2 * threads
@@ -966,8 +1010,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16" if prec == "split_bf16" else "fp32",
-        "data": "synthetic: zero-mean FP32 patterns 0.02*K_c + N(0,1), 16 cameras, seeded; resident in a "
-                "K-blocked pattern store (insert time reported as store_insert_ms)",
+        "data": "synthetic: 1024 seeded 8-bit 2048x1536 frames I = clamp(round(S(1+0.02 K_c) + N(0,2^2))) "
+                "from 16 cameras, extracted on the GPU into FP32 PRNU patterns, resident in a K-blocked "
+                "pattern store (insert time reported as store_insert_ms); max_abs_dncc_vs_fp64 on these patterns",
         "config": {"workload": "ncc1024_2048x1536", "patterns": COUNT, "height": HEIGHT, "width": WIDTH,
                    "pairs": pairs, "precision": prec, "parallelism": "single GPU"},
         "max_abs_dncc_vs_fp64": dncc,

@@ -230,13 +230,17 @@ PRNU_EXPORT int prnu_store_stats(const prnu_store *store, const double **d_sum, 
  * without extraction or conversion.  prnu_store_image_bytes: the size of an
  * image of `count` slots (-1 for a null store).  prnu_store_save writes slots
  * [first_slot, first_slot + count) into h_image; prnu_store_load copies an
- * image into slots [first_slot, first_slot + its count) of a store of the
- * same length and format (PRNU_EINVAL otherwise).  Both are asynchronous on
+ * image of `image_bytes` bytes into slots [first_slot, first_slot + its
+ * count) of a store of the same length and format (PRNU_EINVAL otherwise,
+ * and when image_bytes is shorter than an image of the header's count, so a
+ * truncated file is refused before anything is read past its end).  Both are
+ * asynchronous on
  * `stream` (pinned host memory lets them overlap kernels; with stream 0 they
  * return when done). */
 PRNU_EXPORT int64_t prnu_store_image_bytes(const prnu_store *store, int count);
 PRNU_EXPORT int prnu_store_save(const prnu_store *store, int first_slot, int count, void *h_image, prnu_stream stream);
-PRNU_EXPORT int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, prnu_stream stream);
+PRNU_EXPORT int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, size_t image_bytes,
+                                prnu_stream stream);
 /* All-pairs NCC of slots [first_slot, first_slot + count) in the store's
  * format, same epilogue and output as prnu_gram_ncc.  `ctx` supplies the
  * workspace, the launch shape (its prnu_gram_config of the store's format)
@@ -319,10 +323,13 @@ typedef struct prnu_fpstore prnu_fpstore;
 
 PRNU_EXPORT int prnu_fpstore_create(int64_t n, int capacity, prnu_fpstore **out);
 PRNU_EXPORT int prnu_fpstore_destroy(prnu_fpstore *store);
-/* size = 1 + the highest slot filled; bytes = device memory held */
+/* size = the number of filled slots, which are always slots [0, size);
+ * bytes = device memory held */
 PRNU_EXPORT int prnu_fpstore_info(const prnu_fpstore *store, int64_t *n, int *capacity, int *size, size_t *bytes);
 /* Quantise and copy `count` FP32 fingerprints (DEVICE pointer table) into
- * slots [first_slot, first_slot + count). */
+ * slots [first_slot, first_slot + count).  first_slot may not exceed the
+ * current size (PRNU_EINVAL): a put overwrites filled slots or appends, so a
+ * query never scans a slot that was never written. */
 PRNU_EXPORT int prnu_fpstore_put(prnu_fpstore *store, const float *const *d_patterns, int count, int first_slot,
                                  prnu_stream stream);
 /* Top-k (1 <= k <= 16) fingerprints of each of `nq` query patterns (DEVICE

@@ -65,7 +65,7 @@ def grayscale_from_gray8(gray: np.ndarray) -> np.ndarray:
 
 # --------------------------------------------------------------------------- a3
 def _hgrad(a: np.ndarray) -> np.ndarray:
-    """fastnoisefilter.cu:355-367: one-sided at j=0 and j=w-1, central inside."""
+    """fastnoisefilter.cu:150-161: one-sided at j=0 and j=w-1, central inside."""
     d = np.empty_like(a)
     d[:, 1:-1] = 0.5 * (a[:, 2:] - a[:, :-2])
     d[:, 0] = a[:, 1] - a[:, 0]
@@ -74,12 +74,12 @@ def _hgrad(a: np.ndarray) -> np.ndarray:
 
 
 def _vgrad(a: np.ndarray) -> np.ndarray:
-    """fastnoisefilter.cu:369-381."""
+    """fastnoisefilter.cu:163-175."""
     return _hgrad(a.T).T
 
 
 def fastnoise(x: np.ndarray) -> np.ndarray:
-    """FastNoise (first step total variation), ``fastnoisefilter.cu:383-409``.
+    """FastNoise (first step total variation), ``fastnoisefilter.cu:177-204``.
 
     dx, dy = gradients of x; s = 1 / (1 + sqrt(dx^2 + dy^2)) (EPS = 1,
     ``FastNoiseFilter.java:30``); the output *replaces* the image with

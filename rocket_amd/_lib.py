@@ -118,7 +118,7 @@ _SIGNATURES = {
                           _c_void_p, ctypes.c_int32]),
     "prnu_store_image_bytes": (_c_i64, [_c_void_p, _c_int]),
     "prnu_store_save": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_void_p]),
-    "prnu_store_load": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_void_p]),
+    "prnu_store_load": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_size, _c_void_p]),
     "prnu_ctx_set_timing": (_c_int, [_c_void_p, _c_int]),
     "prnu_ctx_last_kernel_ms": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "prnu_gram_ncc": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_void_p, _c_void_p, _c_void_p,

@@ -466,9 +466,9 @@ class PatternStore:
     def load(self, first_slot: int, image, sync: bool = True) -> None:
         """A slot image (written by save, same length and format) into slots
         from first_slot on."""
-        if image.is_cuda or image.dtype != torch.uint8:
-            raise ValueError("image must be a CPU uint8 tensor")
-        _lib.call("prnu_store_load", self.handle, int(first_slot), image.data_ptr(), _stream())
+        if image.is_cuda or image.dtype != torch.uint8 or not image.is_contiguous():
+            raise ValueError("image must be a contiguous CPU uint8 tensor")
+        _lib.call("prnu_store_load", self.handle, int(first_slot), image.data_ptr(), int(image.numel()), _stream())
         _sync(sync)
 
     def close(self) -> None:

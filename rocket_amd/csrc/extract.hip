@@ -6,7 +6,7 @@
 //
 // Reference semantics (paths under CSI/kernels/filter/):
 //   grayscalefilter.cu:47-66      out = 0.299*c.z + 0.587*c.y + 0.114*c.x
-//   fastnoisefilter.cu:355-409    out = hgrad(dx*s) + vgrad(dy*s), s = 1/(1+|grad|)
+//   fastnoisefilter.cu:150-204    out = hgrad(dx*s) + vgrad(dy*s), s = 1/(1+|grad|)
 //   zeromeantotalfilter.cu        per column then per row: subtract even/odd means
 //   wienerfilter.cu               F = FFT2(x); varest = min_f box_f(|F|^2);
 //                                 F *= var/max(var, varest); x = Re(IFFT2(F))/n
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void gray_kernel(const uint8_t *in, float *out
 // ---------------------------------------------------------------- FastNoise
 constexpr int kFnTX = 64, kFnTY = 32;  // output tile
 
-// One-sided at the borders, central inside (fastnoisefilter.cu:355-381).
+// One-sided at the borders, central inside (fastnoisefilter.cu:150-175).
 __device__ __forceinline__ float grad1(float left, float center, float right, int pos, int len) {
     if (pos == 0) return right - center;
     if (pos == len - 1) return center - left;

@@ -348,6 +348,11 @@ void fpstore_destroy(FpStore *s) {
 int fpstore_put(FpStore *s, const float *const *d_ptrs, int count, int first, hipStream_t st) {
     PRNU_REQUIRE(count >= 1 && first >= 0 && first + count <= s->capacity,
                  "fpstore_put: slots [%d, %d) outside the capacity %d", first, first + count, s->capacity);
+    // the filled slots stay the prefix [0, size) that every query scans: a put
+    // may overwrite filled slots or extend the prefix, never leave a gap of
+    // slots whose memory was never written
+    PRNU_REQUIRE(first <= s->size, "fpstore_put: slots [%d, %d) would leave slots [%d, %d) unfilled", first,
+                 first + count, s->size, first);
     PRNU_TRY(launch_pattern_stats(d_ptrs, count, s->n, s->sum + first, s->sumsq + first, s->absmax + first,
                                   s->ws_stats, st));
     PRNU_TRY(launch_quantize_fp8(d_ptrs, count, s->n, s->absmax + first, s->q + (size_t)first * s->stride,

@@ -1257,6 +1257,10 @@ void plan_drop_graph(GramPlan *p) {
 
 // Work list for slabs of `kcb` blocks: descriptors, partial-tile buffer.
 int plan_layout(GramPlan *p, int64_t kcb) {
+    // a run already queued on any stream (e.g. an asynchronous run on a
+    // non-blocking stream) still reads the descriptors and partial tiles that
+    // are rewritten or freed below: let it finish first
+    if (p->d_groups || p->partials) PRNU_CHECK_HIP(hipDeviceSynchronize());
     plan_drop_graph(p);
     const int64_t tile_elems = (int64_t)kTileS * kTileS;
     std::vector<GroupDev> gd(p->groups.size());
@@ -1622,10 +1626,13 @@ int prnu_store_save(const prnu_store *store, int first_slot, int count, void *h_
     return PRNU_OK;
 }
 
-int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, prnu_stream stream) {
+int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, size_t image_bytes,
+                    prnu_stream stream) {
     PRNU_REQUIRE(store && h_image, "store_load: null pointer");
     Store *s = reinterpret_cast<Store *>(store);
     const uint8_t *h = static_cast<const uint8_t *>(h_image);
+    PRNU_REQUIRE(image_bytes >= sizeof(SlotImageHeader), "store_load: %zu bytes hold no slot image header",
+                 image_bytes);
     SlotImageHeader hd;
     std::memcpy(&hd, h, sizeof(hd));
     PRNU_REQUIRE(std::memcmp(hd.magic, "PRNUSLOT", 8) == 0, "store_load: not a slot image");
@@ -1635,6 +1642,10 @@ int prnu_store_load(prnu_store *store, int first_slot, const void *h_image, prnu
     const int count = hd.count;
     PRNU_REQUIRE(count >= 1 && first_slot >= 0 && first_slot + count <= s->capacity,
                  "store_load: slots [%d, %d) outside the capacity %d", first_slot, first_slot + count, s->capacity);
+    // every byte the copies below read lies inside the caller's buffer
+    PRNU_REQUIRE((int64_t)image_bytes >= slot_image_bytes(s, count),
+                 "store_load: image of %d slots needs %lld bytes, the buffer holds %zu", count,
+                 (long long)slot_image_bytes(s, count), image_bytes);
     hipStream_t st = (hipStream_t)stream;
     const uint8_t *data = h + sizeof(hd);
     const size_t row = (size_t)count * kBlkBytes;

@@ -107,6 +107,7 @@ class Request:
     idx: np.ndarray | None = None   # top-k slots
     score: np.ndarray | None = None  # exact NCC
     batch_size: int = 0
+    error: BaseException | None = None   # set instead of idx/score when the batch failed
     done: threading.Event = field(default_factory=threading.Event)
 
     @property
@@ -116,19 +117,87 @@ class Request:
     def wait(self, timeout: float | None = None) -> bool:
         return self.done.wait(timeout)
 
+    def result(self, timeout: float | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """(idx, score) once answered; raises the batch's error, or
+        TimeoutError when no answer came within `timeout` seconds."""
+        if not self.done.wait(timeout):
+            raise TimeoutError("request not answered in time")
+        if self.error is not None:
+            raise self.error
+        return self.idx, self.score
+
+
+class DeviceArena:
+    """First-fit allocator over one device buffer of ``size`` bytes: the HBM
+    that in-flight batches draw their per-query buffers from.  Offsets are
+    multiples of ``align``; freed ranges are merged with their neighbours.
+    ``base`` is the buffer's device address (0 for bookkeeping alone)."""
+
+    def __init__(self, size: int, align: int = 256, base: int = 0):
+        self.size, self.align, self.base = int(size), int(align), int(base)
+        self.free_ranges: list[tuple[int, int]] = [(0, self.size)] if self.size > 0 else []
+        self.used = 0
+        self.high_water = 0
+
+    def _round(self, nbytes: int) -> int:
+        return (int(nbytes) + self.align - 1) // self.align * self.align
+
+    def largest_free(self) -> int:
+        return max((n for _, n in self.free_ranges), default=0)
+
+    def alloc(self, nbytes: int) -> int | None:
+        """Offset of a fresh range of ``nbytes`` (rounded up), or None."""
+        need = self._round(nbytes)
+        for i, (off, n) in enumerate(self.free_ranges):
+            if n >= need:
+                if n == need:
+                    del self.free_ranges[i]
+                else:
+                    self.free_ranges[i] = (off + need, n - need)
+                self.used += need
+                self.high_water = max(self.high_water, self.used)
+                return off
+        return None
+
+    def free(self, off: int, nbytes: int) -> None:
+        need = self._round(nbytes)
+        if off < 0 or off + need > self.size or off % self.align:
+            raise ValueError(f"range [{off}, {off + need}) is not an allocation of this arena")
+        rs = self.free_ranges
+        i = 0
+        while i < len(rs) and rs[i][0] < off:
+            i += 1
+        if (i > 0 and rs[i - 1][0] + rs[i - 1][1] > off) or (i < len(rs) and off + need > rs[i][0]):
+            raise ValueError(f"range [{off}, {off + need}) is already free")
+        rs.insert(i, (off, need))
+        if i + 1 < len(rs) and off + need == rs[i + 1][0]:        # merge with the next range
+            rs[i] = (off, need + rs[i + 1][1])
+            del rs[i + 1]
+        if i > 0 and rs[i - 1][0] + rs[i - 1][1] == off:          # and with the previous one
+            rs[i - 1] = (rs[i - 1][0], rs[i - 1][1] + rs[i][1])
+            del rs[i]
+        self.used -= need
+
+
+def batch_bytes(n: int, size: int, k: int, m: int, align: int = 256) -> int:
+    """Arena bytes of one batch of ``m`` queries of ``n`` pixels against a
+    store of ``size`` fingerprints: u8 images, FP32 patterns, FP64 scan
+    scores (m x size), int32 indices and FP64 scores of the top k, each
+    buffer rounded up to ``align``."""
+    r = lambda b: (int(b) + align - 1) // align * align   # noqa: E731
+    return r(m * n) + r(4 * m * n) + r(8 * m * size) + r(4 * m * k) + r(8 * m * k)
+
 
 class _Slot:
-    """One in-flight batch: its own context, stream and buffers."""
+    """One in-flight batch: its own context, stream, pinned host staging and
+    device pointer table; the batch's device buffers come from the arena."""
 
     def __init__(self, height: int, width: int, max_batch: int, k: int):
         n = height * width
         self.ctx = _Ctx(height, width, _lib.MODE_NCC)
         self.stream = torch.cuda.Stream()
-        self.images = torch.empty((max_batch, n), dtype=torch.uint8, device="cuda")
-        self.patterns = torch.empty((max_batch, n), dtype=torch.float32, device="cuda")
-        self.ptrs = pattern_pointer_table(self.patterns)
-        self.idx = torch.empty((max_batch, k), dtype=torch.int32, device="cuda")
-        self.score = torch.empty((max_batch, k), dtype=torch.float64, device="cuda")
+        self.d_ptrs = torch.zeros(max_batch, dtype=torch.int64, device="cuda")
+        self.h_ptrs = torch.zeros(max_batch, dtype=torch.int64, pin_memory=True)
         self.h_images = torch.empty((max_batch, n), dtype=torch.uint8, pin_memory=True)
         self.h_idx = torch.empty((max_batch, k), dtype=torch.int32, pin_memory=True)
         self.h_score = torch.empty((max_batch, k), dtype=torch.float64, pin_memory=True)
@@ -137,6 +206,7 @@ class _Slot:
         self.event = torch.cuda.Event()
         self.busy = False
         self.requests: list[Request] = []
+        self.lease: tuple[int, int] | None = None     # (arena offset, bytes) of the batch in flight
 
 
 _gc_lock = threading.Lock()
@@ -163,27 +233,44 @@ def _gc_release() -> None:
 class Server:
     """Continuous-batching query server over a :class:`FingerprintStore`.
 
+    Admission by free HBM: the per-query device buffers of every batch in
+    flight (images, patterns, scan scores, results) are carved out of one
+    arena of ``hbm_budget`` bytes (default: the device's free memory at
+    construction less ``hbm_reserve`` and the slots' fixed workspaces).  A
+    batch takes as many pending requests as ``max_batch``, a free slot and
+    the arena's largest free range allow; when the arena is full, requests
+    wait until a batch in flight completes and gives its range back.
+
     ``freeze_gc``: from ``start()`` to ``stop()`` the objects alive at start
     are moved out of the cyclic collector's reach (``gc.freeze``; shared by
     all running servers of the process, undone when the last one stops)."""
 
     def __init__(self, store: FingerprintStore, k: int = 10, max_batch: int = 64, slots: int = 2,
-                 hbm_reserve: int = 2 << 30, freeze_gc: bool = True):
+                 hbm_reserve: int = 2 << 30, freeze_gc: bool = True, hbm_budget: int | None = None):
         self.store = store
         self.k = int(k)
         self.h, self.w = store.height, store.width
         n = self.h * self.w
-        # admission by free HBM: every slot holds max_batch images, patterns,
-        # results and the scan workspace (FP8 queries, nq x size scores)
+        self.size = store.info()["size"]
+        if self.size < self.k:
+            raise ValueError(f"the store holds {self.size} fingerprints, fewer than k = {self.k}")
+        self.per_query_bytes = batch_bytes(n, self.size, self.k, 1)
+        # fixed per slot: extraction workspace (about 25 n per query of a
+        # batch), FP8 query copies, scan partial tiles, FFT plans
+        fixed = 256 << 20
+        per_slot_ws = 27 * n
         free, _ = self.mem_info()
-        size = max(1, store.info()["size"])
-        # image, pattern, FP8 copy, scan scores, results, extraction workspace
-        per_query = n + 4 * n + n + 8 * size + 64 * self.k + 25 * n
-        fixed = 256 << 20                   # scan partial tiles, FFT plans
-        fit = (free - hbm_reserve - slots * fixed) // max(1, slots * per_query)
+        fit = (free - int(hbm_reserve) - slots * fixed) // max(1, slots * (per_slot_ws + self.per_query_bytes))
         self.max_batch = int(max(1, min(max_batch, fit)))
+        if hbm_budget is None:
+            hbm_budget = free - int(hbm_reserve) - slots * (fixed + per_slot_ws * self.max_batch)
+        budget = int(min(int(hbm_budget), slots * batch_bytes(n, self.size, self.k, self.max_batch)))
+        if budget < batch_bytes(n, self.size, self.k, 1):
+            raise ValueError(f"an HBM budget of {budget} bytes holds no batch of one query "
+                             f"({batch_bytes(n, self.size, self.k, 1)} bytes)")
         self.hbm_reserve = int(hbm_reserve)
-        self.per_query_bytes = int(per_query)
+        self._arena_buf = torch.empty(budget, dtype=torch.uint8, device="cuda")
+        self.arena = DeviceArena(budget, base=self._arena_buf.data_ptr())
         self.slots = [_Slot(self.h, self.w, self.max_batch, self.k) for _ in range(slots)]
         self.pending: "queue.Queue[Request]" = queue.Queue()
         self.inflight: "queue.Queue[_Slot | None]" = queue.Queue()
@@ -203,28 +290,71 @@ class Server:
 
     # ---------------------------------------------------------------- client
     def submit(self, image: np.ndarray) -> Request:
-        r = Request(image=np.ascontiguousarray(image, dtype=np.uint8))
+        """Queue one 8-bit image of the store's frame size (h*w values)."""
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        if img.size != self.h * self.w:
+            raise ValueError(f"query image of {img.size} values, the store holds {self.w}x{self.h} fingerprints")
+        if self._stop.is_set():
+            raise RuntimeError("the server is stopped")
+        r = Request(image=img)
         r.t_submit = time.perf_counter()
         self.pending.put(r)
         return r
 
-    def start(self) -> "Server":
-        # size every slot's workspaces once (a growing workspace synchronises
-        # the device, which would serialise the in-flight batches later)
+    def _views(self, off: int, m: int):
+        """Device buffers of a batch of m queries at arena offset `off`."""
+        n = self.h * self.w
+        a = self.arena._round
+        buf = self._arena_buf
+        o = off
+        images = buf[o:o + m * n].view(m, n)
+        o += a(m * n)
+        patterns = buf[o:o + 4 * m * n].view(torch.float32).view(m, n)
+        o += a(4 * m * n)
+        scan = buf[o:o + 8 * m * self.size].view(torch.float64).view(m, self.size)
+        o += a(8 * m * self.size)
+        idx = buf[o:o + 4 * m * self.k].view(torch.int32).view(m, self.k)
+        o += a(4 * m * self.k)
+        score = buf[o:o + 8 * m * self.k].view(torch.float64).view(m, self.k)
+        return images, patterns, scan, idx, score
+
+    def _launch(self, slot: _Slot, m: int, off: int) -> None:
+        """Enqueue one batch of m queries (images already in slot.h_images) on
+        the slot's stream."""
         lib = _lib.load()
         n = self.h * self.w
+        images, patterns, scan, idx, score = self._views(off, m)
+        base = patterns.data_ptr()
+        hp = slot.h_ptrs.numpy()
+        hp[:m] = base + 4 * n * np.arange(m, dtype=np.int64)
+        with torch.cuda.stream(slot.stream):
+            st = slot.stream.cuda_stream
+            slot.d_ptrs[:m].copy_(slot.h_ptrs[:m], non_blocking=True)
+            images.copy_(slot.h_images[:m], non_blocking=True)
+            _lib.check(lib.prnu_preprocess_batch(slot.ctx.handle, _lib.INPUT_GRAY8, images.data_ptr(), m * n, n, m,
+                                                 patterns.data_ptr(), 4 * n * m, 4 * n, st), "prnu_preprocess_batch")
+            _lib.check(lib.prnu_fpstore_query(slot.ctx.handle, self.store.handle, slot.d_ptrs.data_ptr(), m, self.k,
+                                              idx.data_ptr(), score.data_ptr(), scan.data_ptr(), st),
+                       "prnu_fpstore_query")
+            slot.h_idx[:m].copy_(idx, non_blocking=True)
+            slot.h_score[:m].copy_(score, non_blocking=True)
+            slot.event.record(slot.stream)
+
+    def start(self) -> "Server":
+        # size every slot's workspaces once with a full batch (a growing
+        # workspace synchronises the device, which would serialise the
+        # in-flight batches later); the arena lends the full batch's buffers
+        m = self.max_batch
         for slot in self.slots:
-            with torch.cuda.stream(slot.stream):
-                st = slot.stream.cuda_stream
-                slot.images.zero_()
-                _lib.check(lib.prnu_preprocess_batch(slot.ctx.handle, _lib.INPUT_GRAY8, slot.images.data_ptr(),
-                                                     self.max_batch * n, n, self.max_batch,
-                                                     slot.patterns.data_ptr(), 4 * n * self.max_batch, 4 * n, st))
-                slot.patterns.normal_()
-                _lib.check(lib.prnu_fpstore_query(slot.ctx.handle, self.store.handle, slot.ptrs.data_ptr(),
-                                                  self.max_batch, self.k, slot.idx.data_ptr(),
-                                                  slot.score.data_ptr(), None, st))
+            off = self.arena.alloc(batch_bytes(self.h * self.w, self.size, self.k, m))
+            if off is None:             # a budget below one full batch: borrow temporary buffers
+                self._warm_outside_arena(slot, m)
+                continue
+            slot.h_images[:m].zero_()
+            self._launch(slot, m, off)
             slot.stream.synchronize()
+            self.arena.free(off, batch_bytes(self.h * self.w, self.size, self.k, m))
+        self.arena.high_water = 0
         if self.freeze_gc:
             _gc_hold()
             self._frozen = True
@@ -232,32 +362,73 @@ class Server:
         self._done.start()
         return self
 
+    def _warm_outside_arena(self, slot: _Slot, m: int) -> None:
+        saved_buf, saved_arena = self._arena_buf, self.arena
+        try:
+            self._arena_buf = torch.empty(batch_bytes(self.h * self.w, self.size, self.k, m), dtype=torch.uint8,
+                                          device="cuda")
+            slot.h_images[:m].zero_()
+            self._launch(slot, m, 0)
+            slot.stream.synchronize()
+        finally:
+            self._arena_buf, self.arena = saved_buf, saved_arena
+
     def stop(self) -> None:
+        """Stop admitting; batches in flight complete, requests still pending
+        are failed with RuntimeError."""
         self._stop.set()
-        self._sched.join()
+        if self._sched.is_alive():
+            self._sched.join()
         self.inflight.put(None)
-        self._done.join()
+        if self._done.is_alive():
+            self._done.join()
+        err = RuntimeError("the server stopped before this request was admitted")
+        while True:
+            try:
+                r = self.pending.get_nowait()
+            except queue.Empty:
+                break
+            self._fail([r], err)
         if self._frozen:
             self._frozen = False
             _gc_release()
 
+    @staticmethod
+    def _fail(requests: list[Request], err: BaseException) -> None:
+        t = time.perf_counter()
+        for r in requests:
+            r.error, r.t_done = err, t
+            r.done.set()
+
     # ------------------------------------------------------------- scheduler
-    def _admit(self) -> list[Request]:
-        """Up to max_batch pending requests, fewer if free HBM has shrunk."""
+    def _admit(self) -> tuple[list[Request], int]:
+        """Up to max_batch pending requests that fit the arena, and the
+        batch's arena offset (([], -1) when nothing is pending)."""
         try:
             first = self.pending.get(timeout=0.002)
         except queue.Empty:
-            return []
-        batch = [first]
-        free, _ = self.mem_info()
-        room = max(1, (free - self.hbm_reserve) // self.per_query_bytes)
-        limit = int(min(self.max_batch, room))
-        while len(batch) < limit:
-            try:
-                batch.append(self.pending.get_nowait())
-            except queue.Empty:
-                break
-        return batch
+            return [], -1
+        n = self.h * self.w
+        with self._free:
+            while True:
+                fit = 0
+                room = self.arena.largest_free()
+                while fit < self.max_batch and batch_bytes(n, self.size, self.k, fit + 1) <= room:
+                    fit += 1
+                if fit >= 1 or self._stop.is_set():
+                    break
+                self._free.wait(timeout=0.01)          # the arena is full: wait for a batch to finish
+            if fit < 1:
+                self.pending.put(first)
+                return [], -1
+            batch = [first]
+            while len(batch) < fit:
+                try:
+                    batch.append(self.pending.get_nowait())
+                except queue.Empty:
+                    break
+            off = self.arena.alloc(batch_bytes(n, self.size, self.k, len(batch)))
+        return batch, off
 
     def _free_slot(self) -> _Slot | None:
         with self._free:
@@ -268,34 +439,41 @@ class Server:
                 self._free.wait(timeout=0.01)
         return None
 
+    def _release(self, slot: _Slot) -> None:
+        with self._free:
+            if slot.lease is not None:
+                self.arena.free(*slot.lease)
+                slot.lease = None
+            slot.busy = False
+            slot.requests = []
+            self._free.notify_all()
+
     def _schedule(self) -> None:
-        lib = _lib.load()
         n = self.h * self.w
         while not self._stop.is_set():
             slot = self._free_slot()
             if slot is None:
                 return
-            batch = self._admit()
+            batch, off = self._admit()
             if not batch:
                 continue
             m = len(batch)
-            for i, r in enumerate(batch):
-                slot.h_images[i].numpy()[:] = r.image.reshape(-1)
             slot.busy = True
             slot.requests = batch
+            slot.lease = (off, batch_bytes(n, self.size, self.k, m))
+            try:
+                for i, r in enumerate(batch):
+                    slot.h_images[i].numpy()[:] = r.image.reshape(-1)
+                self._launch(slot, m, off)
+            except Exception as e:          # a failed batch answers its requests with the error
+                try:
+                    slot.stream.synchronize()   # nothing of the batch may still use its lease
+                except Exception:
+                    pass
+                self._fail(batch, e)
+                self._release(slot)
+                continue
             self.batches.append(m)
-            with torch.cuda.stream(slot.stream):
-                st = slot.stream.cuda_stream
-                slot.images[:m].copy_(slot.h_images[:m], non_blocking=True)
-                _lib.check(lib.prnu_preprocess_batch(slot.ctx.handle, _lib.INPUT_GRAY8, slot.images.data_ptr(), m * n,
-                                                     n, m, slot.patterns.data_ptr(), 4 * n * m, 4 * n, st),
-                           "prnu_preprocess_batch")
-                _lib.check(lib.prnu_fpstore_query(slot.ctx.handle, self.store.handle, slot.ptrs.data_ptr(), m,
-                                                  self.k, slot.idx.data_ptr(), slot.score.data_ptr(), None, st),
-                           "prnu_fpstore_query")
-                slot.h_idx[:m].copy_(slot.idx[:m], non_blocking=True)
-                slot.h_score[:m].copy_(slot.score[:m], non_blocking=True)
-                slot.event.record(slot.stream)
             self.inflight.put(slot)
 
     def _complete(self) -> None:
@@ -303,15 +481,15 @@ class Server:
             slot = self.inflight.get()
             if slot is None:
                 return
-            slot.event.synchronize()
-            t = time.perf_counter()
-            m = len(slot.requests)
-            idx = slot.h_idx[:m].numpy().copy()
-            score = slot.h_score[:m].numpy().copy()
-            for i, r in enumerate(slot.requests):
-                r.idx, r.score, r.t_done, r.batch_size = idx[i], score[i], t, m
-                r.done.set()
-            with self._free:
-                slot.busy = False
-                slot.requests = []
-                self._free.notify_all()
+            try:
+                slot.event.synchronize()
+                t = time.perf_counter()
+                m = len(slot.requests)
+                idx = slot.h_idx[:m].numpy().copy()
+                score = slot.h_score[:m].numpy().copy()
+                for i, r in enumerate(slot.requests):
+                    r.idx, r.score, r.t_done, r.batch_size = idx[i], score[i], t, m
+                    r.done.set()
+            except Exception as e:
+                self._fail(slot.requests, e)
+            self._release(slot)

@@ -24,6 +24,14 @@ import numpy as np
 
 from . import csi
 
+# slot-image layout (prnu_store_save): a 64-byte header, then per K block
+# `count` blocks of 128 bytes, then per slot FP64 sum and sum of squares and
+# FP32 max|x| and scale
+SLOT_HEADER_BYTES = 64
+SLOT_BLOCK_BYTES = 128
+SLOT_STATS_BYTES = 2 * 8 + 2 * 4
+SLOT_BLOCK_VALUES = {"split_bf16": 32, "fp32": 32, "fp16": 64, "fp8": 128}
+
 
 class HostPatternTier:
     """``count`` patterns of ``n`` values, in blocks of ``block`` slots, as
@@ -65,13 +73,45 @@ class HostPatternTier:
         return paths
 
     def load_files(self, directory: str) -> None:
-        """Blocks written by save_files back into pinned host memory."""
+        """Blocks written by save_files back into pinned host memory.  Each
+        file must be a whole slot image of its block: the header's pattern
+        length, format and count must match this tier and the block, and the
+        file must hold exactly the bytes that header describes (a truncated
+        or padded file raises ValueError before anything reaches a store)."""
         import torch
-        for b in range(self.nblocks):
-            data = np.fromfile(os.path.join(directory, f"block-{b:05d}.slots"), dtype=np.uint8)
+        paths = [os.path.join(directory, f"block-{b:05d}.slots") for b in range(self.nblocks)]
+        for b, path in enumerate(paths):            # every file checked before any is read whole
+            with open(path, "rb") as fh:
+                head = np.frombuffer(fh.read(SLOT_HEADER_BYTES), dtype=np.uint8)
+            r0, r1 = self.block_range(b)
+            self._check_image(head, r1 - r0, path, size=os.path.getsize(path))
+        for b, path in enumerate(paths):
+            data = np.fromfile(path, dtype=np.uint8)
+            r0, r1 = self.block_range(b)
+            self._check_image(data, r1 - r0, path)
             img = torch.empty(data.size, dtype=torch.uint8, pin_memory=True)
             img.numpy()[:] = data
             self.images[b] = img
+
+    def _check_image(self, data: np.ndarray, count: int, what: str, size: int | None = None) -> None:
+        """The slot-image header (magic "PRNUSLOT", int64 n, int32 format,
+        int32 count, int64 K blocks, 32 reserved bytes) against this tier;
+        ``size`` is the image's length in bytes (default: ``data.size``)."""
+        from . import _lib
+        size = data.size if size is None else int(size)
+        if data.size < SLOT_HEADER_BYTES or bytes(data[:8]) != b"PRNUSLOT":
+            raise ValueError(f"{what}: not a slot image")
+        n = int(data[8:16].view(np.int64)[0])
+        fmt, cnt = (int(v) for v in data[16:24].view(np.int32))
+        nkb = int(data[24:32].view(np.int64)[0])
+        per_block = SLOT_BLOCK_VALUES[self.format]
+        if n != self.n or fmt != _lib.PRECISIONS[self.format] or nkb != (self.n + per_block - 1) // per_block:
+            raise ValueError(f"{what}: image of {n} values in format {fmt}, tier of {self.n} in {self.format}")
+        if cnt != count:
+            raise ValueError(f"{what}: image of {cnt} slots for a block of {count}")
+        want = SLOT_HEADER_BYTES + count * (nkb * SLOT_BLOCK_BYTES + SLOT_STATS_BYTES)
+        if size != want:
+            raise ValueError(f"{what}: {size} bytes, a slot image of {count} slots has {want}")
 
     def host_bytes(self) -> int:
         return sum(int(img.numel()) for img in self._all_images())

@@ -187,3 +187,107 @@ def test_server_continuous_batching_matches_alone():
     ctx.close()
     ref.cleanup()
     store.close()
+
+
+def test_put_may_not_leave_unfilled_slots():
+    """Puts extend the filled prefix or overwrite it: a put starting past the
+    current size is refused, so a query never scans never-written slots."""
+    h, w = 32, 32
+    fp = fingerprints(20, h * w, seed=8)
+    store = serving.FingerprintStore(h, w, 20)
+    with pytest.raises(ValueError):
+        store.put(fp[10:20], first_slot=10)            # slots 0..9 never filled
+    assert store.info()["size"] == 0
+    store.put(fp[:10])
+    store.put(fp[10:20], first_slot=10)
+    store.put(fp[3:5], first_slot=3)                   # overwrite inside the prefix
+    assert store.info()["size"] == 20
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    idx, _ = (t.cpu().numpy() for t in store.query(ctx, queries_from(fp, [12, 17], seed=2, gain=2.0), k=3))
+    assert idx[0, 0] == 12 and idx[1, 0] == 17
+    ctx.close()
+    store.close()
+
+
+def _server_fixture(h, w, cams, enrolled, nq):
+    from rocket_amd import synthetic
+    images = synthetic.make_images(nq + 4 * cams, h, w, cams, workers=1)
+    ref = csi.PRNUFilter(h, w)
+    pats = torch.empty((len(images), h * w), device="cuda")
+    ref.apply_gpu_batch(torch.from_numpy(images.reshape(len(images), -1)).cuda(), pats, _lib.INPUT_GRAY8)
+    fp = torch.stack([pats[[i for i in range(nq, len(images)) if i % cams == c]].mean(dim=0)
+                      for c in range(enrolled)])
+    fp = (fp - fp.mean(dim=1, keepdim=True)).contiguous()
+    ref.cleanup()
+    store = serving.FingerprintStore(h, w, enrolled)
+    store.put(fp)
+    return images[:nq], store
+
+
+def _alone(store, h, w, image, k):
+    ctx = csi._Ctx(h, w, _lib.MODE_NCC)
+    pat = torch.empty((1, h * w), device="cuda")
+    prep = csi.PRNUFilter(h, w, _ctx=ctx)
+    prep.apply_gpu(torch.from_numpy(np.ascontiguousarray(image)).cuda(), pat[0], _lib.INPUT_GRAY8)
+    idx, sc = (t.cpu().numpy() for t in store.query(ctx, pat, k=k))
+    ctx.close()
+    return idx[0], sc[0]
+
+
+def test_server_admission_bounded_by_hbm_budget():
+    """An HBM budget of three queries' buffers binds: with two slots and
+    max_batch 16, no batch holds more than three queries, the queries in
+    flight never hold more than the budget, and every answer is the answer
+    of that image alone."""
+    h, w, cams, k = 48, 64, 8, 4
+    queries, store = _server_fixture(h, w, cams, cams, 30)
+    budget = serving.batch_bytes(h * w, cams, k, 3)
+    srv = serving.Server(store, k=k, max_batch=16, slots=2, hbm_budget=budget).start()
+    assert srv.arena.size == budget
+    reqs = [srv.submit(img) for img in queries]
+    for r in reqs:
+        idx, sc = r.result(60)
+    srv.stop()
+    assert max(srv.batches) <= 3 and sum(srv.batches) == len(queries)
+    assert srv.arena.high_water <= budget and srv.arena.used == 0
+    for img, r in zip(queries, reqs):
+        idx1, sc1 = _alone(store, h, w, img, k)
+        assert np.array_equal(idx1, r.idx) and np.array_equal(sc1, r.score)
+    store.close()
+
+
+def test_server_failed_batch_answers_with_error():
+    """A batch whose launch fails answers its requests with the error and
+    frees its slot and arena range; later batches are served; a wrongly sized
+    image is refused at submit; requests pending at stop() are failed."""
+    h, w, cams, k = 32, 48, 4, 3
+    queries, store = _server_fixture(h, w, cams, cams, 12)
+    srv = serving.Server(store, k=k, max_batch=4, slots=1)
+    with pytest.raises(ValueError):
+        srv.submit(np.zeros(h * w + 1, dtype=np.uint8))
+    real = srv._launch
+    calls = {"n": 0}
+
+    def flaky(slot, m, off):
+        calls["n"] += 1
+        if calls["n"] == 2:                        # the first batch after start()'s warm-up of the slot
+            raise RuntimeError("injected launch failure")
+        return real(slot, m, off)
+    srv._launch = flaky
+    first = [srv.submit(img) for img in queries[:4]]   # queued before start: one batch of four
+    srv.start()
+    for r in first:
+        assert r.wait(60)
+    assert all(isinstance(r.error, RuntimeError) for r in first)
+    with pytest.raises(RuntimeError):
+        first[0].result(1)
+    second = [srv.submit(img) for img in queries[4:]]
+    for img, r in zip(queries[4:], second):
+        idx, sc = r.result(60)
+        idx1, sc1 = _alone(store, h, w, img, k)
+        assert np.array_equal(idx1, idx) and np.array_equal(sc1, sc)
+    srv.stop()
+    assert srv.arena.used == 0
+    with pytest.raises(RuntimeError):
+        srv.submit(queries[0])
+    store.close()

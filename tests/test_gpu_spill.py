@@ -1,9 +1,10 @@
 """GPU tests of the pattern spill tier (prnu_store_save / prnu_store_load,
 rocket_amd/spill.py): slot images round-trip a store's slots and statistics
 bit for bit; the block-streamed all-pairs NCC of a host tier (ragged last
-block, files round trip included) equals the resident store's Gram bit for
-bit when both use the same K slabs; images of another store geometry are
-refused."""
+block, files round trip included) matches the FP64 oracle (split-BF16:
+|dNCC| <= 1e-5; FP16: its rounding bound) and equals the resident store's
+Gram bit for bit when both use the same K slabs; images of another store
+geometry, truncated images and truncated files are refused."""
 import ctypes
 
 import numpy as np
@@ -11,7 +12,9 @@ import pytest
 
 torch = pytest.importorskip("torch")
 
+from oracle import prnu_oracle as O  # noqa: E402
 from rocket_amd import _lib, csi, spill  # noqa: E402
+from test_gpu_groups import fp16_bound  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -109,3 +112,63 @@ def test_load_refuses_other_geometry():
     with pytest.raises(_lib.PrnuError):
         a.load(0, junk)
     a.close()
+
+
+@pytest.mark.parametrize("fmt", ["split_bf16", "fp16"])
+def test_host_tier_ncc_matches_oracle(fmt, tmp_path):
+    """All pairs of 300 patterns streamed in blocks of 128 (ragged last block)
+    through a device store of 3 blocks (one resident row block, so every
+    later block streams), and again after a round trip through files,
+    against the FP64 oracle NCC of the FP32 patterns."""
+    n, count, block = 96 * 80, 300, 128
+    p = patterns(count, n, 4, seed=17)
+    want = O.ncc_matrix(p.cpu().numpy())
+    off = ~np.eye(count, dtype=bool)
+    gram = csi.GramNCC(8, 8)
+    tier = spill.HostPatternTier(n, count, block, fmt)
+    for b in range(tier.nblocks):
+        r0, r1 = tier.block_range(b)
+        tier.put_block(b, p[r0:r1])
+
+    def check(got):
+        assert np.array_equal(got, got.T)
+        if fmt == "split_bf16":
+            assert np.abs(got - want).max() <= 1e-5
+        else:
+            b_off, b_diag = fp16_bound(n)
+            assert np.abs(got - want)[off].max() <= b_off
+            assert np.abs(np.diag(got) - np.diag(want)).max() <= b_diag
+
+    check(tier.ncc(gram, device_blocks=3))
+    paths = tier.save_files(str(tmp_path / "tier"))
+    back = spill.HostPatternTier(n, count, block, fmt)
+    back.load_files(str(tmp_path / "tier"))
+    check(back.ncc(gram, device_blocks=4))
+    # a truncated block file is refused before any copy
+    data = np.fromfile(paths[1], dtype=np.uint8)
+    data[:-1].tofile(paths[1])
+    with pytest.raises(ValueError):
+        spill.HostPatternTier(n, count, block, fmt).load_files(str(tmp_path / "tier"))
+    tier.close()
+    back.close()
+    gram.close()
+
+
+def test_load_refuses_short_buffer():
+    """prnu_store_load reads no byte past the buffer it is given: an image
+    whose buffer is shorter than its header's count describes is refused."""
+    p = patterns(12, 40 * 40, 1, seed=5)
+    a = csi.PatternStore(40 * 40, 12, "split_bf16")
+    a.put(p)
+    img = torch.empty(a.image_bytes(12), dtype=torch.uint8, pin_memory=True)
+    a.save(0, 12, img)
+    b = csi.PatternStore(40 * 40, 12, "split_bf16")
+    with pytest.raises(_lib.PrnuError):
+        b.load(0, img[:-1])
+    with pytest.raises(_lib.PrnuError):
+        b.load(0, img[:32])                     # shorter than the header
+    b.load(0, img)
+    for x, y in zip(a.stats(12), b.stats(12)):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()

@@ -85,3 +85,28 @@ def test_tile_blocks_cover_the_triangle_once(count, tile):
             cov[r0:r0 + rn, c0:c0 + cn] += 1
     iu = np.triu_indices(count, 1)
     assert (cov[iu] == 1).all() and cov.sum() == len(iu[0])
+
+
+# Golden curve orders, derived by hand from the rule table of
+# HilbertCurveIndexSpace.java:10-20 ({p, q, next rule} per quadrant, p the
+# left (row) half, q the right (column) half) and the recursion of
+# launchSplits (:51-98): rule A visits quadrants (0,0)->D, (0,1)->A,
+# (1,1)->A, (1,0)->B; B: (1,1)->C, (0,1)->B, (0,0)->B, (1,0)->A;
+# C: (1,1)->B, (1,0)->C, (0,0)->C, (0,1)->D; D: (0,0)->A, (1,0)->D,
+# (1,1)->D, (0,1)->C.  Quadrants (and cells) with no pair i < j (i <= j with
+# the diagonal) are skipped.
+HILBERT_8_NO_DIAG = [
+    # rows 0-3 x cols 0-3, rule D: (0-1 x 0-1, A), (2-3 x 2-3, D), (0-1 x 2-3, C)
+    (0, 1), (2, 3), (1, 3), (1, 2), (0, 2), (0, 3),
+    # rows 0-3 x cols 4-7, rule A: (0-1 x 4-5, D), (0-1 x 6-7, A), (2-3 x 6-7, A), (2-3 x 4-5, B)
+    (0, 4), (1, 4), (1, 5), (0, 5), (0, 6), (0, 7), (1, 7), (1, 6),
+    (2, 6), (2, 7), (3, 7), (3, 6), (3, 5), (2, 5), (2, 4), (3, 4),
+    # rows 4-7 x cols 4-7, rule A: (4-5 x 4-5, D), (4-5 x 6-7, A), (6-7 x 6-7, A)
+    (4, 5), (4, 6), (4, 7), (5, 7), (5, 6), (6, 7),
+]
+HILBERT_4_DIAG = [(0, 0), (1, 1), (0, 1), (0, 2), (0, 3), (1, 3), (1, 2), (2, 2), (2, 3), (3, 3)]
+
+
+def test_hilbert_golden_order():
+    assert [tuple(p) for p in ix.all_pairs(8, "hilbert", 1, False)] == HILBERT_8_NO_DIAG
+    assert [tuple(p) for p in ix.all_pairs(4, "hilbert", 1, True)] == HILBERT_4_DIAG
