@@ -147,7 +147,8 @@ def c0_bench(torch):
 
 def measured_peaks(torch):
     """What the box delivers at the clock it runs: a 4 GiB device copy (read +
-    write bytes / time) and 8192^3 BF16 and FP16 GEMMs through torch
+    write bytes / time), dense BF16 and FP8 MFMA from registers only (no
+    memory traffic), and 8192^3 BF16 and FP16 GEMMs through torch
     (hipBLASLt), best of 5."""
     def best(fn, reps=5):
         fn()
@@ -164,6 +165,14 @@ def measured_peaks(torch):
     t = best(lambda: y.copy_(x))
     res = {"hbm_copy_GBps": 2 * 4 * (1 << 30) / t / 1e9}
     del x, y
+    # dense MFMA on register operands only (prnu_measure_mfma_peak): the
+    # ceiling at the clock the chip holds under a full matrix load
+    from rocket_amd import _lib
+    lib = _lib.load()
+    for prec, name, iters in ((_lib.PREC_SPLIT_BF16, "bf16", 200000), (_lib.PREC_FP8, "fp8", 100000)):
+        tf, ms = ctypes.c_double(), ctypes.c_float()
+        _lib.check(lib.prnu_measure_mfma_peak(prec, iters, ctypes.byref(tf), ctypes.byref(ms)), "mfma peak")
+        res[f"{name}_mfma_register_only_TFLOPs"] = tf.value
     m = 8192
     for dt, name in ((torch.bfloat16, "bf16"), (torch.float16, "fp16")):
         a = torch.randn((m, m), device="cuda").to(dt)

@@ -185,6 +185,7 @@ PRNU_EXPORT int prnu_filter_spectral(prnu_ctx *ctx, const float *d_image, float 
 PRNU_EXPORT int prnu_pattern_stats(const float *const *d_patterns, int count, int64_t n,
                        double *d_sum, double *d_sumsq, float *d_absmax, prnu_stream stream);
 
+
 /* All-pairs NCC of `count` patterns of the context's size as a triangular
  * Gram on MFMA (`precision` is a prnu_precision), with the FP64 epilogue
  * NCC = G_ij / (sqrt(Sxx_i - Sx_i^2) sqrt(Sxx_j - Sx_j^2)) of
@@ -207,7 +208,9 @@ PRNU_EXPORT int prnu_gram_ncc(prnu_ctx *ctx, const float *const *d_patterns, int
  * [2^14, 2^15)) for `format` = PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32,
  * PRNU_PREC_FP8 or PRNU_PREC_FP16.  Inserting computes
  * the per-pattern FP64 statistics (FP64 sums of FP32 products, as the
- * reference's NCC), so the NCC epilogue is exact whatever the format. */
+ * reference's NCC), so the NCC epilogue is exact whatever the format.
+ * capacity <= 16,776,960 slots (one block column of the store, capacity
+ * rounded up to 256 times 128 bytes, stays below 2^31 bytes). */
 typedef struct prnu_store prnu_store;
 
 PRNU_EXPORT int prnu_store_create(int64_t n, int capacity, int format, prnu_store **out);
@@ -341,6 +344,15 @@ PRNU_EXPORT int prnu_fpstore_put(prnu_fpstore *store, const float *const *d_patt
  * other queries of the call. */
 PRNU_EXPORT int prnu_fpstore_query(prnu_ctx *ctx, prnu_fpstore *store, const float *const *d_queries, int nq, int k,
                                    int32_t *d_idx, double *d_score, double *d_scan, prnu_stream stream);
+
+/* Measurement (no reference counterpart): the dense MFMA rate this device
+ * sustains at the clock it holds under a full matrix load.  Every wave issues
+ * independent MFMAs on register operands only (2 waves per SIMD on every CU),
+ * `iterations` rounds of 16 per wave; precision selects the instruction:
+ * PRNU_PREC_SPLIT_BF16 -> 16x16x32 BF16 (one product), PRNU_PREC_FP16 ->
+ * 16x16x32 FP16, PRNU_PREC_FP8 -> 16x16x128 e4m3.  Best of 3 timed launches
+ * after a warm-up; *tflops = FLOP / time, *ms (optional) = that time. */
+PRNU_EXPORT int prnu_measure_mfma_peak(int precision, int iterations, double *tflops, float *ms);
 
 /* Launch shape of the Gram tile kernel (no reference counterpart).
  * slabs: K slabs per tile (0 = automatic); xcd_remap: 1 keeps the tiles of one

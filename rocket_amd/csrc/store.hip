@@ -55,6 +55,7 @@ constexpr int kTileS = 256;
 constexpr int kRowsS = 2 * kTileS;
 constexpr int kThrS = 512;
 constexpr int kMaxPlanGroups = 4096;
+constexpr int kMaxStoreSlots = (1 << 24) - kTileS;  // a block column (cap_pad x 128 B) stays below 2^31 bytes
 
 __device__ inline uint16_t bf16_bits_s(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 __device__ inline float bf16_val_s(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
@@ -401,25 +402,35 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 
     // Register-staged loads (DMA = false): global -> VGPR st[] -> LDS after
     // the MFMAs, rows past count zeroed.  Direct-to-LDS loads (DMA = true,
-    // global_load_lds_dwordx4): each wave instruction fills 1 KiB of LDS, lane
-    // i at byte 16*i, i.e. rows r0 + 64*it of this thread's wave, LDS slot
-    // (tid & 7); the lane therefore fetches global chunk (tid & 7) ^ (r & 7),
-    // which is the XOR swizzle s_off reads.  Rows past count hold a clamped
-    // duplicate: they only feed output rows/columns >= count, which the
-    // reduction never reads.
+    // buffer_load_dwordx4 ... lds): each wave instruction fills 1 KiB of LDS,
+    // lane i at byte 16*i, i.e. rows r0 + 64*it of this thread's wave, LDS
+    // slot (tid & 7); the lane therefore fetches global chunk
+    // (tid & 7) ^ (r & 7), which is the XOR swizzle s_off reads.  Rows past
+    // count hold a clamped duplicate: they only feed output rows/columns
+    // >= count, which the reduction never reads.  The buffer form (a
+    // resource per block column, 32-bit row offsets; the column is below
+    // 2 GiB) is tracked by the compiler under vmcnt alone, so the fragment
+    // reads of the stage in LDS keep in-order lgkmcnt(k) waits; the global
+    // form would force lgkmcnt(0) before every fragment read while a fill is
+    // in flight.
     uint4 st[DMA ? 1 : 8];
     const int dchunk = chunk ^ (r0 & 7);
+    uint32_t voff[DMA ? 8 : 1];
+    if constexpr (DMA) {
+#pragma unroll
+        for (int it = 0; it < 8; ++it) voff[it] = (uint32_t)row_off(it) + (uint32_t)(dchunk << 4);
+    }
     auto load_to = [&](int64_t b, uint8_t *buf) {
         if constexpr (DMA) {
-            const uint8_t *col = blocks + b * cap_bytes + (dchunk << 4);
+            const __amdgpu_buffer_rsrc_t rsrc =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(blocks + b * cap_bytes), 0, (int)cap_bytes, 0x00020000);
             const int wbase = (tid >> 6) * 8;  // first row of this wave within a 64-row group
 #pragma unroll
             for (int it = 0; it < 8; ++it) {
                 if (it < 4 || !half) {
-                    typedef const __attribute__((address_space(1))) void *gptr;
                     typedef __attribute__((address_space(3))) void *lptr;
-                    __builtin_amdgcn_global_load_lds((gptr)(col + row_off(it)),
-                                                     (lptr)(buf + (64 * it + wbase) * kBlkBytes), 16, 0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lptr)(buf + (64 * it + wbase) * kBlkBytes), 16,
+                                                             voff[it], 0, 0, 0);
                 }
             }
         } else {
@@ -499,25 +510,60 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
                 }
             }
         } else if (kSplit) {
+            // Fragments are read in the order the MFMAs use them: first what
+            // the first MFMAs need (A lo of row blocks 0 and 1, B hi), then
+            // the rest of pair 0 of row blocks; the A fragments of pair p + 1
+            // are issued after the first eight MFMAs of pair p.  LDS returns
+            // reads in order, so each MFMA waits (lgkmcnt(k), k > 0) only for
+            // the reads it uses, not for a burst of twelve; at most 15 reads
+            // are ever outstanding (the counter's range).  Per accumulator
+            // the products keep the order lo*hi, hi*lo, hi*hi; the two row
+            // blocks of a pair alternate, so dependent MFMAs are eight apart.
             const int ch = lane >> 4;  // hi chunks 0..3, lo chunks 4..7
-            bf16x8 bh[4], bl[4];
+            auto rd = [&](int r, int c) { return *reinterpret_cast<const bf16x8 *>(buf + s_off(r, c)); };
+            bf16x8 bh[4], bl[4], ah[4], al[4];  // A fragments: two pairs of row blocks in flight
+            al[0] = rd(wrow0 + frow, ch + 4);
+            al[1] = rd(wrow0 + 16 + frow, ch + 4);
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                const int r = wcol0 + cb * 16 + frow;
-                bh[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
-                bl[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
-            }
+            for (int cb = 0; cb < 4; ++cb) bh[cb] = rd(wcol0 + cb * 16 + frow, ch);
+            __builtin_amdgcn_sched_barrier(0);
+            ah[0] = rd(wrow0 + frow, ch);
+            ah[1] = rd(wrow0 + 16 + frow, ch);
 #pragma unroll
-            for (int rb = 0; rb < 8; ++rb) {
-                const int r = wrow0 + rb * 16 + frow;
-                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
-                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+            for (int cb = 0; cb < 4; ++cb) bl[cb] = rd(wcol0 + cb * 16 + frow, ch + 4);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) {
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+            for (int pr = 0; pr < 4; ++pr) {
+                const int c0 = (2 * pr) & 3, c1 = (2 * pr + 1) & 3;  // this pair's A slots
+                const int rb0 = 2 * pr, rb1 = 2 * pr + 1;
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c0], bh[cb], acc[rb0][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c1], bh[cb], acc[rb1][cb], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (pr + 1 < 4) {
+                    const int n0 = (2 * pr + 2) & 3, n1 = (2 * pr + 3) & 3;
+                    const int r = wrow0 + (2 * pr + 2) * 16 + frow;
+                    al[n0] = rd(r, ch + 4);
+                    al[n1] = rd(r + 16, ch + 4);
+                    ah[n0] = rd(r, ch);
+                    ah[n1] = rd(r + 16, ch);
                 }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c0], bl[cb], acc[rb0][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c1], bl[cb], acc[rb1][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c0], bh[cb], acc[rb0][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[rb1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c1], bh[cb], acc[rb1][cb], 0, 0, 0);
             }
         } else {
 #pragma unroll
@@ -1041,6 +1087,10 @@ int64_t choose_kcb(const int *items, const int *tiles, const int64_t *nkb, int n
 int store_create(int64_t n, int capacity, int format, Store **out) {
     *out = nullptr;
     PRNU_REQUIRE(n >= 1 && capacity >= 1, "store_create: n and capacity must be positive");
+    // one block column (capacity x 128 bytes) is addressed by a buffer
+    // resource with 32-bit offsets in the Gram kernels
+    PRNU_REQUIRE(capacity <= kMaxStoreSlots, "store_create: capacity %d exceeds %d slots", capacity,
+                 kMaxStoreSlots);
     PRNU_REQUIRE(format == PRNU_PREC_SPLIT_BF16 || format == PRNU_PREC_FP32 || format == PRNU_PREC_FP8 ||
                      format == PRNU_PREC_FP16,
                  "store_create: format must be PRNU_PREC_SPLIT_BF16, PRNU_PREC_FP32, PRNU_PREC_FP8 or "
