@@ -148,7 +148,9 @@ def c0_bench(torch):
 def measured_peaks(torch):
     """What the box delivers at the clock it runs: a 4 GiB device copy (read +
     write bytes / time), dense BF16 and FP8 MFMA from registers only (no
-    memory traffic), and 8192^3 BF16 and FP16 GEMMs through torch
+    memory traffic; noise-like random operands, and small constants, on
+    which the chip holds a higher clock), and 8192^3 BF16 and FP16 GEMMs
+    through torch
     (hipBLASLt), best of 5."""
     def best(fn, reps=5):
         fn()
@@ -170,9 +172,10 @@ def measured_peaks(torch):
     from rocket_amd import _lib
     lib = _lib.load()
     for prec, name, iters in ((_lib.PREC_SPLIT_BF16, "bf16", 200000), (_lib.PREC_FP8, "fp8", 100000)):
-        tf, ms = ctypes.c_double(), ctypes.c_float()
-        _lib.check(lib.prnu_measure_mfma_peak(prec, iters, ctypes.byref(tf), ctypes.byref(ms)), "mfma peak")
-        res[f"{name}_mfma_register_only_TFLOPs"] = tf.value
+        for rnd, tag in ((1, "random"), (0, "constant")):
+            tf, ms = ctypes.c_double(), ctypes.c_float()
+            _lib.check(lib.prnu_measure_mfma_peak(prec, rnd, iters, ctypes.byref(tf), ctypes.byref(ms)), "mfma peak")
+            res[f"{name}_mfma_register_only_{tag}_operands_TFLOPs"] = tf.value
     m = 8192
     for dt, name in ((torch.bfloat16, "bf16"), (torch.float16, "fp16")):
         a = torch.randn((m, m), device="cuda").to(dt)

@@ -348,11 +348,15 @@ PRNU_EXPORT int prnu_fpstore_query(prnu_ctx *ctx, prnu_fpstore *store, const flo
 /* Measurement (no reference counterpart): the dense MFMA rate this device
  * sustains at the clock it holds under a full matrix load.  Every wave issues
  * independent MFMAs on register operands only (2 waves per SIMD on every CU),
- * `iterations` rounds of 16 per wave; precision selects the instruction:
+ * `iterations` rounds of 32 per wave; precision selects the instruction:
  * PRNU_PREC_SPLIT_BF16 -> 16x16x32 BF16 (one product), PRNU_PREC_FP16 ->
- * 16x16x32 FP16, PRNU_PREC_FP8 -> 16x16x128 e4m3.  Best of 3 timed launches
- * after a warm-up; *tflops = FLOP / time, *ms (optional) = that time. */
-PRNU_EXPORT int prnu_measure_mfma_peak(int precision, int iterations, double *tflops, float *ms);
+ * 16x16x32 FP16, PRNU_PREC_FP8 -> 16x16x128 e4m3.  random_operands = 1 fills
+ * the operands with hashed bits (noise-like data: the power drawn, hence the
+ * clock the chip holds, depends on how many operand bits toggle), 0 with
+ * small constants.  Best of 3 timed launches after a warm-up; *tflops =
+ * FLOP / time, *ms (optional) = that time. */
+PRNU_EXPORT int prnu_measure_mfma_peak(int precision, int random_operands, int iterations, double *tflops,
+                                       float *ms);
 
 /* Launch shape of the Gram tile kernel (no reference counterpart).
  * slabs: K slabs per tile (0 = automatic); xcd_remap: 1 keeps the tiles of one
@@ -362,7 +366,8 @@ PRNU_EXPORT int prnu_measure_mfma_peak(int precision, int iterations, double *tf
  * flight; for the store Gram and grouped plans, 0 = direct global-to-LDS
  * loads, 1 = loads staged through registers, 2 = direct loads with the wave
  * priority raised over the MFMA block, 3 (store Gram of split-BF16 and FP8
- * stores) = 4 waves of 128 x 128 with accumulators in AGPRs. */
+ * stores) = 4 waves of 128 x 128 with accumulators in AGPRs.  Every variant
+ * gives bitwise the same results. */
 typedef struct {
     int32_t slabs;
     int32_t xcd_remap;

@@ -14,14 +14,28 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 typedef int intx8 __attribute__((ext_vector_type(8)));
 
-constexpr int kChains = 16;  // independent accumulators per wave
+constexpr int kChains = 32;  // independent accumulators per wave (8 x 4, as a Gram wave's 128 x 64 block)
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {  // an integer hash (bit avalanche)
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
 
 // FMT: PRNU_PREC_SPLIT_BF16 -> v_mfma_f32_16x16x32_bf16, PRNU_PREC_FP16 ->
 // v_mfma_f32_16x16x32_f16, PRNU_PREC_FP8 -> v_mfma_scale_f32_16x16x128_f8f6f4
-// (e4m3, unit scales).  `magic` is never equal to the result; the store
-// behind it keeps the loop from being removed.
-template <int FMT>
-__global__ __launch_bounds__(256) void mfma_peak_kernel(int iters, float magic, float *sink) {
+// (e4m3, unit scales).  Chain j multiplies A operand (j >> 2) & 3 by B operand
+// j & 3.
+// RANDOM: the operands hold hashed bits (values in (-2, 2), every mantissa bit
+// toggling between operands, as with noise-like patterns); else small
+// constants (few toggling bits: the chip holds a higher clock on them).
+// `magic` is never equal to the result; the store behind it keeps the loop
+// from being removed.
+template <int FMT, bool RANDOM>
+__global__ __launch_bounds__(512, 1) void mfma_peak_kernel(int iters, float magic, float *sink) {
     const int t = threadIdx.x;
     floatx4 acc[kChains];
 #pragma unroll
@@ -29,39 +43,56 @@ __global__ __launch_bounds__(256) void mfma_peak_kernel(int iters, float magic, 
         const float v = (float)(j * 64 + (t & 63));
         acc[j] = floatx4{v, v + 1.f, v + 2.f, v + 3.f};
     }
+    const uint32_t seed = (uint32_t)(blockIdx.x * 512 + t) * 64u;
+    auto val = [&](int k, int e) -> float {
+        if (!RANDOM) return k < 4 ? 1.0f + 0.001f * (t & 7) : 0.5f - 0.001f * (t & 3);
+        return (float)(int)(mix32(seed + (uint32_t)(k * 8 + e)) & 0xffffu) * (1.0f / 16384.0f) - 2.0f;
+    };
     if constexpr (FMT == PRNU_PREC_FP8) {
-        intx8 a, b;
+        intx8 a[4], b[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            a[e] = 0x38383838 ^ (t & 1);
-            b[e] = 0x30303030 ^ ((t >> 1) & 1);
-        }
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                // e4m3 bytes with the top exponent bit clear: no NaN, |x| < 2
+                const uint32_t ra = RANDOM ? mix32(seed + (uint32_t)(k * 8 + e)) : 0x38383838u ^ (uint32_t)(t & 1);
+                const uint32_t rb = RANDOM ? mix32(seed + 4096u + (uint32_t)(k * 8 + e)) : 0x30303030u;
+                a[k][e] = (int)(ra & 0xBFBFBFBFu);
+                b[k][e] = (int)(rb & 0xBFBFBFBFu);
+            }
         for (int i = 0; i < iters; ++i) {
 #pragma unroll
             for (int j = 0; j < kChains; ++j)
-                acc[j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc[j], 0, 0, 0, 127, 0, 127);
+                acc[j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[(j >> 2) & 3], b[j & 3], acc[j], 0, 0, 0, 127, 0,
+                                                                          127);
         }
     } else if constexpr (FMT == PRNU_PREC_FP16) {
-        halfx8 a, b;
+        halfx8 a[4], b[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            a[e] = (_Float16)(1.0f + 0.001f * (t & 7));
-            b[e] = (_Float16)(0.5f - 0.001f * (t & 3));
-        }
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                a[k][e] = (_Float16)val(k, e);
+                b[k][e] = (_Float16)val(k + 4, e);
+            }
         for (int i = 0; i < iters; ++i) {
 #pragma unroll
-            for (int j = 0; j < kChains; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
+            for (int j = 0; j < kChains; ++j)
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[(j >> 2) & 3], b[j & 3], acc[j], 0, 0, 0);
         }
     } else {
-        bf16x8 a, b;
+        bf16x8 a[4], b[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            a[e] = (__bf16)(1.0f + 0.001f * (t & 7));
-            b[e] = (__bf16)(0.5f - 0.001f * (t & 3));
-        }
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                a[k][e] = (__bf16)val(k, e);
+                b[k][e] = (__bf16)val(k + 4, e);
+            }
         for (int i = 0; i < iters; ++i) {
 #pragma unroll
-            for (int j = 0; j < kChains; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+            for (int j = 0; j < kChains; ++j)
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[(j >> 2) & 3], b[j & 3], acc[j], 0, 0, 0);
         }
     }
     float s = 0.f;
@@ -77,7 +108,7 @@ using namespace prnu;
 
 extern "C" {
 
-int prnu_measure_mfma_peak(int precision, int iterations, double *tflops, float *ms) {
+int prnu_measure_mfma_peak(int precision, int random_operands, int iterations, double *tflops, float *ms) {
     PRNU_REQUIRE(tflops, "measure_mfma_peak: null output");
     PRNU_REQUIRE(precision == PRNU_PREC_SPLIT_BF16 || precision == PRNU_PREC_FP16 || precision == PRNU_PREC_FP8,
                  "measure_mfma_peak: precision %d has no dense MFMA probe (bf16 = split-BF16, fp16, fp8)",
@@ -88,20 +119,24 @@ int prnu_measure_mfma_peak(int precision, int iterations, double *tflops, float 
     PRNU_CHECK_HIP(hipGetDevice(&dev));
     int cus = 0;
     PRNU_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const unsigned grid = (unsigned)cus * 2;  // 2 workgroups of 4 waves per CU: 2 waves per SIMD
+    const unsigned grid = (unsigned)cus;  // one workgroup of 8 waves per CU: 2 waves per SIMD
     float *sink = nullptr;
     PRNU_CHECK_HIP(hipMalloc(&sink, 256));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     auto launch = [&](int it) {
-        if (precision == PRNU_PREC_FP8)
-            hipLaunchKernelGGL(mfma_peak_kernel<PRNU_PREC_FP8>, dim3(grid), dim3(256), 0, nullptr, it, -1.0f, sink);
-        else if (precision == PRNU_PREC_FP16)
-            hipLaunchKernelGGL(mfma_peak_kernel<PRNU_PREC_FP16>, dim3(grid), dim3(256), 0, nullptr, it, -1.0f, sink);
-        else
-            hipLaunchKernelGGL(mfma_peak_kernel<PRNU_PREC_SPLIT_BF16>, dim3(grid), dim3(256), 0, nullptr, it, -1.0f,
-                               sink);
+#define PRNU_PEAK(F, R) hipLaunchKernelGGL((mfma_peak_kernel<F, R>), dim3(grid), dim3(512), 0, nullptr, it, -1.0f, sink)
+        if (random_operands) {
+            if (precision == PRNU_PREC_FP8) PRNU_PEAK(PRNU_PREC_FP8, true);
+            else if (precision == PRNU_PREC_FP16) PRNU_PEAK(PRNU_PREC_FP16, true);
+            else PRNU_PEAK(PRNU_PREC_SPLIT_BF16, true);
+        } else {
+            if (precision == PRNU_PREC_FP8) PRNU_PEAK(PRNU_PREC_FP8, false);
+            else if (precision == PRNU_PREC_FP16) PRNU_PEAK(PRNU_PREC_FP16, false);
+            else PRNU_PEAK(PRNU_PREC_SPLIT_BF16, false);
+        }
+#undef PRNU_PEAK
     };
     float best = 0.f;
     if (e == hipSuccess) {
@@ -125,7 +160,7 @@ int prnu_measure_mfma_peak(int precision, int iterations, double *tflops, float 
         return PRNU_EDEVICE;
     }
     const double flop_per_mfma = precision == PRNU_PREC_FP8 ? 2.0 * 16 * 16 * 128 : 2.0 * 16 * 16 * 32;
-    const double flop = flop_per_mfma * kChains * (double)iterations * 4.0 * grid;
+    const double flop = flop_per_mfma * kChains * (double)iterations * 8.0 * grid;
     *tflops = flop / (best * 1e-3) / 1e12;
     if (ms) *ms = best;
     return PRNU_OK;

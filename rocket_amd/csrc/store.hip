@@ -811,24 +811,46 @@ __global__ __launch_bounds__(kThr4, 1) void store_gram4_kernel(StoreGramArgs a) 
                 }
             }
         } else {
-            bf16x8 bh[8], bl[8];
+            // Reads in use order: A lo of row block 0 and B hi first (the
+            // first eight MFMAs), then A hi and B lo, then each next row
+            // block's A one row block (24 MFMAs) ahead; in-order LDS returns
+            // give each MFMA an lgkmcnt(k) wait for just its operands.  Per
+            // accumulator the products stay lo*hi, hi*lo, hi*hi, eight MFMAs
+            // apart.
+            auto rd = [&](int r, int c) { return *reinterpret_cast<const bf16x8 *>(buf + s_off(r, c)); };
+            bf16x8 bh[8], bl[8], ah[2], al[2];
+            al[0] = rd(a_off + frow, ch + 4);
 #pragma unroll
-            for (int cb = 0; cb < 8; ++cb) {
-                const int r = b_off + cb * 16 + frow;
-                bh[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
-                bl[cb] = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
-            }
+            for (int cb = 0; cb < 8; ++cb) bh[cb] = rd(b_off + cb * 16 + frow, ch);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int rb = 0; rb < 8; ++rb) {
-                const int r = a_off + rb * 16 + frow;
-                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch));
-                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(buf + s_off(r, ch + 4));
+                const int c = rb & 1;
 #pragma unroll
-                for (int cb = 0; cb < 8; ++cb) {
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[rb][cb], 0, 0, 0);
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+                for (int cb = 0; cb < 8; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], bh[cb], acc[rb][cb], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (rb == 0) {
+                    ah[0] = rd(a_off + frow, ch);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int cb = 0; cb < 8; ++cb) {
+                        bl[cb] = rd(b_off + cb * 16 + frow, ch + 4);
+                        __builtin_amdgcn_sched_barrier(0);  // in use order
+                    }
                 }
+                if (rb + 1 < 8) {
+                    const int r = a_off + (rb + 1) * 16 + frow;
+                    al[c ^ 1] = rd(r, ch + 4);
+                    ah[c ^ 1] = rd(r, ch);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bl[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bh[cb], acc[rb][cb], 0, 0, 0);
             }
         }
     };
