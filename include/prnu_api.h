@@ -185,6 +185,19 @@ PRNU_EXPORT int prnu_filter_spectral(prnu_ctx *ctx, const float *d_image, float 
 PRNU_EXPORT int prnu_pattern_stats(const float *const *d_patterns, int count, int64_t n,
                        double *d_sum, double *d_sumsq, float *d_absmax, prnu_stream stream);
 
+/* prnu_pattern_stats without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  The
+ * same kernels and the same results, bit for bit; the partial sums go to a
+ * fixed 3 MiB device array per GPU (131072 partials of 24 bytes) instead of a workspace,
+ * so a call may cover at most 131072 / ceil(n / 65536) patterns (e.g. 2730 of
+ * 2048x1536) and returns PRNU_EINVAL, with that limit in prnu_last_error(),
+ * beyond it.  The array is shared by every call on a GPU: calls that overlap
+ * on one GPU (on different streams or threads) race and must be serialised
+ * by the caller.  Calls no hipMalloc/hipFree; with stream 0 it returns when
+ * done. */
+PRNU_EXPORT int prnu_pattern_stats_noalloc(const float *const *d_patterns, int count, int64_t n, double *d_sum,
+                                           double *d_sumsq, float *d_absmax, prnu_stream stream);
 
 /* All-pairs NCC of `count` patterns of the context's size as a triangular
  * Gram on MFMA (`precision` is a prnu_precision), with the FP64 epilogue

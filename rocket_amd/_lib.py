@@ -78,6 +78,7 @@ _SIGNATURES = {
     "prnu_filter_spectral": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_size, _c_void_p]),
     "prnu_pattern_stats": (_c_int, [_c_void_p, _c_int, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "prnu_measure_mfma_peak": (_c_int, [_c_int, _c_int, _c_int, _c_void_p, _c_void_p]),
+    "prnu_pattern_stats_noalloc": (_c_int, [_c_void_p, _c_int, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "prnu_store_create": (_c_int, [_c_i64, _c_int, _c_int, ctypes.POINTER(_c_void_p)]),
     "prnu_store_destroy": (_c_int, [_c_void_p]),
     "prnu_store_info": (_c_int, [_c_void_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),

@@ -13,6 +13,7 @@
 #include "prnu_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace prnu {
 namespace {
@@ -652,11 +653,9 @@ __global__ __launch_bounds__(kThreads) void gram_reduce_kernel(const float *__re
 }  // namespace
 
 // ================================================================ launchers
-int launch_pattern_stats(const float *const *d_ptrs, int count, int64_t n, double *d_sum,
-                         double *d_sumsq, float *d_absmax, Workspace &ws, hipStream_t st) {
-    PRNU_REQUIRE(count > 0 && n > 0, "pattern_stats: count and n must be positive");
-    PRNU_REQUIRE(count <= 65535, "pattern_stats: count %d exceeds 65535", count);
-    // ~64K elements per block, multiple of 4
+// K split of the statistics: about 64K values per block (a multiple of 4),
+// at most 4096 blocks per pattern.
+static void stats_split(int64_t n, int64_t *chunk_out, int64_t *chunks_out) {
     int64_t chunk = 65536;
     int64_t chunks = (n + chunk - 1) / chunk;
     if (chunks > 4096) {
@@ -664,14 +663,61 @@ int launch_pattern_stats(const float *const *d_ptrs, int count, int64_t n, doubl
         chunk = ((n + chunks - 1) / chunks + 3) & ~int64_t(3);
         chunks = (n + chunk - 1) / chunk;
     }
-    PRNU_TRY(ws.ensure(sizeof(StatPart) * (size_t)chunks * count));
-    StatPart *part = static_cast<StatPart *>(ws.ptr);
+    *chunk_out = chunk;
+    *chunks_out = chunks;
+}
+
+// Both kernels of the statistics over a partial-sum buffer of chunks * count
+// entries (every entry is written by stats_partial_kernel before the final
+// kernel reads it, so the buffer needs no clearing).
+static int launch_stats_kernels(const float *const *d_ptrs, int count, int64_t n, int64_t chunk, int64_t chunks,
+                                StatPart *part, double *d_sum, double *d_sumsq, float *d_absmax, hipStream_t st) {
     hipLaunchKernelGGL(stats_partial_kernel, dim3((unsigned)chunks, (unsigned)count), dim3(kThreads), 0, st,
                        d_ptrs, n, chunk, part);
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(stats_final_kernel, dim3((count + 255) / 256), dim3(256), 0, st, part, (int)chunks, count,
                        d_sum, d_sumsq, d_absmax);
     PRNU_CHECK_LAUNCH();
+    return PRNU_OK;
+}
+
+int launch_pattern_stats(const float *const *d_ptrs, int count, int64_t n, double *d_sum,
+                         double *d_sumsq, float *d_absmax, Workspace &ws, hipStream_t st) {
+    PRNU_REQUIRE(count > 0 && n > 0, "pattern_stats: count and n must be positive");
+    PRNU_REQUIRE(count <= 65535, "pattern_stats: count %d exceeds 65535", count);
+    int64_t chunk, chunks;
+    stats_split(n, &chunk, &chunks);
+    PRNU_TRY(ws.ensure(sizeof(StatPart) * (size_t)chunks * count));
+    return launch_stats_kernels(d_ptrs, count, n, chunk, chunks, static_cast<StatPart *>(ws.ptr), d_sum, d_sumsq,
+                                d_absmax, st);
+}
+
+// ------------------------------------------ statistics without allocation
+// The partial sums of prnu_pattern_stats_noalloc live in a fixed device
+// array (each device holds its own copy, at its own address, until the
+// process ends).  Host code never passes `stats_scratch` itself to a kernel
+// or a copy: the array's name is a host-side stand-in there, so its device
+// address is looked up once per device with hipGetSymbolAddress.
+constexpr int64_t kStatsScratchParts = 131072;  // 3 MiB of 24-byte (sum, sumsq, absmax) partials
+static __device__ __align__(256) StatPart stats_scratch[kStatsScratchParts];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static StatPart *g_scratch_ptr[kMaxDevices] = {};
+
+static int stats_scratch_ptr(StatPart **out) {
+    int dev = 0;
+    PRNU_CHECK_HIP(hipGetDevice(&dev));
+    PRNU_REQUIRE(dev >= 0 && dev < kMaxDevices, "pattern_stats_noalloc: device %d beyond %d devices", dev,
+                 kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        PRNU_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(stats_scratch)));
+        PRNU_REQUIRE(p, "pattern_stats_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<StatPart *>(p);
+    }
+    *out = g_scratch_ptr[dev];
     return PRNU_OK;
 }
 
@@ -839,3 +885,29 @@ int autotune_gram(Context *c, const float *const *d_ptrs, int count, int precisi
 }
 
 }  // namespace prnu
+
+using namespace prnu;
+
+extern "C" {
+
+int prnu_pattern_stats_noalloc(const float *const *d_patterns, int count, int64_t n, double *d_sum, double *d_sumsq,
+                               float *d_absmax, prnu_stream stream) {
+    PRNU_REQUIRE(d_patterns && d_sum && d_sumsq, "pattern_stats_noalloc: null pointer");
+    PRNU_REQUIRE(count > 0 && n > 0, "pattern_stats_noalloc: count and n must be positive");
+    PRNU_REQUIRE(count <= 65535, "pattern_stats_noalloc: count %d exceeds 65535", count);
+    int64_t chunk, chunks;
+    stats_split(n, &chunk, &chunks);
+    PRNU_REQUIRE(chunks * count <= kStatsScratchParts,
+                 "pattern_stats_noalloc: %d patterns of %lld values need %lld partial sums, the fixed scratch holds "
+                 "%lld (at most %lld patterns of this length per call)",
+                 count, (long long)n, (long long)(chunks * count), (long long)kStatsScratchParts,
+                 (long long)(kStatsScratchParts / chunks));
+    StatPart *part = nullptr;
+    PRNU_TRY(stats_scratch_ptr(&part));
+    hipStream_t st = (hipStream_t)stream;
+    PRNU_TRY(launch_stats_kernels(d_patterns, count, n, chunk, chunks, part, d_sum, d_sumsq, d_absmax, st));
+    if (!stream) PRNU_CHECK_HIP(hipStreamSynchronize(st));
+    return PRNU_OK;
+}
+
+}  // extern "C"
