@@ -7,13 +7,22 @@
 // couple of pairs, inputs are R2C half spectra h x (w/2+1):
 //   1. pce_rows2_kernel  builds row i of Z and runs the inverse row FFT
 //                        (length w) in LDS;
-//   2. pce_cols2_kernel  loads a strip of columns (coalesced rows of the
-//                        strip, no transpose), inverse FFT of each column
-//                        (length h) in LDS; writes Re c of both pairs and per
-//                        column the max |Re c| with its index and sum (Re c)^2
-//                        in FP64;
-//   3. pce_final_kernel  per pair: argmax over columns, energy = total -
-//                        window (FP64), peak value, PCE.
+//   2. transpose         the row-pass output, so that a column is a
+//                        contiguous run of h values;
+//      pce_cols_kernel   one column per workgroup: inverse FFT (length h)
+//                        through fft_row, whose first pass loads the column
+//                        and whose last pass hands every output to the
+//                        per-column sums: the max |Re c| with its index and
+//                        sum (Re c)^2 in FP64 for each pair of the couple
+//                        (the correlation planes themselves are never
+//                        written);
+//   3. pce_window_kernel per pair and window row: argmax over columns; the
+//                        few values the PCE needs beyond the sums -- the
+//                        exclusion window around the peak and the peak value
+//                        -- are recomputed as direct column DFTs of the
+//                        row-pass output (FP64 sums, about a hundred values of
+//                        h terms each);
+//      pce_score_kernel  energy = total - window (FP64), PCE.
 // A pair's result depends on its couple partner only through FP32 rounding
 // (a fixed pair list always forms the same couples).
 // Reference variant (quirk list in the header of prnu_api.h's correlate):
@@ -128,12 +137,24 @@ __device__ __forceinline__ float2 herm_at(const float2 *X, const float2 *Y, int 
 // Row i of Z = H_a + i H_b for couple blockIdx.y (pairs 2q, 2q+1 of the
 // batch; the second may be absent), inverse row FFT: the first pass reads
 // the Hermitian parts straight from the half spectra, the last pass writes
-// the row of A.  grid (h, couples), LDS w complex.
+// the row of A.  grid (16 ceil((h/2 + 1) / 8), couples), LDS w complex.
 template <int VARIANT, int MAXN>
 __global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h, int nb, PceBatch bt,
                                                                 float2 *__restrict__ Aall, int64_t pstride) {
     extern __shared__ float2 buf[];
-    const int q = blockIdx.y, i = blockIdx.x;
+    // Row i reads spectrum rows i and (h - i) mod h (the mirrored half of the
+    // cross spectrum), and so does row (h - i) mod h: workgroups x and x + 8
+    // of each group of 16 -- the same XCD, as workgroup ids go to the 8 XCDs
+    // round-robin -- take the two rows of a mirror pair u, h - u, so the
+    // second reads them from that XCD's L2.  Rows 0 and h/2 are their own
+    // mirrors (the second workgroup of the pair leaves); ids past the last
+    // pair leave too (grid rows_grid(h) >= h).
+    const int x = blockIdx.x;
+    const int u = (x >> 4) * 8 + (x & 7), side = (x >> 3) & 1;
+    if (u > h / 2) return;
+    const int i = side ? (h - u) % h : u;
+    if (side && i == u) return;
+    const int q = blockIdx.y;
     const int w = d.n;
     const float2 *Xa, *Ya, *Xb = nullptr, *Yb = nullptr;
     pce_operands(bt, 2 * q, Xa, Ya);
@@ -151,77 +172,57 @@ __global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h
         [&](int j, float2 f) { A[j] = f; });
 }
 
-constexpr int kMaxStrip = 16;
-// Values of one column-pass strip (64 KiB of LDS, two workgroups per CU): a
-// strip of 1024-row columns is 8 wide, so its rows are read in 64-byte
-// pieces; the columns are transformed kFftMultiMax values at a time.
-constexpr int kPceStripMax = 8192;
-
-// Columns [j0, j0 + sw) of couple blockIdx.y: rows of the strip are read
-// contiguously into column-major LDS, the columns are inverse-transformed
-// together (length h), and Re / Im go to the two pairs' correlation planes
-// (column-major, R[j*h + i]) with per-column energy and |value| argmax.
-// grid (ceil(w/sw), couples), LDS sw*h complex, sw <= kMaxStrip.
-template <bool BIG>
-__global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_cols2_kernel(FftDesc d, int w, int sw, int nb,
-                                                                             const float2 *__restrict__ Aall,
-                                                                             int64_t pstride, float *__restrict__ Rall,
-                                                                             int64_t rstride,
-                                                                             PeakPart *__restrict__ partall) {
+// Column j of couple blockIdx.y from the transposed row-pass output
+// (B[j*h + i] = A[i][j]): inverse FFT of length h; every output value goes
+// straight into this thread's sums of Re (pair 2q) and Im (pair 2q + 1) --
+// FP64 energy and the max |value| with its linear index i*w + j, smallest
+// index on ties -- which a fixed-order block reduction turns into the
+// column's partials.  grid (w, couples), LDS h complex.
+template <int MAXN>
+__global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w, int nb, const float2 *__restrict__ Ball,
+                                                               int64_t pstride, PeakPart *__restrict__ partall) {
     extern __shared__ float2 buf[];
-    __shared__ double sh_e[2 * kMaxStrip][4];
-    __shared__ float sh_v[2 * kMaxStrip][4];
-    __shared__ int sh_i[2 * kMaxStrip][4];
-    const int h = d.n, q = blockIdx.y;
-    const int j0 = blockIdx.x * sw;
-    const int ncol = min(sw, w - j0);
-    const float2 *A = Aall + (int64_t)q * pstride;
-    for (int idx = threadIdx.x; idx < h * ncol; idx += kFftThreads) {
-        const int i = idx / ncol, c = idx - (idx / ncol) * ncol;
-        buf[c * h + i] = A[(int64_t)i * w + j0 + c];
-    }
-    __syncthreads();
-    if constexpr (BIG)
-        fft_lds<1>(buf, d);  // one column longer than kFftMultiMax (sw == 1)
-    else {
-        const int per = max(1, kFftMultiMax / h);  // columns per multi-transform call
-        for (int c0 = 0; c0 < ncol; c0 += per) fft_lds_multi<1>(buf + c0 * h, d, min(per, ncol - c0));
-    }
+    __shared__ double sh_e[2][4];
+    __shared__ float sh_v[2][4];
+    __shared__ int sh_i[2][4];
+    const int h = d.n, q = blockIdx.y, j = blockIdx.x;
+    const float2 *col = Ball + (int64_t)q * pstride + (int64_t)j * h;
+    double e0 = 0.0, e1 = 0.0;
+    float b0 = -1.0f, b1 = -1.0f;
+    int i0 = 0x7fffffff, i1 = 0x7fffffff;
+    fft_row<1, MAXN>(
+        buf, d, [&](int i) { return col[i]; },
+        [&](int i, float2 z) {
+            const int idx = i * w + j;
+            e0 += (double)z.x * (double)z.x;
+            e1 += (double)z.y * (double)z.y;
+            argmax_merge(b0, i0, fabsf(z.x), idx);
+            argmax_merge(b1, i1, fabsf(z.y), idx);
+        });
     const int halves = 2 * q + 1 < nb ? 2 : 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c = 0; c < ncol; ++c) {
-        const int j = j0 + c;
-        for (int hf = 0; hf < halves; ++hf) {
-            float *R = Rall + (int64_t)(2 * q + hf) * rstride;
-            double e = 0.0;
-            float best = -1.0f;
-            int bi = 0x7fffffff;
-            for (int i = threadIdx.x; i < h; i += kFftThreads) {
-                const float2 z = buf[c * h + i];
-                const float v = hf ? z.y : z.x;
-                R[(int64_t)j * h + i] = v;
-                e += (double)v * (double)v;
-                argmax_merge(best, bi, fabsf(v), i * w + j);
-            }
-            e = wave_sum_d(e);
+    e0 = wave_sum_d(e0);
+    e1 = wave_sum_d(e1);
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const float v2 = __shfl_xor(best, off, 64);
-                const int i2 = __shfl_xor(bi, off, 64);
-                argmax_merge(best, bi, v2, i2);
-            }
-            if (lane == 0) {
-                sh_e[2 * c + hf][wave] = e;
-                sh_v[2 * c + hf][wave] = best;
-                sh_i[2 * c + hf][wave] = bi;
-            }
-        }
+    for (int off = 32; off > 0; off >>= 1) {
+        float v2 = __shfl_xor(b0, off, 64);
+        int k2 = __shfl_xor(i0, off, 64);
+        argmax_merge(b0, i0, v2, k2);
+        v2 = __shfl_xor(b1, off, 64);
+        k2 = __shfl_xor(i1, off, 64);
+        argmax_merge(b1, i1, v2, k2);
+    }
+    if (lane == 0) {
+        sh_e[0][wave] = e0;
+        sh_e[1][wave] = e1;
+        sh_v[0][wave] = b0;
+        sh_v[1][wave] = b1;
+        sh_i[0][wave] = i0;
+        sh_i[1][wave] = i1;
     }
     __syncthreads();
-    // one thread per (column, pair): fixed-order merge of the 4 waves
     const int t = threadIdx.x;
-    if (t < 2 * ncol && (t & 1) < halves) {
-        const int c = t >> 1, hf = t & 1;
+    if (t < halves) {  // fixed-order merge of the 4 waves
         PeakPart p;
         p.energy = (sh_e[t][0] + sh_e[t][1]) + (sh_e[t][2] + sh_e[t][3]);
         float best = sh_v[t][0];
@@ -229,86 +230,169 @@ __global__ __launch_bounds__(kFftThreads, BIG ? 1 : 2) void pce_cols2_kernel(Fft
         for (int k = 1; k < 4; ++k) argmax_merge(best, bi, sh_v[t][k], sh_i[t][k]);
         p.absmax = best;
         p.index = bi;
-        partall[(int64_t)(2 * q + hf) * w + j0 + c] = p;
+        partall[(int64_t)(2 * q + t) * w + j] = p;
     }
 }
 
-// R is the correlation plane transposed: c[i][j] = R[j*h + i].  grid (batch).
-template <int VARIANT>
-__global__ __launch_bounds__(256) void pce_final_kernel(const PeakPart *__restrict__ partall,
-                                                        const float *__restrict__ Rall, int h, int w,
-                                                        double *__restrict__ outall, int base, int64_t rstride) {
-    __shared__ double shd[4];
-    __shared__ float shv[4];
-    __shared__ int shi[4];
-    __shared__ int s_peak;
-    const PeakPart *part = partall + (int64_t)blockIdx.x * w;
-    const float *R = Rall + (int64_t)blockIdx.x * rstride;
+// Value c[i][j] of pair hf (0: Re, 1: Im of the couple's complex plane) as
+// the direct inverse DFT of column j of the row-pass output A (h x w,
+// row-major): sum_k A[k][j] exp(+2 pi i i k / h), summed in FP64 per thread
+// over k = t, t + 256, ... and then over the block (fixed order).  `tw` is
+// the length-h table exp(-2 pi i m / h); ncells cells (i[c], j[c]) at once.
+constexpr int kMaxCells = 11;
+
+__device__ void dft_cells(const float2 *__restrict__ A, int h, int w, const float2 *__restrict__ tw, int hf,
+                          const int *ci, const int *cj, int ncells, double *out, double (*sh)[4]) {
+    double acc[kMaxCells];
+#pragma unroll
+    for (int c = 0; c < kMaxCells; ++c) acc[c] = 0.0;
+    for (int k = threadIdx.x; k < h; k += 256) {
+        const float2 *row = A + (int64_t)k * w;
+#pragma unroll
+        for (int c = 0; c < kMaxCells; ++c) {
+            if (c < ncells) {
+                const float2 a = row[cj[c]];
+                const float2 t = tw[(int)(((int64_t)ci[c] * k) % h)];  // exp(-i theta); the inverse uses exp(+i theta)
+                // (a.x + i a.y)(t.x - i t.y): Re = a.x t.x + a.y t.y, Im = a.y t.x - a.x t.y
+                acc[c] += hf ? ((double)a.y * t.x - (double)a.x * t.y) : ((double)a.x * t.x + (double)a.y * t.y);
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < kMaxCells; ++c) {
+        if (c < ncells) {
+            const double v = wave_sum_d(acc[c]);
+            if (lane == 0) sh[c][wave] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < ncells) {
+        const int c = threadIdx.x;
+        out[c] = (sh[c][0] + sh[c][1]) + (sh[c][2] + sh[c][3]);
+    }
+    __syncthreads();
+}
+
+// Per pair p of the batch, 16 doubles of window work: [0, 11) the sums of
+// squares of window rows, [11] the peak value, [12] the total energy, [13]
+// the excluded cell count of the energy divisor.
+constexpr int kWinSlots = 16;
+
+// Argmax and total energy of pair p from its column partials (every block of
+// the pair computes them the same way, in a fixed order).
+__device__ void pair_peak(const PeakPart *part, int w, int h, double *shd, float *shv, int *shi, int *s_peak,
+                          double *total) {
     double e = 0.0;
     float best = -1.0f;
     int bi = 0x7fffffff;
     for (int j = threadIdx.x; j < w; j += 256) {
-        const PeakPart p = part[j];
-        e += p.energy;
-        argmax_merge(best, bi, p.absmax, p.index);
+        const PeakPart q = part[j];
+        e += q.energy;
+        argmax_merge(best, bi, q.absmax, q.index);
     }
-    const double total = block_sum_256(e, shd);
+    *total = block_sum_256(e, shd);
     block_argmax_256(best, bi, shv, shi);
     // no argmax at all (every value NaN): index 0 keeps the window and peak
     // reads inside the plane; the energy, hence the PCE, is NaN anyway
-    if (threadIdx.x == 0) s_peak = (bi >= 0 && (int64_t)bi < (int64_t)h * w) ? bi : 0;
+    if (threadIdx.x == 0) *s_peak = (bi >= 0 && (int64_t)bi < (int64_t)h * w) ? bi : 0;
     __syncthreads();
+}
+
+// Block (p, y): pair p's window row y (y < 11), or its peak value (y == 11:
+// the last element c[h-1][w-1] in the reference variant; the textbook
+// variant's peak value comes with the window row that holds it).  grid
+// (batch, 12).
+template <int VARIANT>
+__global__ __launch_bounds__(256) void pce_window_kernel(const PeakPart *__restrict__ partall,
+                                                         const float2 *__restrict__ Aall, int64_t pstride,
+                                                         const float2 *__restrict__ tw, int h, int w,
+                                                         double *__restrict__ wbuf) {
+    __shared__ double shd[4];
+    __shared__ float shv[4];
+    __shared__ int shi[4];
+    __shared__ int s_peak;
+    __shared__ double sh_cell[kMaxCells][4];
+    __shared__ double cells[kMaxCells];
+    __shared__ int ci[kMaxCells], cj[kMaxCells];
+    const int p = blockIdx.x, y = blockIdx.y, hf = p & 1;
+    const float2 *A = Aall + (int64_t)(p >> 1) * pstride;
+    double *slots = wbuf + (int64_t)p * kWinSlots;
+    double total;
+    pair_peak(partall + (int64_t)p * w, w, h, shd, shv, shi, &s_peak, &total);
     const int peak = s_peak;
     const int py = peak / w, px = peak - py * w;
-    // window energy
-    double win = 0.0;
-    int excluded = 0;
-    if (VARIANT == PRNU_PCE_REFERENCE) {
-        const int r0 = max(0, py - kRadius + 1), r1 = min(h - 1, py + kRadius - 1);
-        const int c0 = max(0, px - kRadius + 1), c1 = min(w - 1, px + kRadius - 1);
-        const int nr = r1 - r0 + 1, nc = c1 - c0 + 1;
-        for (int k = threadIdx.x; k < nr * nc; k += 256) {
-            const int i = r0 + k / nc, jj = c0 + k % nc;
-            const double v = R[(int64_t)jj * h + i];
-            win += v * v;
-        }
-        excluded = kSquare * kSquare;  // the reference divides by N - 121
-    } else {
-        const int nr = h < kSquare ? h : kSquare, nc = w < kSquare ? w : kSquare;
-        const int r0 = h < kSquare ? 0 : py - kRadius, c0 = w < kSquare ? 0 : px - kRadius;
-        for (int k = threadIdx.x; k < nr * nc; k += 256) {
-            const int i = ((r0 + k / nc) % h + h) % h, jj = ((c0 + k % nc) % w + w) % w;
-            const double v = R[(int64_t)jj * h + i];
-            win += v * v;
-        }
-        excluded = nr * nc;
+    int r0, nr, c0, nc;
+    bool wrap;
+    if (VARIANT == PRNU_PCE_REFERENCE) {  // strictly inside (p - 5, p + 5), clipped
+        r0 = max(0, py - kRadius + 1);
+        nr = min(h - 1, py + kRadius - 1) - r0 + 1;
+        c0 = max(0, px - kRadius + 1);
+        nc = min(w - 1, px + kRadius - 1) - c0 + 1;
+        wrap = false;
+    } else {  // 11 x 11 around the peak, cyclic
+        nr = h < kSquare ? h : kSquare;
+        nc = w < kSquare ? w : kSquare;
+        r0 = h < kSquare ? 0 : py - kRadius;
+        c0 = w < kSquare ? 0 : px - kRadius;
+        wrap = true;
     }
-    win = block_sum_256(win, shd);
-    if (threadIdx.x == 0) {
-        const double energy = (total - win) / (double)((int64_t)h * w - excluded);
-        double pce;
-        if (VARIANT == PRNU_PCE_REFERENCE) {
-            const float pk = R[(int64_t)(w - 1) * h + (h - 1)];  // last element c[h-1][w-1]
-            const float sq = pk * pk;                            // squared in FP32
-            pce = (double)sq / energy;
-        } else {
-            const double pk = R[(int64_t)px * h + py];
-            pce = (pk < 0 ? -1.0 : 1.0) * pk * pk / energy;
+    if (y == 0 && threadIdx.x == 0) {
+        slots[12] = total;
+        slots[13] = VARIANT == PRNU_PCE_REFERENCE ? (double)(kSquare * kSquare) : (double)(nr * nc);  // the reference divides by N - 121
+    }
+    const int t = (int)threadIdx.x;  // signed: c0 may be negative before the wrap
+    if (y < kSquare) {
+        if (y >= nr) {
+            if (t == 0) slots[y] = 0.0;
+            return;
         }
-        outall[base + blockIdx.x] = pce;
+        if (t < nc) {
+            ci[t] = wrap ? ((r0 + y) % h + h) % h : r0 + y;
+            cj[t] = wrap ? ((c0 + t) % w + w) % w : c0 + t;
+        }
+        __syncthreads();
+        dft_cells(A, h, w, tw, hf, ci, cj, nc, cells, sh_cell);
+        if (t == 0) {
+            double win = 0.0;
+            for (int c = 0; c < nc; ++c) {
+                win += cells[c] * cells[c];
+                if (VARIANT != PRNU_PCE_REFERENCE && ci[c] == py && cj[c] == px) slots[11] = cells[c];
+            }
+            slots[y] = win;
+        }
+    } else if (VARIANT == PRNU_PCE_REFERENCE) {  // the peak value is the last element c[h-1][w-1]
+        if (t == 0) {
+            ci[0] = h - 1;
+            cj[0] = w - 1;
+        }
+        __syncthreads();
+        dft_cells(A, h, w, tw, hf, ci, cj, 1, cells, sh_cell);
+        if (t == 0) slots[11] = cells[0];
     }
 }
 
-// Columns per strip of the column pass: a power of two with the strip's
-// values (<= kPceStripMax, 64 KiB) in one workgroup's LDS, transformed
-// kFftMultiMax values at a time; a column longer than kFftMultiMax runs alone
-// through fft_lds (the BIG kernel, up to kFftMaxN).  The row pass runs one row per workgroup through fft_row (rows
-// grouped per workgroup measured slower: 184 VGPRs, 2 waves per SIMD).
-int pce_strip(int h) {
-    int sw = kMaxStrip;
-    if (h > kFftMultiMax) return 1;
-    while (sw > 1 && (size_t)sw * h > (size_t)kPceStripMax) sw >>= 1;
-    return sw;
+// PCE of each pair of the batch from its window work (window rows summed in
+// order).  grid (ceil(batch / 64)), 64 threads.
+template <int VARIANT>
+__global__ __launch_bounds__(64) void pce_score_kernel(const double *__restrict__ wbuf, int nb, int h, int w,
+                                                       double *__restrict__ outall, int base) {
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= nb) return;
+    const double *slots = wbuf + (int64_t)p * kWinSlots;
+    double win = 0.0;
+    for (int y = 0; y < kSquare; ++y) win += slots[y];
+    const double energy = (slots[12] - win) / ((double)h * (double)w - slots[13]);
+    double pce;
+    if (VARIANT == PRNU_PCE_REFERENCE) {
+        const float pk = (float)slots[11];
+        const float sq = pk * pk;  // squared in FP32
+        pce = (double)sq / energy;
+    } else {
+        const double pk = slots[11];
+        pce = (pk < 0 ? -1.0 : 1.0) * pk * pk / energy;
+    }
+    outall[base + p] = pce;
 }
 
 int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, hipStream_t st) {
@@ -316,18 +400,18 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     const int64_t n = c->pixels();
     const int couples = (nb + 1) / 2;
     const size_t plane = ((sizeof(float2) * n) + 255) & ~size_t(255);
-    const size_t rbytes = ((sizeof(float) * n) + 255) & ~size_t(255);
     const size_t pbytes = ((sizeof(PeakPart) * w) + 255) & ~size_t(255);
-    const int64_t pstride = (int64_t)(plane / sizeof(float2)), rstride = (int64_t)(rbytes / sizeof(float));
-    PRNU_TRY(c->ws_pce.ensure(plane * couples + (rbytes + pbytes) * nb));
+    const int64_t pstride = (int64_t)(plane / sizeof(float2));
+    const size_t wbytes = ((sizeof(double) * kWinSlots * nb) + 255) & ~size_t(255);
+    PRNU_TRY(c->ws_pce.ensure(2 * plane * couples + pbytes * nb + wbytes));
     char *base = static_cast<char *>(c->ws_pce.ptr);
-    float2 *A = reinterpret_cast<float2 *>(base);
-    float *R = reinterpret_cast<float *>(base + plane * couples);
-    PeakPart *part = reinterpret_cast<PeakPart *>(base + plane * couples + rbytes * nb);
+    float2 *A = reinterpret_cast<float2 *>(base);                  // row-pass output, h x w per couple
+    float2 *B = reinterpret_cast<float2 *>(base + plane * couples);  // its transpose, w x h per couple
+    PeakPart *part = reinterpret_cast<PeakPart *>(base + 2 * plane * couples);
+    double *wbuf = reinterpret_cast<double *>(base + 2 * plane * couples + pbytes * nb);
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
-    const bool big_h = h > kFftMultiMax;
-    const dim3 rgrid(h, couples);
+    const dim3 rgrid((unsigned)((h / 2 + 1 + 7) / 8 * 16), couples);  // mirror pairs of rows (pce_rows2_kernel)
     const size_t rsm = fft_row_smem(w);
 #define PRNU_PCE_ROWS(V, M) \
     hipLaunchKernelGGL((pce_rows2_kernel<V, M>), rgrid, dim3(kFftThreads), rsm, st, dw, h, nb, bt, A, pstride)
@@ -342,22 +426,32 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     }
 #undef PRNU_PCE_ROWS
     PRNU_CHECK_LAUNCH();
-    const int sw = pce_strip(h);
-    const dim3 cgrid((w + sw - 1) / sw, couples);
-    const size_t csm = sizeof(float2) * sw * h;  // <= 64 KiB (one column of up to kFftMaxN)
-    if (big_h)
-        hipLaunchKernelGGL((pce_cols2_kernel<true>), cgrid, dim3(kFftThreads), csm, st, dh, w, sw, nb, A, pstride, R,
-                           rstride, part);
+    PRNU_TRY(launch_transpose_c(A, B, h, w, couples, pstride, st));
+    const dim3 cgrid(w, couples);
+    const size_t csm = fft_row_smem(h);
+    if (h <= kFftSmallN)
+        hipLaunchKernelGGL((pce_cols_kernel<kFftSmallN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, B, pstride,
+                           part);
+    else if (h <= kFftMidN)
+        hipLaunchKernelGGL((pce_cols_kernel<kFftMidN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, B, pstride,
+                           part);
     else
-        hipLaunchKernelGGL((pce_cols2_kernel<false>), cgrid, dim3(kFftThreads), csm, st, dh, w, sw, nb, A, pstride, R,
-                           rstride, part);
+        hipLaunchKernelGGL((pce_cols_kernel<kFftMaxN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, B, pstride,
+                           part);
     PRNU_CHECK_LAUNCH();
-    if (variant == PRNU_PCE_REFERENCE)
-        hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_REFERENCE>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,
-                           bt.base, rstride);
-    else
-        hipLaunchKernelGGL(pce_final_kernel<PRNU_PCE_TEXTBOOK>, dim3(nb), dim3(256), 0, st, part, R, h, w, out,
-                           bt.base, rstride);
+    const dim3 wgrid(nb, kSquare + 1), sgrid((nb + 63) / 64);
+    if (variant == PRNU_PCE_REFERENCE) {
+        hipLaunchKernelGGL(pce_window_kernel<PRNU_PCE_REFERENCE>, wgrid, dim3(256), 0, st, part, A, pstride, dh.tw, h,
+                           w, wbuf);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(pce_score_kernel<PRNU_PCE_REFERENCE>, sgrid, dim3(64), 0, st, wbuf, nb, h, w, out,
+                           bt.base);
+    } else {
+        hipLaunchKernelGGL(pce_window_kernel<PRNU_PCE_TEXTBOOK>, wgrid, dim3(256), 0, st, part, A, pstride, dh.tw, h,
+                           w, wbuf);
+        PRNU_CHECK_LAUNCH();
+        hipLaunchKernelGGL(pce_score_kernel<PRNU_PCE_TEXTBOOK>, sgrid, dim3(64), 0, st, wbuf, nb, h, w, out, bt.base);
+    }
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -388,7 +482,7 @@ int run_pce_pairs(Context *c, int variant, const void *const *d_tab, const int *
     PRNU_TRY(ensure_fft_plans(c));
     if (batch <= 0) {
         // about 1 GiB of planes
-        const int64_t per = 12 * c->pixels();  // half a complex plane + a real plane per pair
+        const int64_t per = 8 * c->pixels();  // half of a couple's complex plane and of its transpose per pair
         batch = (int)std::max<int64_t>(2, std::min<int64_t>(512, (int64_t(1) << 30) / per)) & ~1;
     }
     time_begin(c, st);
