@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
-    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill"],
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill", "gram"],
                     help="run one secondary leg alone and print its result (for profiling)")
     ap.add_argument("--extract-shape", default=None, metavar="HxW",
                     help="extraction leg: one frame shape only (default: 1536x2048 and 512x512)")
@@ -184,6 +184,34 @@ def measured_peaks(torch):
         res[f"{name}_gemm_TFLOPs"] = 2.0 * m ** 3 / t / 1e12
         del a, b
     return res
+
+
+def gram_leg(torch, fmt="split_bf16", launches=4):
+    """The headline store Gram alone, for profiling: 1024 zero-mean N(0, 1)
+    patterns of 2048x1536 (few kernels before it, unlike the extracted set)
+    in a store of `fmt`, then `launches` Gram launches; kernel ms by the
+    context's HIP events."""
+    from rocket_amd import _lib, csi
+    lib = _lib.load()
+    n = HEIGHT * WIDTH
+    g = torch.Generator(device="cuda").manual_seed(1)
+    st = csi.PatternStore(n, COUNT, fmt)
+    for b in range(0, COUNT, 128):
+        blk = torch.randn((128, n), generator=g, device="cuda")
+        blk -= blk.mean(dim=1, keepdim=True)
+        st.put(blk, first_slot=b)
+        del blk
+    gram = csi.GramNCC(HEIGHT, WIDTH)
+    _lib.check(lib.prnu_ctx_set_timing(gram._ctx.handle, 1))
+    out = torch.empty((COUNT, COUNT), dtype=torch.float64, device="cuda")
+    ms, kms = ctypes.c_float(), []
+    for _ in range(launches):
+        st.gram(gram, COUNT, out=out)
+        _lib.check(lib.prnu_ctx_last_kernel_ms(gram._ctx.handle, ctypes.byref(ms)))
+        kms.append(ms.value)
+    st.close()
+    gram.close()
+    return {"format": fmt, "kernel_ms": kms}
 
 
 def load_traffic(kernel: str):
@@ -839,7 +867,8 @@ def main():
                "groups": lambda: groups_bench(torch),
                "serving": lambda: serving_bench(torch, args.serving_seconds),
                "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch),
-               "jpeg": lambda: jpeg_bench(torch), "spill": lambda: spill_bench(torch)}[args.only]
+               "jpeg": lambda: jpeg_bench(torch), "spill": lambda: spill_bench(torch),
+               "gram": lambda: gram_leg(torch)}[args.only]
         out = leg()
         if isinstance(out, dict):
             out.pop("_check", None)
