@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
-    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill", "gram"],
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill", "gram",
+                                       "pce_pairs"],
                     help="run one secondary leg alone and print its result (for profiling)")
     ap.add_argument("--extract-shape", default=None, metavar="HxW",
                     help="extraction leg: one frame shape only (default: 1536x2048 and 512x512)")
@@ -212,6 +213,32 @@ def gram_leg(torch, fmt="split_bf16", launches=4):
     st.close()
     gram.close()
     return {"format": fmt, "kernel_ms": kms}
+
+
+def pce_pairs_leg(torch):
+    """The PCE kernels alone, for profiling: R2C spectra of 256 N(0, 1)
+    patterns of 1024x1024 (one random kernel, then the library's spectral
+    filter), all 32,640 pairs through prnu_pce_pairs, reference then
+    textbook variant."""
+    from rocket_amd import _lib, csi
+    n = PCE_H * PCE_W
+    g = torch.Generator(device="cuda").manual_seed(3)
+    pats = torch.randn((PCE_COUNT, n), generator=g, device="cuda")
+    f = csi.PRNUFilter(PCE_H, PCE_W, apply_spectral_filter=True)
+    spec = torch.empty((PCE_COUNT, f.get_output_size() // 4), device="cuda")
+    for i in range(PCE_COUNT):
+        f.get_spectral_filter().apply_gpu(pats[i], spec[i], sync=False)
+    del pats
+    pairs = np.array([(i, j) for i in range(PCE_COUNT) for j in range(i + 1, PCE_COUNT)], dtype=np.int32)
+    res = {"pairs": int(len(pairs))}
+    for variant, name in ((_lib.PCE_REFERENCE, "reference"), (_lib.PCE_TEXTBOOK, "textbook")):
+        cmp = csi.PeakToCorrelationEnergy(PCE_H, PCE_W, variant, _ctx=f._ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        cmp.pairs(spec, pairs)
+        res[f"{name}_pairs_per_s"] = len(pairs) / (time.perf_counter() - t1)
+    f.cleanup()
+    return res
 
 
 def load_traffic(kernel: str):
@@ -868,7 +895,7 @@ def main():
                "serving": lambda: serving_bench(torch, args.serving_seconds),
                "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch),
                "jpeg": lambda: jpeg_bench(torch), "spill": lambda: spill_bench(torch),
-               "gram": lambda: gram_leg(torch)}[args.only]
+               "gram": lambda: gram_leg(torch), "pce_pairs": lambda: pce_pairs_leg(torch)}[args.only]
         out = leg()
         if isinstance(out, dict):
             out.pop("_check", None)
