@@ -287,16 +287,32 @@ __device__ __forceinline__ int s_off(int row, int chunk) { return row * kBlkByte
 
 constexpr int kBufS = kRowsS * kBlkBytes;  // 64 KiB per LDS stage
 
-// Work items.  Per K slab a group has nt(nt-1)/2 off-diagonal 256 x 256
-// tiles and ceil(6 nt / 8) diagonal packs.  A diagonal tile only needs the
-// 128 x 64 wave blocks that meet its upper triangle: 6 of 8, namely
-// (wr, wc) = (0,0) (0,1) (0,2) (0,3) (1,2) (1,3).  The 6*nt such blocks of a
-// group are numbered e = 6*d + l; pack p gives block 8p + w to wave w.  Eight
-// consecutive blocks span at most two diagonal tiles d0 = 8p/6 and d0 + 1,
-// whose rows fill the two LDS halves, so a pack costs one tile and the
-// diagonal costs 3/4 of the tiles it would otherwise take.
+// Work items.  The LDS stage holds four quarters of 128 pattern rows; a
+// wave multiplies the 128 rows of one quarter by 64 rows of another (a
+// 128 x 64 block of the output), and waves 2k, 2k+1 together cover the
+// 128 x 128 sub-tile of one quarter pair (unit k of the item).  The triangle
+// of Q = ceil(count / 128) quarters has Q (Q + 1) / 2 units (quarter pairs
+// qa <= qb); an item holds at most 4 units over at most 4 quarters, so a
+// slab needs at least ceil(Q (Q + 1) / 8) items.  With m = floor(Q / 2)
+// full 256-row tiles, per slab:
+//  - m (m - 1) / 2 off-diagonal tiles (I, J): quarters 2I, 2I+1, 2J, 2J+1,
+//    all four units of the pair;
+//  - Q even: ceil(6 m / 8) diagonal packs.  A diagonal tile needs 3 units,
+//    the 6 wave blocks (wr, wc) = (0,0) (0,1) (0,2) (0,3) (1,2) (1,3); the
+//    6m such blocks are numbered e = 6 d + l and pack p gives block 8p + w
+//    to wave w.  Eight consecutive blocks span at most two diagonal tiles
+//    d0 = 8p / 6 and d0 + 1, whose quarters fill the LDS.
+//  - Q odd (last quarter L = 2m alone in tile m): m "edge" items d with
+//    quarters 2d, 2d+1, L and units (2d,2d) (2d,2d+1) (2d,L) (2d+1,L), then
+//    ceil((m + 1) / 4) items of the remaining one-quarter units
+//    (2j+1, 2j+1), j < m, and (L, L), four per item.
+// Both reach the bound: m(m-1)/2 + ceil(6m/8) = ceil(Q(Q+1)/8) for even Q,
+// m(m-1)/2 + m + ceil((m+1)/4) = ceil(Q(Q+1)/8) for odd Q.  The output of
+// unit (qa, qb) is the sub-tile at rows 128 (qa & 1), columns 128 (qb & 1)
+// of partial tile (qa / 2, qb / 2).
 struct ItemWaves {
-    int rows_x, rows_y;   // tiles whose 256 rows go to LDS rows [0,256) / [256,512); rows_y < 0: none
+    int q[4];             // first pattern (relative to the range) of LDS rows [128 k, 128 k + 128)
+    int nq;               // quarters loaded: LDS rows [0, 128 nq)
     int a_off, b_off;     // this wave's first A row / first B row in LDS
     int out_tile;         // triangle index of the tile this wave writes, -1: idle wave
     int out_off;          // offset of the wave's 128 x 64 block inside that tile
@@ -304,20 +320,29 @@ struct ItemWaves {
 
 __device__ __forceinline__ int tri_index(int I, int J, int nt) { return I * nt - I * (I - 1) / 2 + (J - I); }
 
-// Item u of a slab (u < nt(nt-1)/2: off-diagonal tile, else a diagonal pack).
-__device__ __forceinline__ ItemWaves item_waves(int u, int nt, int wave) {
+__device__ __forceinline__ void item_tiles(ItemWaves &r, int x, int y) {
+    r.q[0] = x * kTileS;
+    r.q[1] = x * kTileS + 128;
+    r.q[2] = y * kTileS;
+    r.q[3] = y * kTileS + 128;
+}
+
+// Item u of a slab of the triangle of `count` patterns (nt = ceil(count /
+// 256) partial tiles), for wave `wave`.
+__device__ __forceinline__ ItemWaves item_waves(int u, int count, int wave) {
     ItemWaves r;
+    const int Q = (count + 127) >> 7, m = Q >> 1, nt = (count + kTileS - 1) / kTileS;
     const int wr = wave >> 2, wc = wave & 3;
-    const int toff = nt * (nt - 1) / 2;
+    const int toff = m * (m - 1) / 2;
     if (u < toff) {
         int I = 0, t = u;
-        while (t >= nt - 1 - I) {
-            t -= nt - 1 - I;
+        while (t >= m - 1 - I) {
+            t -= m - 1 - I;
             ++I;
         }
         const int J = I + 1 + t;
-        r.rows_x = I;
-        r.rows_y = J;
+        item_tiles(r, I, J);
+        r.nq = 4;
         r.a_off = wr * 128;
         r.b_off = kTileS + wc * 64;
         r.out_tile = tri_index(I, J, nt);
@@ -325,36 +350,70 @@ __device__ __forceinline__ ItemWaves item_waves(int u, int nt, int wave) {
         return r;
     }
     const int p = u - toff;
-    const int d0 = (8 * p) / 6;
-    const int dl = min(nt - 1, (8 * p + 7) / 6);
-    r.rows_x = d0;
-    r.rows_y = dl > d0 ? d0 + 1 : -1;
-    const int e = 8 * p + wave;
-    if (e >= 6 * nt) {
-        r.a_off = r.b_off = 0;
-        r.out_tile = -1;
-        r.out_off = 0;
+    r.out_tile = -1;
+    r.a_off = r.b_off = r.out_off = 0;
+    if (!(Q & 1)) {
+        const int d0 = (8 * p) / 6;
+        const int dl = min(m - 1, (8 * p + 7) / 6);
+        item_tiles(r, d0, dl > d0 ? d0 + 1 : d0);
+        r.nq = dl > d0 ? 4 : 2;
+        const int e = 8 * p + wave;
+        if (e >= 6 * m) return r;
+        const int d = e / 6, l = e - 6 * (e / 6);
+        const int bwr = l < 4 ? 0 : 1;
+        const int bwc = l < 4 ? l : l - 2;
+        const int base = (d - d0) * kTileS;
+        r.a_off = base + bwr * 128;
+        r.b_off = base + bwc * 64;
+        r.out_tile = tri_index(d, d, nt);
+        r.out_off = bwr * 128 * kTileS + bwc * 64;
         return r;
     }
-    const int d = e / 6, l = e - 6 * (e / 6);
-    const int bwr = l < 4 ? 0 : 1;
-    const int bwc = l < 4 ? l : l - 2;
-    const int base = (d - d0) * kTileS;
-    r.a_off = base + bwr * 128;
-    r.b_off = base + bwc * 64;
-    r.out_tile = tri_index(d, d, nt);
-    r.out_off = bwr * 128 * kTileS + bwc * 64;
+    const int k = wave >> 1, h = wave & 1;
+    if (p < m) {  // edge item of tile d = p: quarters 2d, 2d+1, L
+        const int d = p;
+        r.q[0] = d * kTileS;
+        r.q[1] = d * kTileS + 128;
+        r.q[2] = r.q[3] = m * kTileS;
+        r.nq = 3;
+        const int sa = k == 3 ? 1 : 0;           // A quarter: 2d for units 0..2, 2d+1 for unit 3
+        const int sb = k == 0 ? 0 : k == 1 ? 1 : 2;
+        r.a_off = sa * 128;
+        r.b_off = sb * 128 + h * 64;
+        r.out_tile = tri_index(d, k < 2 ? d : m, nt);
+        r.out_off = sa * 128 * kTileS + (k == 1 ? 128 : 0) + h * 64;
+        return r;
+    }
+    // one-quarter units 4j' .. 4j'+3 of the list (2j+1, 2j+1) j < m, then (L, L)
+    const int j0 = 4 * (p - m);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = min(j0 + i, m);
+        r.q[i] = j * kTileS + (j < m ? 128 : 0);
+    }
+    r.nq = min(4, m + 1 - j0);
+    const int j = j0 + k;
+    if (j > m) return r;
+    r.a_off = k * 128;
+    r.b_off = k * 128 + h * 64;
+    r.out_tile = tri_index(j, j, nt);
+    r.out_off = (j < m ? 128 * kTileS + 128 : 0) + h * 64;
     return r;
 }
 
-__device__ __forceinline__ int items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
+// Items per slab of the triangle of `count` patterns (item_waves).
+__host__ __device__ inline int tri_items_per_slab(int count) {
+    const int Q = (count + 127) >> 7, m = Q >> 1;
+    return m * (m - 1) / 2 + ((Q & 1) ? m + (m + 4) / 4 : (6 * m + 7) / 8);
+}
 
-// One work item over the K blocks [b_begin, b_end) of the slots
-// [first, first+count): LDS rows [0,256) hold the rows of tile w.rows_x,
-// [256,512) those of w.rows_y; each wave multiplies its 128 A rows by its 64
-// B rows and writes its FP32 128 x 64 partial block to `out` (row stride
-// 256; null for an idle wave).  Ends with a barrier, so a caller may start
-// the next item on the same LDS.
+// One work item over the K blocks [b_begin, b_end): LDS rows
+// [128 k, 128 k + 128) hold the patterns w.q[k] + [0, 128) of the slots
+// [first, first+count) for k = 0, 1 and of [first_b, first_b+count_b) for
+// k = 2, 3 (the same range for a triangle), for k < w.nq; each wave
+// multiplies its 128 A rows by its 64 B rows and writes its FP32 128 x 64
+// partial block to `out` (row stride 256; null for an idle wave).  Ends
+// with a barrier, so a caller may start the next item on the same LDS.
 // VAR: 0 = direct-to-LDS loads, 1 = loads staged through registers,
 // 2 = direct-to-LDS loads and raised wave priority over the MFMA block.
 template <int FMT, int VAR>
@@ -367,10 +426,9 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     constexpr bool kF16 = FMT == PRNU_PREC_FP16;
     constexpr bool DMA = VAR != 1;
     constexpr bool kPrio = VAR == 2;
-    const bool half = w.rows_y < 0;
+    const bool half = w.nq <= 2;  // an item of three quarters loads a fourth (it is never read)
     const int nrows = half ? kTileS : kRowsS;
     const bool active = out != nullptr;
-    const int2 tile = make_int2(w.rows_x, half ? w.rows_x : w.rows_y);
 
     const int lane = tid & 63;
     const int wrow0 = w.a_off, wcol0 = w.b_off;
@@ -381,17 +439,17 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
     uint32_t valid = 0;
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
-        const int r = r0 + 64 * it;
-        const bool in_a = r < kTileS;
-        const int loc = in_a ? tile.x * kTileS + r : tile.y * kTileS + (r - kTileS);
-        valid |= (uint32_t)(r < nrows && loc < (in_a ? count : count_b)) << it;
+        const int r = r0 + 64 * it;  // in quarter it >> 1
+        const int loc = w.q[it >> 1] + r0 + 64 * (it & 1);
+        valid |= (uint32_t)(r < nrows && loc < (it < 4 ? count : count_b)) << it;
     }
     // per-row byte offsets inside one block column (rows clamped into their
     // range: LDS rows [0,256) come from slots [first, first+count), rows
     // [256,512) from [first_b, first_b+count_b))
     auto row_off = [&](int it) -> int64_t {
-        if (it < 4) return (int64_t)(first + min(tile.x * kTileS + r0 + 64 * it, count - 1)) * kBlkBytes;
-        return (int64_t)(first_b + min(tile.y * kTileS + r0 + 64 * (it - 4), count_b - 1)) * kBlkBytes;
+        const int loc = w.q[it >> 1] + r0 + 64 * (it & 1);
+        if (it < 4) return (int64_t)(first + min(loc, count - 1)) * kBlkBytes;
+        return (int64_t)(first_b + min(loc, count_b - 1)) * kBlkBytes;
     };
 
     floatx4 acc[8][4];
@@ -622,8 +680,8 @@ __device__ __forceinline__ void gram_item(const uint8_t *__restrict__ blocks, in
 __device__ __forceinline__ ItemWaves rect_waves(int u, int ntb, int wave) {
     ItemWaves r;
     const int wr = wave >> 2, wc = wave & 3;
-    r.rows_x = u / ntb;
-    r.rows_y = u - (u / ntb) * ntb;
+    item_tiles(r, u / ntb, u - (u / ntb) * ntb);
+    r.nq = 4;
     r.a_off = wr * 128;
     r.b_off = kTileS + wc * 64;
     r.out_tile = u;
@@ -633,13 +691,14 @@ __device__ __forceinline__ ItemWaves rect_waves(int u, int ntb, int wave) {
 
 template <int FMT, int VAR>
 __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_bytes, int first, int count, int first_b,
-                                         int count_b, int nt, int ntb, int S, int64_t nkb, int64_t kcb,
+                                         int count_b, int ntb, int S, int64_t nkb, int64_t kcb,
                                          float *partials, int u, int s, uint8_t *lds, int tid = threadIdx.x) {
     // wave-uniform: keep it in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    ItemWaves w = ntb > 0 ? rect_waves(u, ntb, wave) : item_waves(u, nt, wave);
-    w.rows_x = __builtin_amdgcn_readfirstlane(w.rows_x);
-    w.rows_y = __builtin_amdgcn_readfirstlane(w.rows_y);
+    ItemWaves w = ntb > 0 ? rect_waves(u, ntb, wave) : item_waves(u, count, wave);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w.q[k] = __builtin_amdgcn_readfirstlane(w.q[k]);
+    w.nq = __builtin_amdgcn_readfirstlane(w.nq);
     w.a_off = __builtin_amdgcn_readfirstlane(w.a_off >> 6) << 6;  // multiples of 64 rows: the swizzle
     w.b_off = __builtin_amdgcn_readfirstlane(w.b_off >> 6) << 6;  // term (row & 7) stays per lane
     w.out_tile = __builtin_amdgcn_readfirstlane(w.out_tile);
@@ -661,8 +720,15 @@ __device__ __forceinline__ void run_item(const uint8_t *blocks, int64_t cap_byte
 // tiles per pack).
 constexpr int kThr4 = 256;
 
-__device__ __forceinline__ ItemWaves item_waves4(int u, int nt, int wave) {
-    ItemWaves r;
+struct ItemWaves4 {
+    int rows_x, rows_y;   // tiles whose 256 rows go to LDS rows [0,256) / [256,512); rows_y < 0: none
+    int a_off, b_off;     // this wave's first A row / first B row in LDS
+    int out_tile;         // triangle index of the tile this wave writes, -1: idle wave
+    int out_off;          // offset of the wave's 128 x 128 block inside that tile
+};
+
+__device__ __forceinline__ ItemWaves4 item_waves4(int u, int nt, int wave) {
+    ItemWaves4 r;
     const int wr = wave >> 1, wc = wave & 1;
     const int toff = nt * (nt - 1) / 2;
     if (u < toff) {
@@ -716,7 +782,7 @@ __global__ __launch_bounds__(kThr4, 1) void store_gram4_kernel(StoreGramArgs a) 
         sl = bid / a.T;
     }
     if (sl >= a.S) return;
-    ItemWaves w = item_waves4(u, a.nt, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+    ItemWaves4 w = item_waves4(u, a.nt, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
     const int rows_x = __builtin_amdgcn_readfirstlane(w.rows_x);
     const int rows_y = __builtin_amdgcn_readfirstlane(w.rows_y);
     const int a_off = __builtin_amdgcn_readfirstlane(w.a_off >> 6) << 6;
@@ -896,7 +962,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
         s = bid / a.T;
     }
     if (s >= a.S) return;
-    run_item<FMT, VAR>(a.blocks, a.cap_bytes, a.first, a.count, a.first, a.count, a.nt, 0, a.S, a.nkb, a.kcb,
+    run_item<FMT, VAR>(a.blocks, a.cap_bytes, a.first, a.count, a.first, a.count, 0, a.S, a.nkb, a.kcb,
                        a.partials, u, s, lds);
 }
 
@@ -909,8 +975,6 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_kernel(StoreGramArgs a) {
 // K blocks in every group, so all items cost the same; the dispatcher hands
 // workgroups to CUs in id order as they free up, which is the work queue.
 // v & 7 is the XCD, so the T items of a slab share that XCD's L2.
-// (A persistent form that loops over items per workgroup needs ~836 B/lane
-// of scratch at this kernel's 256-VGPR budget, so each workgroup runs one.)
 struct GroupDev {
     const uint8_t *blocks;
     float *partials;
@@ -936,7 +1000,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
     const int u = local % g.T;
     const int s = (local / g.T) * 8 + (int)(vl & 7);
     if (s >= g.S) return;  // padding item
-    run_item<FMT, VAR>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.nt, g.ntb, g.S, g.nkb,
+    run_item<FMT, VAR>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.ntb, g.S, g.nkb,
                        g.kcb, g.partials, u, s, lds);
 }
 
@@ -977,7 +1041,7 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_persistent_kernel(
         // hoisted out of this loop (they would stay live across every item)
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        run_item<FMT, 0>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.nt, g.ntb, g.S, g.nkb,
+        run_item<FMT, 0>(g.blocks, g.cap_bytes, g.first, g.count, g.first_b, g.count_b, g.ntb, g.S, g.nkb,
                          g.kcb, g.partials, u, s, lds, tid);
     }
 }
@@ -1069,8 +1133,9 @@ int block_values(int format) {
     return format == PRNU_PREC_FP8 ? kBlk8 : format == PRNU_PREC_FP16 ? kBlk16 : kBlk;
 }
 
-// Work items per K slab: off-diagonal tiles plus diagonal packs (item_waves).
-int host_items_per_slab(int nt) { return nt * (nt - 1) / 2 + (6 * nt + 7) / 8; }
+// Work items per K slab of the 4-wave kernel: off-diagonal tiles plus
+// diagonal packs of 4 blocks (item_waves4).
+int items_per_slab4(int nt) { return nt * (nt - 1) / 2 + (3 * nt + 3) / 4; }
 
 int device_cus(int device) {
     int cus = 0;
@@ -1218,9 +1283,11 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     a.first = first;
     a.count = count;
     a.nt = (count + kTileS - 1) / kTileS;
-    a.T = host_items_per_slab(a.nt);
-    const int tiles = a.nt * (a.nt + 1) / 2;  // partial tiles per slab (triangle incl. diagonal)
     const prnu_gram_config cfg = c->gram_cfg[s->format];
+    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 + priority, 3 four waves (not FP32)
+    const bool four = var == 3 && s->format != PRNU_PREC_FP32;
+    a.T = four ? items_per_slab4(a.nt) : tri_items_per_slab(count);
+    const int tiles = a.nt * (a.nt + 1) / 2;  // partial tiles per slab (triangle incl. diagonal)
     const int64_t tile_bytes = (int64_t)kTileS * kTileS * sizeof(float);
     if (cfg.slabs > 0) {
         int64_t S = std::min<int64_t>(cfg.slabs, s->nkb);
@@ -1237,9 +1304,7 @@ int store_gram_ncc(Context *c, Store *s, int first, int count, double *d_ncc, hi
     const int64_t S8 = a.xcd_remap ? (a.S + 7) / 8 * 8 : a.S;
     const dim3 grid((unsigned)(a.T * S8));
     time_begin(c, st);
-    const int var = cfg.variant;  // 0 direct-to-LDS, 1 register-staged, 2 + priority, 3 four waves (not FP32)
-    if (var == 3 && s->format != PRNU_PREC_FP32) {
-        // same items per slab as the 8-wave packing (ceil(3nt/4) = ceil(6nt/8))
+    if (four) {
         if (s->format == PRNU_PREC_FP8)
             hipLaunchKernelGGL(store_gram4_kernel<PRNU_PREC_FP8>, grid, dim3(kThr4), 0, st, a);
         else if (s->format == PRNU_PREC_FP16)
@@ -1292,7 +1357,7 @@ struct PlanBlock {
     bool tri;
     int nta() const { return (row_count + kTileS - 1) / kTileS; }
     int ntb() const { return (col_count + kTileS - 1) / kTileS; }
-    int items() const { return tri ? host_items_per_slab(nta()) : nta() * ntb(); }
+    int items() const { return tri ? tri_items_per_slab(row_count) : nta() * ntb(); }
     int tiles() const { return tri ? nta() * (nta() + 1) / 2 : nta() * ntb(); }
 };
 

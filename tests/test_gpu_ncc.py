@@ -189,8 +189,11 @@ def test_gram_config_and_autotune():
 
 
 @pytest.mark.parametrize("fmt", ["split_bf16", "fp32"])
-@pytest.mark.parametrize("count,h,w,first,cap", [(64, 512, 512, 0, 64), (300, 200, 321, 5, 310), (7, 16, 9, 3, 12)])
+@pytest.mark.parametrize("count,h,w,first,cap", [(64, 512, 512, 0, 64), (300, 200, 321, 5, 310), (7, 16, 9, 3, 12),
+                                                 (840, 40, 50, 2, 845), (1100, 32, 33, 0, 1100)])
 def test_store_gram_matches_fp64_oracle(fmt, count, h, w, first, cap):
+    """Ragged counts with an odd number of 128-pattern quarters (300, 840,
+    1100: the edge and one-quarter items of the packing) and an even one."""
     p = patterns(count, h, w, 4, seed=50 + count, gain=0.2)
     store = csi.PatternStore(h * w, cap, fmt)
     store.put(p, first_slot=first)
