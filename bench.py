@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--no-extract", action="store_true", help="skip the extraction / PCE legs")
     ap.add_argument("--no-groups", action="store_true", help="skip the uneven-groups leg")
-    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill", "gram",
+    ap.add_argument("--only", choices=["pce", "extract", "groups", "serving", "c0", "peaks", "jpeg", "spill", "gram", "groups_launch",
                                        "pce_pairs"],
                     help="run one secondary leg alone and print its result (for profiling)")
     ap.add_argument("--extract-shape", default=None, metavar="HxW",
@@ -213,6 +213,43 @@ def gram_leg(torch, fmt="split_bf16", launches=4):
     st.close()
     gram.close()
     return {"format": fmt, "kernel_ms": kms}
+
+
+def groups_launch_leg(torch, reps=3):
+    """The split-BF16 grouped plan of the uneven-groups workload alone, for
+    profiling: three stores of zero-mean N(0, 1) patterns (2000 @ 1024x1024,
+    1500 @ 2048x1536, 596 @ 4000x3000), the plan at the default slab, then
+    `reps` launches per item and `reps` persistent launches."""
+    from rocket_amd import csi
+    g = torch.Generator(device="cuda").manual_seed(5)
+    stores = []
+    for count, h, w in GROUPS_C3:
+        n = h * w
+        st = csi.PatternStore(n, count, "split_bf16")
+        chunk = max(24, int((1 << 30) // (4 * n)))
+        for b in range(0, count, chunk):
+            blk = torch.randn((min(count, b + chunk) - b, n), generator=g, device="cuda")
+            blk -= blk.mean(dim=1, keepdim=True)
+            st.put(blk, first_slot=b)
+            del blk
+        stores.append(st)
+    gram = csi.GramNCC(8, 8)
+    plan = csi.GramPlan(gram, [(st, 0, c) for st, (c, _, _) in zip(stores, GROUPS_C3)])
+    res = {"work_items": plan.info()["work_items"], "slab_values": plan.info()["slab_values"]}
+    for form in (False, True):
+        plan.persistent = form
+        plan.run(use_graph=False)
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            plan.run(use_graph=False)
+            t.append(1000.0 * (time.perf_counter() - t0))
+        res["persistent_ms" if form else "per_item_ms"] = t
+    plan.close()
+    for st in stores:
+        st.close()
+    gram.close()
+    return res
 
 
 def pce_pairs_leg(torch):
@@ -895,7 +932,8 @@ def main():
                "serving": lambda: serving_bench(torch, args.serving_seconds),
                "c0": lambda: c0_bench(torch), "peaks": lambda: measured_peaks(torch),
                "jpeg": lambda: jpeg_bench(torch), "spill": lambda: spill_bench(torch),
-               "gram": lambda: gram_leg(torch), "pce_pairs": lambda: pce_pairs_leg(torch)}[args.only]
+               "gram": lambda: gram_leg(torch), "pce_pairs": lambda: pce_pairs_leg(torch),
+               "groups_launch": lambda: groups_launch_leg(torch)}[args.only]
         out = leg()
         if isinstance(out, dict):
             out.pop("_check", None)

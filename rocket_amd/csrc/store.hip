@@ -1012,18 +1012,30 @@ __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_kernel(const Group
 // takes the next item instead of waiting for the launch's tail.  Every
 // workgroup leaves once its queue passes v_total; the counters must be zero
 // at launch (store_queue_reset_kernel).
+// With items of equal cost this form gains nothing over one item per
+// workgroup and measures 1-4 % slower: the hardware dispatcher starts the
+// items of a slab together (in order, gated by the busiest XCD), so they
+// read each K block in step and the L2 serves it once, while persistent
+// workgroups drift apart item by item (more HBM reads, a lower clock under
+// the power cap).  The plan autotuner times both.
+// The item number taken by thread 0 reaches the other waves through the
+// first word of the stage buffers (free between items): with a second LDS
+// variable the compiler can no longer tell the stage being read from the one
+// the global-to-LDS loads are filling, and waits for those loads (vmcnt(0))
+// before every stage's fragment reads, which serialises loads and MFMAs
+// (items ran 21 % longer).
 template <int FMT>
 __global__ __launch_bounds__(kThrS, 1) void store_gram_groups_persistent_kernel(const GroupDev *__restrict__ groups,
                                                                                int ngroups, int64_t v_total,
                                                                                int *__restrict__ queue) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2 * kBufS];
-    __shared__ int taken;
+    int *taken = reinterpret_cast<int *>(lds);
     const int x = blockIdx.x & 7;
     for (;;) {
-        if (threadIdx.x == 0) taken = atomicAdd(queue + 16 * x, 1);
+        if (threadIdx.x == 0) *taken = atomicAdd(queue + 16 * x, 1);
         __syncthreads();
-        const int64_t v = (int64_t)__builtin_amdgcn_readfirstlane(taken) * 8 + x;
-        __syncthreads();  // every thread has read `taken` before thread 0 writes it again
+        const int64_t v = (int64_t)__builtin_amdgcn_readfirstlane(*taken) * 8 + x;
+        __syncthreads();  // every thread has read the word before the item's loads overwrite it
         if (v >= v_total) break;
         int lo = 0, hi = ngroups - 1;
         while (lo < hi) {
