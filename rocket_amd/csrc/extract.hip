@@ -203,6 +203,201 @@ __global__ __launch_bounds__(256) void fastnoise_kernel(const uint8_t *in_all, i
     if (c == 0 && gy < h) rowpart_all[(int64_t)bz * rp_stride + ((int64_t)txi * h + gy) * 2 + p] = rs;
 }
 
+// Row-walking FastNoise for widths that are multiples of 4 (and 4-byte
+// aligned inputs): lane l of a wave holds the 4 pixels of columns
+// x0 + 4 (l - 1) + [0, 4) of one row, x0 = 248 * segment; lanes 0 and 63 are
+// the halo (their pixels feed lanes 1 and 62 and are not written).  A wave
+// walks down a band of 64 rows keeping three gray rows, one U row and three
+// V rows in registers: at step t it loads gray row t (one 12-byte load per
+// lane for RGB8), forms U and V of row t - 1 and the output of row t - 2.
+// Horizontal neighbours come from the adjacent lanes through DPP moves, so
+// nothing goes through LDS.  The same arithmetic per pixel as
+// fastnoise_kernel (fastnoisefilter.cu:150-204).  Zero-mean partial sums:
+// per column and row parity over the band (FP64, bands of 64 rows), per row
+// and column parity over each 64-column group of 16 lanes (FP32 sums of the
+// group's 32 pixels of a parity, stored as FP64).
+// grid (8 * per) x 256 threads: a workgroup = 4 waves = 4 bands of one
+// segment; units (image, segment, band group) are dealt in order to the
+// XCDs as in fastnoise_kernel.
+constexpr int kFnSeg = 248, kFnBand = 64;
+
+__device__ __forceinline__ float dpp_from_left(float v) {   // lane i <- lane i - 1 (0 for lane 0)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float dpp_from_right(float v) {  // lane i <- lane i + 1 (0 for lane 63)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+}
+// sum over the 16 lanes of each DPP row, in lane 15 of the row
+__device__ __forceinline__ float dpp_row_sum16(float v) {
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, true));
+    return v;
+}
+
+// Gray values of 4 consecutive pixels starting at pixel `px` (a multiple of
+// 4) of an image at `in`.  8-bit images may start at any byte: the pixels'
+// bytes are read as the aligned dwords that hold them (each holds at least
+// one of the wanted bytes, so no read leaves the pages of the image) and
+// realigned with v_alignbyte; `off` = (address of `in`) & 3, the same for
+// every group since 3 * px and px are multiples of 4.
+template <int FMT>
+__device__ __forceinline__ void gray4(const uint8_t *in, int64_t px, int off, float g[4]) {
+    if constexpr (FMT == PRNU_INPUT_RGB8) {
+        const uint32_t *q = reinterpret_cast<const uint32_t *>(in + 3 * px - off);
+        uint32_t a = q[0], b = q[1], c = q[2];
+        if (off) {
+            const uint32_t d = q[3];
+            a = __builtin_amdgcn_alignbyte(b, a, off);
+            b = __builtin_amdgcn_alignbyte(c, b, off);
+            c = __builtin_amdgcn_alignbyte(d, c, off);
+        }
+        const uint32_t by[12] = {a & 0xff, (a >> 8) & 0xff, (a >> 16) & 0xff, a >> 24,
+                                 b & 0xff, (b >> 8) & 0xff, (b >> 16) & 0xff, b >> 24,
+                                 c & 0xff, (c >> 8) & 0xff, (c >> 16) & 0xff, c >> 24};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            g[k] = kWR * (float)by[3 * k + 2] + kWG * (float)by[3 * k + 1] + kWB * (float)by[3 * k];
+    } else if constexpr (FMT == PRNU_INPUT_GRAY8) {
+        const uint32_t *q = reinterpret_cast<const uint32_t *>(in + px - off);
+        uint32_t a = q[0];
+        if (off) a = __builtin_amdgcn_alignbyte(q[1], a, off);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float v = (float)((a >> (8 * k)) & 0xff);
+            g[k] = kWR * v + kWG * v + kWB * v;
+        }
+    } else {
+        // FP32 images are 4-byte aligned; a 16-byte load needs no more
+        typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+        const f4a4 v = *reinterpret_cast<const f4a4 *>(reinterpret_cast<const float *>(in) + px);
+        g[0] = v.x;
+        g[1] = v.y;
+        g[2] = v.z;
+        g[3] = v.w;
+    }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void fastnoise_rows_kernel(const uint8_t *in_all, int64_t in_stride, float *out_all,
+                                                             int64_t out_stride, int h, int w, double *colpart_all,
+                                                             int64_t cp_stride, double *rowpart_all, int64_t rp_stride,
+                                                             int nseg, int ngrp, int nunits) {
+    const int per = (nunits + 7) / 8;
+    const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (t >= nunits) return;
+    const int bz = t / (nseg * ngrp), rem = t - bz * (nseg * ngrp);
+    const int seg = rem / ngrp, grp = rem - (rem / ngrp) * ngrp;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int band = grp * 4 + wave;
+    const int y0 = band * kFnBand;
+    if (y0 >= h) return;  // no barriers in this kernel: a wave may leave alone
+    const int y1 = min(h, y0 + kFnBand);
+    const uint8_t *in = in_all + (int64_t)bz * in_stride;
+    const int off = FMT == PRNU_INPUT_FLOAT32 ? 0 : (int)(reinterpret_cast<uintptr_t>(in) & 3);
+    float *out = out_all + (int64_t)bz * out_stride;
+    const int xl = seg * kFnSeg + 4 * (lane - 1);  // first column of this lane
+    const bool inx = xl >= 0 && xl < w;             // all 4 pixels (w is a multiple of 4)
+    const bool own = inx && lane >= 1 && lane <= 62;
+    const bool first_col = xl == 0, last_col = xl + 4 == w;
+    const bool zm = colpart_all != nullptr;
+
+    // gray rows t-2, t-1, t; U row t-2 (output row) and t-1; V rows t-3 .. t-1
+    float G0[4] = {}, G1[4] = {}, G2[4] = {};
+    float Ua[4] = {}, Ub[4] = {};
+    float V0[4] = {}, V1[4] = {}, V2[4] = {};
+    double ce[4] = {}, co[4] = {};  // column sums of even / odd rows
+
+    auto dx4 = [&](const float g[4], float d[4]) {
+        const float left = dpp_from_left(g[3]), right = dpp_from_right(g[0]);
+        d[0] = first_col ? g[1] - g[0] : 0.5f * (g[1] - left);
+        d[1] = 0.5f * (g[2] - g[0]);
+        d[2] = 0.5f * (g[3] - g[1]);
+        d[3] = last_col ? g[3] - g[2] : 0.5f * (right - g[2]);
+    };
+    for (int r = y0 - 2; r <= y1 + 1; ++r) {
+        // shift the windows by one row
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            G0[k] = G1[k];
+            G1[k] = G2[k];
+            V0[k] = V1[k];
+            V1[k] = V2[k];
+            Ua[k] = Ub[k];
+        }
+        // gray row r
+        if (inx && r >= 0 && r < h) {
+            gray4<FMT>(in, (int64_t)r * w + xl, off, G2);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) G2[k] = 0.f;
+        }
+        // U, V of row r - 1 from gray rows r - 2, r - 1, r
+        // (every lane runs the DPP moves; lanes outside the image keep zeros)
+        const int ru = r - 1;
+        if (ru >= y0 - 1) {
+            const bool valid = inx && ru >= 0 && ru < h;
+            float dx[4];
+            dx4(G1, dx);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float dy = ru == 0 ? G2[k] - G1[k] : ru == h - 1 ? G1[k] - G0[k] : 0.5f * (G2[k] - G0[k]);
+                const float norm = __builtin_amdgcn_sqrtf(dx[k] * dx[k] + dy * dy);
+                const float scale = __builtin_amdgcn_rcpf(1.0f + norm);
+                Ub[k] = valid ? scale * dx[k] : 0.f;
+                V2[k] = valid ? scale * dy : 0.f;
+            }
+        }
+        // output row r - 2 from U row r - 2 and V rows r - 3 .. r - 1
+        const int ro = r - 2;
+        if (ro >= y0) {
+            float a[4];
+            dx4(Ua, a);  // the same one-sided / central rule on U
+            float o[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float b = ro == 0 ? V2[k] - V1[k] : ro == h - 1 ? V1[k] - V0[k] : 0.5f * (V2[k] - V0[k]);
+                o[k] = own ? a[k] + b : 0.f;
+            }
+            if (own) *reinterpret_cast<float4 *>(out + (int64_t)ro * w + xl) = make_float4(o[0], o[1], o[2], o[3]);
+            if (zm) {
+                if (ro & 1) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) co[k] += (double)o[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) ce[k] += (double)o[k];
+                }
+                // columns xl + k have parity k & 1 (xl is a multiple of 4)
+                const float se = dpp_row_sum16(o[0] + o[2]);
+                const float so = dpp_row_sum16(o[1] + o[3]);
+                if ((lane & 15) == 15) {
+                    double *rp = rowpart_all + (int64_t)bz * rp_stride +
+                                 ((int64_t)(seg * 4 + (lane >> 4)) * h + ro) * 2;
+                    rp[0] = (double)se;
+                    rp[1] = (double)so;
+                }
+            }
+        }
+    }
+    if (zm && own) {
+        double *cp = colpart_all + (int64_t)bz * cp_stride + ((int64_t)band * w + xl) * 2;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cp[2 * k + 0] = ce[k];
+            cp[2 * k + 1] = co[k];
+        }
+    }
+}
+
+// Zero-mean partial-sum tiles of the FastNoise kernels: colpart holds one
+// row of column sums per tile row, rowpart one column of row sums per tile
+// column.
+constexpr int kZmTileRows = kFnTY, kZmTileCols = kFnTX;             // fastnoise_kernel, zm_partials_kernel
+int zm_tiles_y_rows(int h) { return (h + kFnBand - 1) / kFnBand; }  // fastnoise_rows_kernel
+int zm_tiles_x_rows(int w) { return 4 * ((w + kFnSeg - 1) / kFnSeg); }
+
 // -------------------------------------------------------------- zero-mean
 // Partial sums of an arbitrary image in the fastnoise tiling.  grid (tx, ty, B)
 __global__ __launch_bounds__(256) void zm_partials_kernel(const float *img_all, int64_t img_stride, int h, int w,
@@ -652,9 +847,13 @@ struct ExtractWs {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Zero-mean partial-sum tiles the workspace holds: enough for either FastNoise kernel.
+int zm_tiles_x_max(int w) { return std::max((w + kZmTileCols - 1) / kZmTileCols, zm_tiles_x_rows(w)); }
+int zm_tiles_y_max(int h) { return std::max((h + kZmTileRows - 1) / kZmTileRows, zm_tiles_y_rows(h)); }
+
 size_t per_image_bytes(int h, int w) {
     const int64_t n = (int64_t)h * w;
-    const int tx = (w + kFnTX - 1) / kFnTX, ty = (h + kFnTY - 1) / kFnTY;
+    const int tx = zm_tiles_x_max(w), ty = zm_tiles_y_max(h);
     return (strips_for(h) ? 16 : 24) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
            11 * 256;
 }
@@ -664,10 +863,11 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
     const int64_t n = c->pixels();
     e.nb = nb;
     e.n = n;
-    e.tiles_x = (w + kFnTX - 1) / kFnTX;
-    e.tiles_y = (h + kFnTY - 1) / kFnTY;
-    e.cp_stride = 2 * (int64_t)w * e.tiles_y;
-    e.rp_stride = 2 * (int64_t)h * e.tiles_x;
+    // the tiles of the zm_partials_kernel layout; launch_fastnoise reports its own
+    e.tiles_x = (w + kZmTileCols - 1) / kZmTileCols;
+    e.tiles_y = (h + kZmTileRows - 1) / kZmTileRows;
+    e.cp_stride = 2 * (int64_t)w * zm_tiles_y_max(h);
+    e.rp_stride = 2 * (int64_t)h * zm_tiles_x_max(w);
     const size_t sz[] = {align256(sizeof(float2) * n * nb),
                          strips_for(h) ? 0 : align256(sizeof(float2) * n * nb),
                          align256(sizeof(float) * n * nb),    align256(sizeof(float) * n * nb),
@@ -703,9 +903,33 @@ size_t fft_smem(const FftDesc &d) { return sizeof(float2) * d.n; }
             hipLaunchKernelGGL((K<kFftMaxN>), grid, dim3(kFftThreads), fft_row_smem(n), st, __VA_ARGS__);       \
     } while (0)
 
+// The row-walking kernel needs widths that are multiples of 4 (>= 8), two
+// rows, and 16-byte aligned output rows (the chain's workspace planes).  The
+// choice depends on the shape alone for the chain, so a batch and its
+// images one by one take the same kernel.
+bool fastnoise_rows_ok(const float *out, int64_t out_stride, int h, int w, int nb) {
+    if (w % 4 != 0 || w < 8 || h < 2) return false;
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return false;
+    return nb == 1 || out_stride % 4 == 0;
+}
+
 template <int FMT>
 int launch_fastnoise_fmt(const void *in, int64_t in_stride, float *out, int64_t out_stride, int h, int w, int nb,
-                         double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, hipStream_t st) {
+                         double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, int *tiles_x,
+                         int *tiles_y, hipStream_t st) {
+    if (fastnoise_rows_ok(out, out_stride, h, w, nb)) {
+        const int nseg = (w + kFnSeg - 1) / kFnSeg, ngrp = (zm_tiles_y_rows(h) + 3) / 4;
+        const int64_t nunits = (int64_t)nseg * ngrp * nb;
+        PRNU_REQUIRE(nunits < (int64_t(1) << 30), "fastnoise: %lld units", (long long)nunits);
+        const int per = (int)((nunits + 7) / 8);
+        hipLaunchKernelGGL(fastnoise_rows_kernel<FMT>, dim3((unsigned)(8 * per)), dim3(256), 0, st,
+                           static_cast<const uint8_t *>(in), in_stride, out, out_stride, h, w, colpart, cp_stride,
+                           rowpart, rp_stride, nseg, ngrp, (int)nunits);
+        PRNU_CHECK_LAUNCH();
+        *tiles_x = zm_tiles_x_rows(w);
+        *tiles_y = zm_tiles_y_rows(h);
+        return PRNU_OK;
+    }
     const int ntx = (w + kFnTX - 1) / kFnTX, nty = (h + kFnTY - 1) / kFnTY;
     const int64_t ntiles = (int64_t)ntx * nty * nb;
     PRNU_REQUIRE(ntiles < (int64_t(1) << 30), "fastnoise: %lld tiles", (long long)ntiles);
@@ -714,21 +938,26 @@ int launch_fastnoise_fmt(const void *in, int64_t in_stride, float *out, int64_t 
                        static_cast<const uint8_t *>(in), in_stride, out, out_stride, h, w, colpart, cp_stride, rowpart,
                        rp_stride, ntx, nty, (int)ntiles);
     PRNU_CHECK_LAUNCH();
+    *tiles_x = ntx;
+    *tiles_y = nty;
     return PRNU_OK;
 }
 
+// FastNoise of nb images; *tiles_x / *tiles_y: the zero-mean partial-sum
+// tiles the kernel wrote (for zm_reduce_kernel).
 int launch_fastnoise(int fmt, const void *in, int64_t in_stride, float *out, int64_t out_stride, int h, int w, int nb,
-                     double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, hipStream_t st) {
+                     double *colpart, int64_t cp_stride, double *rowpart, int64_t rp_stride, int *tiles_x,
+                     int *tiles_y, hipStream_t st) {
     switch (fmt) {
         case PRNU_INPUT_RGB8:
             return launch_fastnoise_fmt<PRNU_INPUT_RGB8>(in, in_stride, out, out_stride, h, w, nb, colpart, cp_stride,
-                                                         rowpart, rp_stride, st);
+                                                         rowpart, rp_stride, tiles_x, tiles_y, st);
         case PRNU_INPUT_GRAY8:
             return launch_fastnoise_fmt<PRNU_INPUT_GRAY8>(in, in_stride, out, out_stride, h, w, nb, colpart,
-                                                          cp_stride, rowpart, rp_stride, st);
+                                                          cp_stride, rowpart, rp_stride, tiles_x, tiles_y, st);
         default:
             return launch_fastnoise_fmt<PRNU_INPUT_FLOAT32>(in, in_stride, out, out_stride, h, w, nb, colpart,
-                                                            cp_stride, rowpart, rp_stride, st);
+                                                            cp_stride, rowpart, rp_stride, tiles_x, tiles_y, st);
     }
 }
 
@@ -872,8 +1101,8 @@ int run_preprocess_batch(Context *c, int fmt, const void *in, size_t in_bytes, i
         uint8_t *dst = static_cast<uint8_t *>(out) + (size_t)b0 * out_stride;
         // gray + FastNoise into S (FP32 images n apart), zero-mean sums on the side
         float *noise = e.S;
-        PRNU_TRY(launch_fastnoise(fmt, src, in_stride, noise, n, c->h, c->w, nb,
-                                  e.colpart, e.cp_stride, e.rowpart, e.rp_stride, st));
+        PRNU_TRY(launch_fastnoise(fmt, src, in_stride, noise, n, c->h, c->w, nb, e.colpart, e.cp_stride, e.rowpart,
+                                  e.rp_stride, &e.tiles_x, &e.tiles_y, st));
         PRNU_TRY(launch_zm_means(c, e, nb, st));
         if (c->mode == PRNU_MODE_NCC) {
             PRNU_TRY(launch_wiener(c, e, nb, noise, n, reinterpret_cast<float *>(dst), out_stride / 4, true, st));
@@ -916,8 +1145,9 @@ int run_filter_fastnoise(Context *c, float *img, hipStream_t st) {
     // the kernel reads a halo around each tile, so it cannot write in place
     ExtractWs e;
     PRNU_TRY(get_ws(c, 1, e));
+    int tx = 0, ty = 0;
     PRNU_TRY(launch_fastnoise(PRNU_INPUT_FLOAT32, img, 4 * c->pixels(), e.P, c->pixels(), c->h, c->w, 1, nullptr, 0,
-                              nullptr, 0, st));
+                              nullptr, 0, &tx, &ty, st));
     PRNU_CHECK_HIP(hipMemcpyAsync(img, e.P, 4 * (size_t)c->pixels(), hipMemcpyDeviceToDevice, st));
     return PRNU_OK;
 }
