@@ -570,20 +570,22 @@ __global__ __launch_bounds__(256) void wiener_var_kernel(const double *rowsq_all
 
 // Variance estimate: min over f in {3,5,7,9} of the zero-padded f x f mean of
 // P (wienerfilter.cu:164-247), then S = var / max(var, varest).
-// A workgroup takes a 64 x 32 output tile with its 4-pixel halo in LDS; a
-// thread owns one output column and 8 output rows: it forms the horizontal
-// 3/5/7/9 sums of the 16 LDS rows those need (9 LDS reads a row) and sums
-// them vertically in registers (18 LDS reads an output instead of 80 for
-// ring-by-ring 2-D sums).  Tiles are numbered image by image, row by row;
-// workgroup ids go to the 8 XCDs round-robin, so XCD x is given the tile
-// range [x * per, (x + 1) * per) in order and neighbouring tiles share halo
-// lines in its L2.  grid (8 * per)
+// A workgroup takes a 64-column x 128-row output tile with its 4-pixel halo
+// in LDS (136 x 72 floats); wave k owns output rows [32 k, 32 k + 32) and a
+// lane one column.  The lane walks its 40 LDS rows once (fully unrolled):
+// per row it forms the horizontal 3/5/7/9 sums around its column (9 LDS
+// reads), and every output row sums the horizontal sums of the rows its box
+// covers, so a row's horizontal sums are formed once for 32 outputs instead
+// of once per 8.  All sums add nonnegative terms directly (no running
+// differences, which would carry the rounding error of a large bin past its
+// window).  Tiles are numbered image by image, row by row, and dealt to the
+// XCDs in ranges as in fastnoise_kernel.  grid (8 * per)
 // P holds half of the power spectrum, which is symmetric, P(-k) = P(k):
 // MIRROR_ROWS, rows [0, rows/2] of length cols (full row r > rows/2 is row
 // rows - r reversed: P(r, c) = P(rows - r, (cols - c) % cols)); otherwise
 // all rows with columns [0, cols/2] (P(r, c) = P((rows - r) % rows, cols - c)
 // for c > cols/2).  S is written on the full rows x cols grid.
-constexpr int kVX = 64, kVY = 32, kVB = 4;
+constexpr int kVX = 64, kVY = 128, kVB = 4, kVRows = 32;
 template <bool MIRROR_ROWS>
 __device__ __forceinline__ float power_full(const float *P, int r, int c, int rows, int cols) {
     if (MIRROR_ROWS) {
@@ -598,56 +600,60 @@ __device__ __forceinline__ float power_full(const float *P, int r, int c, int ro
 template <bool MIRROR_ROWS>
 __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, float *S_all, int rows, int cols,
                                                            const double *var_all, int ntx, int nty, int ntiles) {
-    __shared__ float T[kVY + 2 * kVB][kVX + 2 * kVB];
+    constexpr int LW = kVX + 2 * kVB, LH = kVY + 2 * kVB;
+    __shared__ float T[LH][LW];
     const int per = (ntiles + 7) / 8;
     const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (t >= ntiles) return;
     const int b = t / (ntx * nty), rem = t - b * (ntx * nty);
-    const int r0 = (rem / ntx) * kVY, c0 = (rem % ntx) * kVX;
+    const int r0 = (rem / ntx) * kVY, c0 = (rem - (rem / ntx) * ntx) * kVX;
     const int64_t plane = (int64_t)rows * cols;
     const float *P = P_all + (int64_t)b * plane;
     float *S = S_all + (int64_t)b * plane;
-    constexpr int LW = kVX + 2 * kVB, LH = kVY + 2 * kVB;
-    for (int idx = threadIdx.x; idx < LW * LH; idx += 256) {
-        const int ly = idx / LW, lx = idx - (idx / LW) * LW;
-        const int gy = r0 + ly - kVB, gx = c0 + lx - kVB;
-        T[ly][lx] = (gy >= 0 && gy < rows && gx >= 0 && gx < cols) ? power_full<MIRROR_ROWS>(P, gy, gx, rows, cols)
-                                                                   : 0.f;
+    // thread -> (LDS row, group of 4 columns): 18 groups per row
+    for (int q = threadIdx.x; q < LH * (LW / 4); q += 256) {
+        const int ly = q / (LW / 4), lx = (q - ly * (LW / 4)) * 4;
+        const int gy = r0 + ly - kVB;
+        const bool row_in = gy >= 0 && gy < rows;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int gx = c0 + lx + k - kVB;
+            T[ly][lx + k] = (row_in && gx >= 0 && gx < cols) ? power_full<MIRROR_ROWS>(P, gy, gx, rows, cols) : 0.f;
+        }
     }
     __syncthreads();
-    const int c = threadIdx.x & 63, oy0 = (threadIdx.x >> 6) * 8;
-    // horizontal sums of LDS rows oy0 .. oy0 + 15 around column c + kVB
-    float h3[16], h5[16], h7[16], h9[16];
+    const int c = threadIdx.x & 63, oy0 = (threadIdx.x >> 6) * kVRows;
+    const float v = (float)var_all[b];
+    const int gx = c0 + c;
+    // LDS rows oy0 .. oy0 + 39; output row j sits at LDS row oy0 + j + 4
+    float h3[kVRows + 8], h5[kVRows + 8], h7[kVRows + 8], h9[kVRows + 8];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < kVRows + 8; ++i) {
         const float *row = &T[oy0 + i][c + kVB];
         h3[i] = row[-1] + row[0] + row[1];
         h5[i] = h3[i] + row[-2] + row[2];
         h7[i] = h5[i] + row[-3] + row[3];
         h9[i] = h7[i] + row[-4] + row[4];
-    }
-    const float v = (float)var_all[b];
-    const int gx = c0 + c;
+        const int j = i - 8;  // the last row output j needs
+        if (j >= 0) {
+            float s3 = 0.f, s5 = 0.f, s7 = 0.f, s9 = 0.f;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        // output row oy0 + j sits at LDS row oy0 + j + 4 = local index j + 4
-        float s3 = 0.f, s5 = 0.f, s7 = 0.f, s9 = 0.f;
+            for (int d = -1; d <= 1; ++d) s3 += h3[j + 4 + d];
 #pragma unroll
-        for (int d = -1; d <= 1; ++d) s3 += h3[j + 4 + d];
+            for (int d = -2; d <= 2; ++d) s5 += h5[j + 4 + d];
 #pragma unroll
-        for (int d = -2; d <= 2; ++d) s5 += h5[j + 4 + d];
+            for (int d = -3; d <= 3; ++d) s7 += h7[j + 4 + d];
 #pragma unroll
-        for (int d = -3; d <= 3; ++d) s7 += h7[j + 4 + d];
-#pragma unroll
-        for (int d = -4; d <= 4; ++d) s9 += h9[j + 4 + d];
-        // means by the rounded reciprocals of f^2 (within an ulp of the
-        // quotient; the divisions were most of this kernel's VALU time)
-        float best = s3 * (1.f / 9.f);
-        best = fminf(best, s5 * (1.f / 25.f));
-        best = fminf(best, s7 * (1.f / 49.f));
-        best = fminf(best, s9 * (1.f / 81.f));
-        const int gy = r0 + oy0 + j;
-        if (gy < rows && gx < cols) S[(int64_t)gy * cols + gx] = v / fmaxf(v, best);
+            for (int d = -4; d <= 4; ++d) s9 += h9[j + 4 + d];
+            // means by the rounded reciprocals of f^2, the quotient by the
+            // hardware reciprocal (each within an ulp or so)
+            float best = s3 * (1.f / 9.f);
+            best = fminf(best, s5 * (1.f / 25.f));
+            best = fminf(best, s7 * (1.f / 49.f));
+            best = fminf(best, s9 * (1.f / 81.f));
+            const int gy = r0 + oy0 + j;
+            if (gy < rows && gx < cols) S[(int64_t)gy * cols + gx] = v * __builtin_amdgcn_rcpf(fmaxf(v, best));
+        }
     }
 }
 
