@@ -421,7 +421,7 @@ __global__ __launch_bounds__(256) void zm_partials_kernel(const float *img_all, 
 __global__ __launch_bounds__(256) void zm_reduce_kernel(const double *colpart_all, int64_t cp_stride,
                                                         const double *rowpart_all, int64_t rp_stride, int h, int w,
                                                         int tiles_y, int tiles_x, double *colsum_all,
-                                                        double *rowmean_all) {
+                                                        double *rowmean_all, float *rz_all) {
     const int b = blockIdx.y;
     const double *colpart = colpart_all + (int64_t)b * cp_stride;
     const double *rowpart = rowpart_all + (int64_t)b * rp_stride;
@@ -445,12 +445,20 @@ __global__ __launch_bounds__(256) void zm_reduce_kernel(const double *colpart_al
         }
         rowmean[i] = se / (double)((w + 1) >> 1);
         rowmean[h + i] = so / (double)(w >> 1);
+        if (rz_all) {
+            float *rz = rz_all + (int64_t)b * 2 * h;
+            rz[i] = (float)rowmean[i];
+            rz[h + i] = (float)rowmean[h + i];
+        }
     }
 }
 
 // g[pi*2+pj] = mean of the (row parity pi, column parity pj) sub-lattice; also
-// turns colsum into column means in place.  grid (B), block 256.
-__global__ __launch_bounds__(256) void zm_lattice_kernel(double *colsum_all, int h, int w, double *g_all) {
+// turns colsum into column means in place, and (cz_all non-null) writes the
+// FP32 column terms cz[pi][x] = colmean[pi][x] - g[pi][x & 1] of the
+// Wiener row kernel.  grid (B), block 256.
+__global__ __launch_bounds__(256) void zm_lattice_kernel(double *colsum_all, int h, int w, double *g_all,
+                                                         float *cz_all) {
     __shared__ double sh[4];
     double *colsum = colsum_all + (int64_t)blockIdx.x * 2 * w;
     double *g = g_all + (int64_t)blockIdx.x * 4;
@@ -470,9 +478,16 @@ __global__ __launch_bounds__(256) void zm_lattice_kernel(double *colsum_all, int
         g[2] = tot[2] / (rows_o * cols_e);
         g[3] = tot[3] / (rows_o * cols_o);
     }
+    const double gl[4] = {tot[0] / (rows_e * cols_e), tot[1] / (rows_e * cols_o), tot[2] / (rows_o * cols_e),
+                          tot[3] / (rows_o * cols_o)};
+    float *cz = cz_all ? cz_all + (int64_t)blockIdx.x * 2 * w : nullptr;
     for (int x = threadIdx.x; x < w; x += 256) {
         colsum[x] /= rows_e;
         colsum[w + x] /= rows_o;
+        if (cz) {
+            cz[x] = (float)(colsum[x] - gl[x & 1]);
+            cz[w + x] = (float)(colsum[w + x] - gl[2 + (x & 1)]);
+        }
     }
 }
 
@@ -512,22 +527,27 @@ __global__ __launch_bounds__(256) void zm_apply_kernel(float *img_all, int64_t i
 // Rows 2r and 2r+1 of image b: (optional zero-mean) -> sums of squares
 // (FP64) -> forward FFT (w) -> half spectra into A (h x (w/2+1)).
 // grid (ceil(h/2), B).
+// Zero-mean (cz non-null): x - cz[row parity][j] - rz[j & 1][row] in FP32,
+// the column terms cz = colmean - g and row means rz of zm_lattice_kernel /
+// zm_reduce_kernel (within an ulp of the FP64 closed form zm_value).
 template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img_all,
                                                                       int64_t img_stride, int h, float2 *A_all,
                                                                       int64_t plane, double *rowsq_all,
-                                                                      const double *colmean_all,
-                                                                      const double *rowmean_all, const double *g_all) {
+                                                                      const float *cz_all, const float *rz_all) {
     extern __shared__ float2 buf[];
-    __shared__ double sh[4];
+    __shared__ double sh[8];
     const int i0 = 2 * blockIdx.x, i1 = i0 + 1, b = blockIdx.y;
     const bool two = i1 < h;
     const int w = d.n, wh = w / 2 + 1;
     const float *img = img_all + (int64_t)b * img_stride;
-    const bool zm = colmean_all != nullptr;
-    const double *colmean = zm ? colmean_all + (int64_t)b * 2 * w : nullptr;
-    const double *rowmean = zm ? rowmean_all + (int64_t)b * 2 * h : nullptr;
-    const double *g = zm ? g_all + (int64_t)b * 4 : nullptr;
+    const bool zm = cz_all != nullptr;
+    // rows i0 (even) and i1 (odd): column terms of parities 0 and 1
+    const float *cz0 = zm ? cz_all + (int64_t)b * 2 * w : nullptr;
+    const float *cz1 = zm ? cz0 + w : nullptr;
+    const float *rz = zm ? rz_all + (int64_t)b * 2 * h : nullptr;
+    const float ra0 = zm ? rz[i0] : 0.f, ra1 = zm ? rz[h + i0] : 0.f;   // row i0, even / odd columns
+    const float rb0 = zm && two ? rz[i1] : 0.f, rb1 = zm && two ? rz[h + i1] : 0.f;
     double sq0 = 0.0, sq1 = 0.0;
     const float *r0 = img + (int64_t)i0 * w;
     const float *r1 = img + (int64_t)(two ? i1 : i0) * w;
@@ -536,8 +556,9 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
         [&](int j) {
             float va = r0[j], vb = two ? r1[j] : 0.f;
             if (zm) {
-                va = zm_value(va, i0, j, h, w, colmean, rowmean, g);
-                if (two) vb = zm_value(vb, i1, j, h, w, colmean, rowmean, g);
+                const bool odd = j & 1;
+                va = (va - cz0[j]) - (odd ? ra1 : ra0);
+                vb = two ? (vb - cz1[j]) - (odd ? rb1 : rb0) : 0.f;
             }
             sq0 += (double)va * (double)va;
             sq1 += (double)vb * (double)vb;
@@ -550,11 +571,17 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
         a0[k] = make_float2(0.5f * (zk.x + zm_.x), 0.5f * (zk.y - zm_.y));
         if (two) a0[wh + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
     }
-    sq0 = block_sum_256(sq0, sh);
-    sq1 = block_sum_256(sq1, sh);
+    // both row sums in one fixed-order block reduction
+    sq0 = wave_sum_d(sq0);
+    sq1 = wave_sum_d(sq1);
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = sq0;
+        sh[4 + (threadIdx.x >> 6)] = sq1;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        rowsq_all[(int64_t)b * h + i0] = sq0;
-        if (two) rowsq_all[(int64_t)b * h + i1] = sq1;
+        rowsq_all[(int64_t)b * h + i0] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        if (two) rowsq_all[(int64_t)b * h + i1] = (sh[4] + sh[5]) + (sh[6] + sh[7]);
     }
 }
 
@@ -847,6 +874,7 @@ struct ExtractWs {
     float2 *A, *B;          // complex planes of the Wiener filter (B: transposed path only)
     float *P, *S;           // power spectrum / scale (S also holds the FastNoise output)
     double *rowsq, *var, *colpart, *rowpart, *colmean, *rowmean, *g;
+    float *cz, *rz;         // FP32 zero-mean column terms (2w) and row means (2h) per image
     int64_t cp_stride, rp_stride;
     int tiles_x, tiles_y;
 };
@@ -861,7 +889,7 @@ size_t per_image_bytes(int h, int w) {
     const int64_t n = (int64_t)h * w;
     const int tx = zm_tiles_x_max(w), ty = zm_tiles_y_max(h);
     return (strips_for(h) ? 16 : 24) * (size_t)n + 8 * (size_t)(h + 1 + 2 * w * ty + 2 * h * tx + 2 * w + 2 * h + 4) +
-           11 * 256;
+           4 * (size_t)(2 * w + 2 * h) + 13 * 256;
 }
 
 int get_ws(Context *c, int nb, ExtractWs &e) {
@@ -880,15 +908,17 @@ int get_ws(Context *c, int nb, ExtractWs &e) {
                          align256(sizeof(double) * h * nb),   align256(sizeof(double) * nb),
                          align256(sizeof(double) * e.cp_stride * nb), align256(sizeof(double) * e.rp_stride * nb),
                          align256(sizeof(double) * 2 * w * nb), align256(sizeof(double) * 2 * h * nb),
-                         align256(sizeof(double) * 4 * nb)};
+                         align256(sizeof(double) * 4 * nb),   align256(sizeof(float) * 2 * w * nb),
+                         align256(sizeof(float) * 2 * h * nb)};
     size_t total = 0;
     for (size_t s : sz) total += s;
     PRNU_TRY(c->ws_filter.ensure(total));
     char *p = static_cast<char *>(c->ws_filter.ptr);
     void **slots[] = {(void **)&e.A,       (void **)&e.B,       (void **)&e.P,       (void **)&e.S,
                       (void **)&e.rowsq,   (void **)&e.var,     (void **)&e.colpart, (void **)&e.rowpart,
-                      (void **)&e.colmean, (void **)&e.rowmean, (void **)&e.g};
-    for (int k = 0; k < 11; ++k) {
+                      (void **)&e.colmean, (void **)&e.rowmean, (void **)&e.g,       (void **)&e.cz,
+                      (void **)&e.rz};
+    for (int k = 0; k < 13; ++k) {
         *slots[k] = p;
         p += sz[k];
     }
@@ -971,9 +1001,9 @@ int launch_zm_means(Context *c, ExtractWs &e, int nb, hipStream_t st) {
     const int h = c->h, w = c->w;
     const int m = h > w ? h : w;
     hipLaunchKernelGGL(zm_reduce_kernel, dim3((m + 255) / 256, nb), dim3(256), 0, st, e.colpart, e.cp_stride,
-                       e.rowpart, e.rp_stride, h, w, e.tiles_y, e.tiles_x, e.colmean, e.rowmean);
+                       e.rowpart, e.rp_stride, h, w, e.tiles_y, e.tiles_x, e.colmean, e.rowmean, e.rz);
     PRNU_CHECK_LAUNCH();
-    hipLaunchKernelGGL(zm_lattice_kernel, dim3(nb), dim3(256), 0, st, e.colmean, h, w, e.g);
+    hipLaunchKernelGGL(zm_lattice_kernel, dim3(nb), dim3(256), 0, st, e.colmean, h, w, e.g, e.cz);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -1010,7 +1040,7 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     const FftDesc &dh = fft_desc(c->fft_h);
     const int pairs = (h + 1) / 2;
     PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(pairs, nb), st, dw, img, img_stride, h, e.A, n, e.rowsq,
-                    zm ? e.colmean : nullptr, zm ? e.rowmean : nullptr, zm ? e.g : nullptr);
+                    zm ? e.cz : nullptr, zm ? e.rz : nullptr);
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
     PRNU_CHECK_LAUNCH();
