@@ -449,7 +449,8 @@ def synth_images(torch, fingerprints, owners, seed, h=None, w=None):
 def serving_bench(torch, duration: float):
     """Poisson bursts (uniform 1..64 queries, mean rate 200 q/s) against a
     20,000-fingerprint FP8 store at 1024x1024; p50/p99 latency from submit
-    to result, queries/s over the run.  Slots 0..63 hold enrolled cameras
+    to result, queries/s over the run; then 512 queries at once (the
+    saturated rate, both slots busy).  Slots 0..63 hold enrolled cameras
     (fingerprint = mean extracted pattern of 4 images of the camera), the
     rest random fingerprints; half of the queries come from enrolled cameras."""
     from rocket_amd import _lib, csi, serving
@@ -489,7 +490,7 @@ def serving_bench(torch, duration: float):
     for b in range(0, total, 128):
         imgs.append(synth_images(torch, known, owners[b:b + 128], seed=b).cpu().numpy())
     imgs = np.concatenate(imgs)
-    srv = serving.Server(store, k=10, max_batch=64, slots=2).start()
+    srv = serving.Server(store, k=10, max_batch=64, slots=2, timeline=True).start()
     reqs = []
     start = time.perf_counter()
     q = 0
@@ -503,12 +504,33 @@ def serving_bench(torch, duration: float):
     for r in reqs:
         r.wait(120)
     wall = max(r.t_done for r in reqs) - start
+    nb_poisson = len(srv.batches)
+    # saturation: 512 queries submitted at once, so both slots stay busy (one
+    # batch's extraction beside the other's store scan); then the same with
+    # one slot.  Device spans of the batches from the server's timeline.
+    def saturate(server, nskip):
+        t_sat = time.perf_counter()
+        sreqs = [server.submit(imgs[i % len(imgs)]) for i in range(512)]
+        for r in sreqs:
+            r.wait(120)
+        wall = max(r.t_done for r in sreqs) - t_sat
+        spans = server.timeline()[nskip:]
+        overlap = sum(max(0.0, min(a[3], b[3]) - b[2]) for a, b in zip(spans, spans[1:]) if a[0] != b[0])
+        return len(sreqs) / wall, spans, overlap
+    sat_qps, sat_spans, sat_overlap = saturate(srv, nb_poisson)
     srv.stop()
+    one = serving.Server(store, k=10, max_batch=64, slots=1, timeline=True).start()
+    one_qps, _, _ = saturate(one, 0)
+    one.stop()
     lat = np.array([r.latency for r in reqs]) * 1000.0
     hit = [r.idx[0] == o for r, o in zip(reqs, owners) if o >= 0]
     res = {"queries": len(reqs), "bursts": len(bursts), "offered_qps": len(reqs) / bursts[-1][0],
            "qps": len(reqs) / wall, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-           "mean_batch": float(np.mean(srv.batches)), "max_batch": int(max(srv.batches)),
+           "mean_batch": float(np.mean(srv.batches[:nb_poisson])), "max_batch": int(max(srv.batches[:nb_poisson])),
+           "saturated_qps": sat_qps, "saturated_batches": len(sat_spans),
+           "saturated_qps_one_slot": one_qps,
+           "saturated_overlap_ms": sat_overlap,
+           "saturated_spans_ms": [[s_, m_, round(a_, 3), round(b_, 3)] for s_, m_, a_, b_ in sat_spans[:4]],
            "top1_known_camera": float(np.mean(hit)) if hit else None,
            "store": f"{SERVE_STORE} x {SERVE_W}x{SERVE_H} FP8-e4m3 + FP32 originals", "k": 10}
     # the store scan of one batch of 64 queries, timed by the context's HIP
