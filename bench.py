@@ -529,6 +529,7 @@ def serving_bench(torch, duration: float):
            "mean_batch": float(np.mean(srv.batches[:nb_poisson])), "max_batch": int(max(srv.batches[:nb_poisson])),
            "saturated_qps": sat_qps, "saturated_batches": len(sat_spans),
            "saturated_qps_one_slot": one_qps,
+           "slot_stream_check": srv.stream_check,
            "saturated_overlap_ms": sat_overlap,
            "saturated_spans_ms": [[s_, m_, round(a_, 3), round(b_, 3)] for s_, m_, a_, b_ in sat_spans[:4]],
            "top1_known_camera": float(np.mean(hit)) if hit else None,

@@ -189,6 +189,28 @@ def batch_bytes(n: int, size: int, k: int, m: int, align: int = 256) -> int:
     return r(m * n) + r(4 * m * n) + r(8 * m * size) + r(4 * m * k) + r(8 * m * k)
 
 
+def _new_stream() -> "torch.cuda.ExternalStream":
+    """A HIP stream created by the library, wrapped for torch.  It is never
+    destroyed: torch's pinned-memory allocator remembers the streams that used
+    a pinned block (the staging copies run on it) and records events on them
+    when the block is freed, possibly after the server is gone."""
+    h = ctypes.c_void_p()
+    _lib.call("prnu_stream_create", ctypes.byref(h))
+    return torch.cuda.ExternalStream(h.value)
+
+
+def _spin_together(streams, cycles: int) -> float:
+    """Seconds for a bounded spin kernel (torch's clock-cycle sleep, one
+    thread) on each of `streams`, launched together."""
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for st in streams:
+        with torch.cuda.stream(st):
+            torch.cuda._sleep(cycles)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t
+
+
 class _Slot:
     """One in-flight batch: its own context, stream, pinned host staging and
     device pointer table; the batch's device buffers come from the arena."""
@@ -196,16 +218,9 @@ class _Slot:
     def __init__(self, height: int, width: int, max_batch: int, k: int):
         n = height * width
         self.ctx = _Ctx(height, width, _lib.MODE_NCC)
-        # the slot's own HIP stream (created by the library, wrapped for
-        # torch): streams handed out by torch's pool can share one hardware
-        # queue, which serialises the slots' batches on the device.  It is
-        # never destroyed: torch's pinned-memory allocator remembers the
-        # streams that used a pinned block (the staging copies run on this
-        # one) and records events on them when the block is freed, possibly
-        # after the server is gone.
-        h = ctypes.c_void_p()
-        _lib.call("prnu_stream_create", ctypes.byref(h))
-        self.stream = torch.cuda.ExternalStream(h.value)
+        # the slot's own HIP stream (Server._spread_streams checks that the
+        # slots' streams run side by side)
+        self.stream = _new_stream()
         self.d_ptrs = torch.zeros(max_batch, dtype=torch.int64, device="cuda")
         self.h_ptrs = torch.zeros(max_batch, dtype=torch.int64, pin_memory=True)
         self.h_images = torch.empty((max_batch, n), dtype=torch.uint8, pin_memory=True)
@@ -283,6 +298,7 @@ class Server:
         self._arena_buf = torch.empty(budget, dtype=torch.uint8, device="cuda")
         self.arena = DeviceArena(budget, base=self._arena_buf.data_ptr())
         self.slots = [_Slot(self.h, self.w, self.max_batch, self.k) for _ in range(slots)]
+        self._spread_streams()
         self.pending: "queue.Queue[Request]" = queue.Queue()
         self.inflight: "queue.Queue[_Slot | None]" = queue.Queue()
         self._stop = threading.Event()
@@ -302,6 +318,30 @@ class Server:
         # (device-side spans; a profiler's kernel trace serialises the slots'
         # queues, so overlap is measured this way)
         self._timeline: list | None = [] if timeline else None
+
+    def _spread_streams(self, cycles: int = 2_000_000, attempts: int = 8) -> None:
+        """Give every slot a stream on its own hardware queue.  HIP maps
+        streams onto a few hardware queues (GPU_MAX_HW_QUEUES, 4 by default),
+        each to the least used one, so two streams created one after the
+        other can share a queue; the slots' batches then run one after the
+        other on the device.  Checked by timing: spin kernels on streams of
+        different queues run together, on one queue they take the sum.  A
+        slot whose stream shares a queue gets a new stream (up to `attempts`
+        times; the old one is kept, unused)."""
+        self.stream_check: dict = {"slots": len(self.slots)}
+        if len(self.slots) < 2:
+            return
+        _spin_together([self.slots[0].stream], cycles)           # the spin kernel's first launch loads it
+        alone = _spin_together([self.slots[0].stream], cycles)
+        self.stream_check.update(alone_ms=1e3 * alone, together_ms=[], new_streams=0)
+        for i in range(1, len(self.slots)):
+            for _ in range(attempts):
+                t = _spin_together([s.stream for s in self.slots[:i + 1]], cycles)
+                self.stream_check["together_ms"].append(1e3 * t)
+                if t < 1.5 * alone:
+                    break
+                self.slots[i].stream = _new_stream()
+                self.stream_check["new_streams"] += 1
 
     @staticmethod
     def mem_info() -> tuple[int, int]:
