@@ -220,9 +220,14 @@ inline size_t fft_row_smem(int n) { return sizeof(float2) * (size_t)n; }
 constexpr int kFftSmallN = 2048;
 constexpr int kFftMidN = 4096;
 
-template <int R, int DIR, bool FIRST, bool LAST, int MAXN, class Get, class Put>
-__device__ __forceinline__ void fft_pass_io(float2 *buf, int n, int Ns, const float2 *tw, Get &get, Put &put) {
-    constexpr int kMaxB = (MAXN / R + kFftThreads - 1) / kFftThreads;
+// CN > 0: the length (and CNS the pass's Ns) are compile-time constants, so
+// strides, twiddle steps and the butterfly index's modulus fold into
+// immediates (no integer division, LDS offsets in the instructions).
+template <int R, int DIR, bool FIRST, bool LAST, int MAXN, int CN = 0, int CNS = 0, class Get, class Put>
+__device__ __forceinline__ void fft_pass_io(float2 *buf, int n_rt, int Ns_rt, const float2 *tw, Get &get, Put &put) {
+    constexpr int kMaxB = ((CN > 0 ? CN : MAXN) / R + kFftThreads - 1) / kFftThreads;
+    const int n = CN > 0 ? CN : n_rt;
+    const int Ns = CN > 0 ? CNS : Ns_rt;
     const int nb = n / R;
     const int twstep = n / (Ns * R);
     const int tid = threadIdx.x;
@@ -277,12 +282,75 @@ __device__ __forceinline__ void fft_pass_io_r(int R, float2 *buf, int n, int Ns,
         fft_pass_io<5, DIR, FIRST, LAST, MAXN>(buf, n, Ns, tw, get, put);
 }
 
+// Radix plan of fft_factor (fft.hip) as a compile-time value, for the row
+// lengths the chain and PCE run most.
+struct RadixPlanC {
+    int passes;
+    int radix[kFftMaxPasses];
+};
+constexpr RadixPlanC radix_plan_c(int n) {
+    RadixPlanC p{0, {}};
+    int e2 = 0;
+    while (n % 2 == 0) {
+        ++e2;
+        n /= 2;
+    }
+    int n8 = e2 / 3, n4 = 0, n2 = 0;
+    if (e2 % 3 == 2) {
+        n4 = 1;
+    } else if (e2 % 3 == 1) {
+        if (n8 > 0) {
+            --n8;
+            n4 = 2;
+        } else {
+            n2 = 1;
+        }
+    }
+    for (int i = 0; i < n8; ++i) p.radix[p.passes++] = 8;
+    for (int i = 0; i < n4; ++i) p.radix[p.passes++] = 4;
+    for (int i = 0; i < n2; ++i) p.radix[p.passes++] = 2;
+    while (n % 3 == 0) {
+        p.radix[p.passes++] = 3;
+        n /= 3;
+    }
+    while (n % 5 == 0) {
+        p.radix[p.passes++] = 5;
+        n /= 5;
+    }
+    return p;
+}
+
+template <int DIR, int N, int P, int NS, class Get, class Put>
+__device__ __forceinline__ void fft_passes_c(float2 *buf, const float2 *tw, Get &get, Put &put) {
+    constexpr RadixPlanC pl = radix_plan_c(N);
+    if constexpr (P < pl.passes) {
+        constexpr int R = pl.radix[P];
+        fft_pass_io<R, DIR, P == 0, P == pl.passes - 1, N, N, NS>(buf, N, NS, tw, get, put);
+        fft_passes_c<DIR, N, P + 1, NS * R>(buf, tw, get, put);
+    }
+}
+
+// fft_row for a compile-time length N (at least two passes).
+template <int DIR, int N, class Get, class Put>
+__device__ __forceinline__ void fft_row_c(float2 *buf, const float2 *tw, Get &get, Put &put) {
+    static_assert(radix_plan_c(N).passes >= 2, "fft_row_c: at least two passes");
+    fft_passes_c<DIR, N, 0, 1>(buf, tw, get, put);
+}
+
 // Transform of one row of length d.n <= MAXN from get(i) (i in [0, n), each
 // read once) to put(i, X[i]) (each written once), through LDS buf
 // (fft_row_smem(n) bytes) for the passes between; ends with a barrier, so
 // the caller may reuse buf.
 template <int DIR, int MAXN, class Get, class Put>
 __device__ inline void fft_row(float2 *buf, const FftDesc &d, Get get, Put put) {
+    // the hot lengths with compile-time strides
+    if constexpr (MAXN >= 2048) {
+        if (d.n == 2048) return fft_row_c<DIR, 2048>(buf, d.tw, get, put);
+        if (d.n == 1536) return fft_row_c<DIR, 1536>(buf, d.tw, get, put);
+    }
+    if constexpr (MAXN >= 1024) {
+        if (d.n == 1024) return fft_row_c<DIR, 1024>(buf, d.tw, get, put);
+    }
     const int P = d.passes;
     if (P == 1) {
         fft_pass_io_r<DIR, true, true, MAXN>(d.radix[0], buf, d.n, 1, d.tw, get, put);
