@@ -172,6 +172,53 @@ __global__ __launch_bounds__(kFftThreads) void pce_rows2_kernel(FftDesc d, int h
         [&](int j, float2 f) { A[j] = f; });
 }
 
+// Both rows of a mirror pair u, h - u (one when u is its own mirror) of
+// couple blockIdx.y.  The Hermitian parts of row u, H(u, j) for every j,
+// are formed once into LDS; row -u needs no loads of its own, since
+// H(-u, j) = conj(H(u, -j)) (H is the Hermitian part of the cross spectrum
+// in both variants), so its transform input is
+// Z(-u, j) = conj(H_a(u, -j)) + i conj(H_b(u, -j)).  That halves the
+// spectrum reads of pce_rows2_kernel, whose two workgroups per pair each
+// form their own row's Hermitian parts.  grid (h/2 + 1, couples), LDS 3w
+// complex (H_a, H_b of row u, the transform buffer).
+template <int VARIANT, int MAXN>
+__global__ __launch_bounds__(kFftThreads) void pce_rows_mirror_kernel(FftDesc d, int h, int nb, PceBatch bt,
+                                                                      float2 *__restrict__ Aall, int64_t pstride) {
+    extern __shared__ float2 smem[];
+    const int w = d.n;
+    float2 *ha = smem, *hb = smem + w, *buf = smem + 2 * w;
+    const int u = blockIdx.x;
+    if (u > h / 2) return;
+    const int um = (h - u) % h;
+    const int q = blockIdx.y;
+    const float2 *Xa, *Ya, *Xb = nullptr, *Yb = nullptr;
+    pce_operands(bt, 2 * q, Xa, Ya);
+    const bool two = 2 * q + 1 < nb;
+    if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
+    for (int j = threadIdx.x; j < w; j += kFftThreads) {
+        ha[j] = herm_at<VARIANT>(Xa, Ya, h, w, u, j);
+        hb[j] = two ? herm_at<VARIANT>(Xb, Yb, h, w, u, j) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 *A = Aall + (int64_t)q * pstride;
+    fft_row<1, MAXN>(
+        buf, d,
+        [&](int j) {
+            const float2 a = ha[j], b = hb[j];
+            return make_float2(a.x - b.y, a.y + b.x);
+        },
+        [&](int j, float2 f) { A[(int64_t)u * w + j] = f; });
+    if (um == u) return;
+    fft_row<1, MAXN>(
+        buf, d,
+        [&](int j) {
+            const int mj = j == 0 ? 0 : w - j;
+            const float2 a = ha[mj], b = hb[mj];
+            return make_float2(a.x + b.y, b.x - a.y);
+        },
+        [&](int j, float2 f) { A[(int64_t)um * w + j] = f; });
+}
+
 // Column j of couple blockIdx.y from the transposed row-pass output
 // (B[j*h + i] = A[i][j]): inverse FFT of length h; every output value goes
 // straight into this thread's sums of Re (pair 2q) and Im (pair 2q + 1) --
@@ -413,8 +460,19 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     const FftDesc &dh = fft_desc(c->fft_h);
     const dim3 rgrid((unsigned)((h / 2 + 1 + 7) / 8 * 16), couples);  // mirror pairs of rows (pce_rows2_kernel)
     const size_t rsm = fft_row_smem(w);
-#define PRNU_PCE_ROWS(V, M) \
-    hipLaunchKernelGGL((pce_rows2_kernel<V, M>), rgrid, dim3(kFftThreads), rsm, st, dw, h, nb, bt, A, pstride)
+    // rows up to 1024: both rows of a mirror pair in one workgroup (24 KiB of LDS)
+    const bool mirror = w <= 1024;
+    const dim3 mgrid((unsigned)(h / 2 + 1), couples);
+    const size_t msm = 3 * sizeof(float2) * (size_t)w;
+#define PRNU_PCE_ROWS(V, M)                                                                                          \
+    do {                                                                                                             \
+        if (mirror)                                                                                                  \
+            hipLaunchKernelGGL((pce_rows_mirror_kernel<V, M>), mgrid, dim3(kFftThreads), msm, st, dw, h, nb, bt, A, \
+                               pstride);                                                                             \
+        else                                                                                                         \
+            hipLaunchKernelGGL((pce_rows2_kernel<V, M>), rgrid, dim3(kFftThreads), rsm, st, dw, h, nb, bt, A,       \
+                               pstride);                                                                             \
+    } while (0)
     if (variant == PRNU_PCE_REFERENCE) {
         if (w <= kFftSmallN) PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, kFftSmallN);
         else if (w <= kFftMidN) PRNU_PCE_ROWS(PRNU_PCE_REFERENCE, kFftMidN);
