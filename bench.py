@@ -847,6 +847,25 @@ def spill_bench(torch):
            "block_loads": info["block_loads"], "device_blocks": SPILL_DEVICE_BLOCKS,
            "resident_pairs_per_s": pairs / dt_res, "resident_ms": 1000.0 * dt_res,
            "max_abs_diff_vs_resident": float(np.abs(got[iu] - ref[iu]).max())}
+    # the same tier through the device-cache policy: 4 block slots, leaves of
+    # the block triangle in Hilbert order and, for contrast, row by row
+    from rocket_amd import residency
+    cg = residency.CachedBlockGram(gram, SPILL_N, SPILL_BLOCK, "fp16", nslots=SPILL_DEVICE_BLOCKS, window=3)
+    cg.warm(SPILL_BLOCK)
+    src = residency.TierSource(tier)
+    cached = {"slots": SPILL_DEVICE_BLOCKS, "window": 3}
+    for order in ("hilbert", "tiling"):
+        cg.reset()                                            # cold cache, plans built
+        torch.cuda.synchronize()
+        m, st = cg.ncc(src, range(tier.nblocks), order=order)
+        cached[order] = {"pairs_per_s": pairs / st["seconds"], "ms": 1000.0 * st["seconds"],
+                         "block_loads": st["block_loads"], "hits": st["hits"], "misses": st["misses"],
+                         "evictions": st["evictions"], "bytes_to_device_GB": st["bytes_to_device"] / 1e9,
+                         "max_abs_diff_vs_resident": float(np.abs(m[iu] - ref[iu]).max()),
+                         "equal_to_spill_schedule": bool(np.array_equal(m[iu], got[iu]))}
+        del m
+    cg.close()
+    res["device_cache"] = cached
     tier.close()
     resident.close()
     gram.close()
@@ -1101,7 +1120,11 @@ def main():
         spl = spill_bench(torch)
         log(f"spill tier: {spl['pairs_per_s']:.4g} pairs/s streamed from {spl['host_tier_GB']:.1f} GB of host "
             f"slot images ({spl['effective_h2d_GBps']:.1f} GB/s to the device), resident "
-            f"{spl['resident_pairs_per_s']:.4g} pairs/s")
+            f"{spl['resident_pairs_per_s']:.4g} pairs/s; through the device cache "
+            f"{spl['device_cache']['hilbert']['pairs_per_s']:.4g} pairs/s in Hilbert order "
+            f"({spl['device_cache']['hilbert']['block_loads']} block loads), "
+            f"{spl['device_cache']['tiling']['pairs_per_s']:.4g} row by row "
+            f"({spl['device_cache']['tiling']['block_loads']})")
         log("JPEG ...")
         jpg = jpeg_bench(torch)
         log(f"JPEG: {jpg['decode_images_per_s']:.0f} images/s decoded "

@@ -427,6 +427,61 @@ PRNU_EXPORT int prnu_index_divide(int kind, const prnu_index_node *node, int par
 PRNU_EXPORT int prnu_index_pairs(int kind, int count, int param, int include_diagonal, int32_t *pairs,
                                  int64_t pairs_cap, int64_t *n_pairs);
 
+/* ------------------------------------------------------ device-cache policy
+ * Host-only (no device call).  Which keys (pattern blocks) occupy which of
+ * `nslots` slots of a pattern store: the policy of R/cache/AbstractCache.java:
+ * 19-409 over the buffer stack of R/cache/DeviceCache.java:19-62.  Entries
+ * are reference counted; status WRITING (one holder fills it), READONLY
+ * (filled, shared) or INVALID (a writer released it unfilled).  Transactions
+ * are served in submission order; one that does not fit waits at the head,
+ * holding what it has, while unreferenced entries are evicted least recently
+ * used first (a released readable entry is most recently used, a new or
+ * invalid one least).  Thread-safe: one lock per cache. */
+typedef struct prnu_slotcache prnu_slotcache;
+typedef enum { PRNU_SLOT_WRITING = 0, PRNU_SLOT_READONLY = 1, PRNU_SLOT_INVALID = 2 } prnu_slot_status;
+
+/* Slots for entries of `entry_bytes`: R/RocketLauncher.java:371,391-398 (the
+ * request if positive, else free memory plus that negative margin, at most
+ * 5000 entries) with DeviceCache.java:27-31 (the largest n with
+ * n * entry_bytes below the total). */
+PRNU_EXPORT int prnu_slotcache_slots_for(size_t entry_bytes, int64_t requested, size_t free_bytes,
+                                         int64_t *nslots);
+/* new DeviceCache(ctx, total, entry) without the allocation: slots
+ * 0..nslots-1 (the caller's store), nslots >= 2 (DeviceWorker.java:87-89). */
+PRNU_EXPORT int prnu_slotcache_create(int64_t nslots, prnu_slotcache **cache);
+PRNU_EXPORT int prnu_slotcache_destroy(prnu_slotcache *cache);
+/* AbstractCache.acquireAllAsync(keys...): duplicates removed; *txn numbers
+ * the transaction (1, 2, ... per cache). */
+PRNU_EXPORT int prnu_slotcache_acquire(prnu_slotcache *cache, const int64_t *keys, int n, int64_t *txn);
+/* The transaction's future: *done = 0 while it waits (*n = its key count);
+ * once complete, keys / slots / writer (1: the caller fills the slot and
+ * calls prnu_slotcache_ready, a miss -- DeviceWorker.java:208-226) for its
+ * *n distinct keys, and the transaction is retired (polling it again is
+ * PRNU_EINVAL).  Hits and misses are counted here.  With keys, slots and
+ * writer all NULL only *done and *n are reported (nothing is retired). */
+PRNU_EXPORT int prnu_slotcache_poll(prnu_slotcache *cache, int64_t txn, int *done, int64_t *keys, int64_t *slots,
+                                    int32_t *writer, int cap, int *n);
+/* Entry.downgradeToReader(meta): PRNU_EINVAL unless the entry is WRITING. */
+PRNU_EXPORT int prnu_slotcache_ready(prnu_slotcache *cache, int64_t key, int64_t meta);
+/* Entry.release(): one reference dropped; PRNU_EINVAL at refcount 0. */
+PRNU_EXPORT int prnu_slotcache_release(prnu_slotcache *cache, int64_t key);
+/* tryAcquireImmediately(key): *slot of a READONLY entry, now referenced, or -1. */
+PRNU_EXPORT int prnu_slotcache_try_acquire(prnu_slotcache *cache, int64_t key, int64_t *slot);
+/* Entry.size(): the meta of a READONLY entry, else PRNU_EINVAL. */
+PRNU_EXPORT int prnu_slotcache_meta(prnu_slotcache *cache, int64_t key, int64_t *meta);
+/* Inspection: slot, prnu_slot_status (-1: not resident) and refcount. */
+PRNU_EXPORT int prnu_slotcache_entry(prnu_slotcache *cache, int64_t key, int64_t *slot, int *status,
+                                     int64_t *refcount);
+/* Unreferenced keys in eviction order (first evicted first); keys == NULL
+ * only counts. */
+PRNU_EXPORT int prnu_slotcache_lru(prnu_slotcache *cache, int64_t *keys, int64_t cap, int64_t *n);
+/* stats[8]: hits, misses, evictions, resident entries, free slots, waiting
+ * transactions, slots, unreferenced entries. */
+PRNU_EXPORT int prnu_slotcache_stats(prnu_slotcache *cache, int64_t *stats);
+/* AbstractCache.cleanup(): evict everything; PRNU_EINVAL if any entry is
+ * still referenced. */
+PRNU_EXPORT int prnu_slotcache_cleanup(prnu_slotcache *cache);
+
 /* ------------------------------------------------------------ JPEG input
  * ReadJPEG.readJPEG / readJPEG (CSI/ReadJPEG.java:25-45, readjpeg/src/main/
  * cpp/native.cpp:59-134): libjpeg's default decode of a 3-component JPEG to

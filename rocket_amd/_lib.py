@@ -111,6 +111,21 @@ _SIGNATURES = {
     "prnu_index_divide": (_c_int, [_c_int, _c_void_p, _c_int, _c_int, _c_void_p, _c_int, ctypes.POINTER(_c_int),
                                    _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
     "prnu_index_pairs": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_slots_for": (_c_int, [_c_size, _c_i64, _c_size, ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_create": (_c_int, [_c_i64, ctypes.POINTER(_c_void_p)]),
+    "prnu_slotcache_destroy": (_c_int, [_c_void_p]),
+    "prnu_slotcache_acquire": (_c_int, [_c_void_p, _c_void_p, _c_int, ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_poll": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(_c_int), _c_void_p, _c_void_p, _c_void_p,
+                                     _c_int, ctypes.POINTER(_c_int)]),
+    "prnu_slotcache_ready": (_c_int, [_c_void_p, _c_i64, _c_i64]),
+    "prnu_slotcache_release": (_c_int, [_c_void_p, _c_i64]),
+    "prnu_slotcache_try_acquire": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_meta": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_entry": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int),
+                                      ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_lru": (_c_int, [_c_void_p, _c_void_p, _c_i64, ctypes.POINTER(_c_i64)]),
+    "prnu_slotcache_stats": (_c_int, [_c_void_p, _c_void_p]),
+    "prnu_slotcache_cleanup": (_c_int, [_c_void_p]),
     "prnu_jpeg_decoder_create": (_c_int, [ctypes.POINTER(_c_void_p)]),
     "prnu_jpeg_decoder_destroy": (_c_int, [_c_void_p]),
     "prnu_jpeg_info": (_c_int, [_c_void_p, _c_i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
