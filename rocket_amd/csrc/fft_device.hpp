@@ -146,6 +146,47 @@ __device__ __forceinline__ void dft<8, -1>(float2 *a) { dft8<-1>(a); }
 template <>
 __device__ __forceinline__ void dft<8, 1>(float2 *a) { dft8<1>(a); }
 
+// Radix 16 as 4 x 4 (n = 4 n1 + n2, k = k1 + 4 k2): DFT-4 over n1, the
+// twiddle W16^(n2 k1), DFT-4 over n2; W16 = exp(DIR * 2*pi*i / 16).
+template <int DIR>
+__device__ __forceinline__ void dft16(float2 *a) {
+    constexpr float c1 = 0.923879532511286756128183189397f;  // cos(pi/8)
+    constexpr float s1 = 0.382683432365089771728459984030f;  // sin(pi/8)
+    constexpr float h = 0.707106781186547524400844362105f;   // sqrt(2)/2
+    constexpr float d = DIR < 0 ? -1.0f : 1.0f;
+    float2 t[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        t[n2][0] = a[n2];
+        t[n2][1] = a[4 + n2];
+        t[n2][2] = a[8 + n2];
+        t[n2][3] = a[12 + n2];
+        dft4<DIR>(t[n2]);
+    }
+    t[1][1] = cmul(t[1][1], make_float2(c1, d * s1));   // W16^1
+    t[1][2] = cmul(t[1][2], make_float2(h, d * h));     // W16^2
+    t[1][3] = cmul(t[1][3], make_float2(s1, d * c1));   // W16^3
+    t[2][1] = cmul(t[2][1], make_float2(h, d * h));     // W16^2
+    t[2][2] = DIR < 0 ? make_float2(t[2][2].y, -t[2][2].x) : make_float2(-t[2][2].y, t[2][2].x);  // W16^4 = DIR i
+    t[2][3] = cmul(t[2][3], make_float2(-h, d * h));    // W16^6
+    t[3][1] = cmul(t[3][1], make_float2(s1, d * c1));   // W16^3
+    t[3][2] = cmul(t[3][2], make_float2(-h, d * h));    // W16^6
+    t[3][3] = cmul(t[3][3], make_float2(-c1, -d * s1)); // W16^9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 u[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+        dft4<DIR>(u);
+        a[k1] = u[0];
+        a[k1 + 4] = u[1];
+        a[k1 + 8] = u[2];
+        a[k1 + 12] = u[3];
+    }
+}
+template <>
+__device__ __forceinline__ void dft<16, -1>(float2 *a) { dft16<-1>(a); }
+template <>
+__device__ __forceinline__ void dft<16, 1>(float2 *a) { dft16<1>(a); }
+
 // j mod Ns for the butterfly's twiddle exponent; Ns is a power of two in
 // every pass before the first radix-3 or radix-5 pass (fft_factor orders the
 // radices 8, 4, 2, 3, 5).
@@ -335,6 +376,63 @@ template <int DIR, int N, class Get, class Put>
 __device__ __forceinline__ void fft_row_c(float2 *buf, const float2 *tw, Get &get, Put &put) {
     static_assert(radix_plan_c(N).passes >= 2, "fft_row_c: at least two passes");
     fft_passes_c<DIR, N, 0, 1>(buf, tw, get, put);
+}
+
+// ------------------------------------------------ one-wave transforms
+// A transform of N values by ONE wave (a 64-thread workgroup), N / 64 values
+// per lane: the same Stockham passes, with every lane busy in every pass and
+// a block barrier that is a single wave's (no waiting on other waves).  LDS
+// holds N values plus one pad slot per 16 (index p at p + p/16), so the
+// stride-16 writes of a radix-16 pass spread over all banks.
+constexpr int fft_wave_lds(int n) { return n + n / 16; }
+__device__ __forceinline__ int fft_wave_pad(int p) { return p + (p >> 4); }
+
+template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, class Get, class Put>
+__device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get &get, Put &put) {
+    constexpr int nb = N / R;
+    static_assert(nb % 64 == 0, "fft_wave_pass: N / R must be a multiple of 64");
+    constexpr int per = nb / 64;
+    constexpr int twstep = N / (NS * R);
+    const int lane = threadIdx.x & 63;
+    float2 v[per][R];
+#pragma unroll
+    for (int b = 0; b < per; ++b) {
+        const int j = lane + 64 * b;
+        if constexpr (FIRST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = get(j + r * nb);
+        } else {
+            const int k = j & (NS - 1);
+            v[b][0] = buf[fft_wave_pad(j)];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[fft_wave_pad(j + r * nb)], twiddle<DIR>(tw, r * k * twstep));
+        }
+    }
+    if constexpr (!FIRST) __syncthreads();  // every read of buf precedes the writes below
+#pragma unroll
+    for (int b = 0; b < per; ++b) {
+        const int j = lane + 64 * b;
+        dft<R, DIR>(v[b]);
+        const int k = FIRST ? 0 : (j & (NS - 1));
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if constexpr (LAST)
+                put(base + q * NS, v[b][q]);
+            else
+                buf[fft_wave_pad(base + q * NS)] = v[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// 1024 points as radix 16, 16, 4 (16 values per lane); get(i) and put(i, X)
+// as fft_row's.  buf: fft_wave_lds(1024) values.
+template <int DIR, class Get, class Put>
+__device__ __forceinline__ void fft_wave_1024(float2 *buf, const float2 *tw, Get get, Put put) {
+    fft_wave_pass<16, DIR, true, false, 1024, 1>(buf, tw, get, put);
+    fft_wave_pass<16, DIR, false, false, 1024, 16>(buf, tw, get, put);
+    fft_wave_pass<4, DIR, false, true, 1024, 256>(buf, tw, get, put);
 }
 
 // Transform of one row of length d.n <= MAXN from get(i) (i in [0, n), each

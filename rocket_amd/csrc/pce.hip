@@ -281,6 +281,50 @@ __global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w,
     }
 }
 
+// pce_cols_kernel for h = 1024 with one wave per column (fft_wave_1024: 16
+// values per lane, radix 16, 16, 4): every lane works in every pass and the
+// barriers between passes are one wave's.  Same sums; the FP64 energy is
+// added in another order (one wave's lanes instead of four waves' partials).
+__global__ __launch_bounds__(64) void pce_cols_wave_kernel(const float2 *__restrict__ tw, int w, int nb,
+                                                           const float2 *__restrict__ Ball, int64_t pstride,
+                                                           PeakPart *__restrict__ partall) {
+    __shared__ float2 buf[fft_wave_lds(1024)];
+    const int q = blockIdx.y, j = blockIdx.x;
+    const float2 *col = Ball + (int64_t)q * pstride + (int64_t)j * 1024;
+    double e0 = 0.0, e1 = 0.0;
+    float b0 = -1.0f, b1 = -1.0f;
+    int i0 = 0x7fffffff, i1 = 0x7fffffff;
+    fft_wave_1024<1>(
+        buf, tw, [&](int i) { return col[i]; },
+        [&](int i, float2 z) {
+            const int idx = i * w + j;
+            e0 += (double)z.x * (double)z.x;
+            e1 += (double)z.y * (double)z.y;
+            argmax_merge(b0, i0, fabsf(z.x), idx);
+            argmax_merge(b1, i1, fabsf(z.y), idx);
+        });
+    e0 = wave_sum_d(e0);
+    e1 = wave_sum_d(e1);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float v2 = __shfl_xor(b0, off, 64);
+        int k2 = __shfl_xor(i0, off, 64);
+        argmax_merge(b0, i0, v2, k2);
+        v2 = __shfl_xor(b1, off, 64);
+        k2 = __shfl_xor(i1, off, 64);
+        argmax_merge(b1, i1, v2, k2);
+    }
+    const int t = threadIdx.x;
+    const int halves = 2 * q + 1 < nb ? 2 : 1;
+    if (t < halves) {
+        PeakPart p;
+        p.energy = t == 0 ? e0 : e1;
+        p.absmax = t == 0 ? b0 : b1;
+        p.index = t == 0 ? i0 : i1;
+        partall[(int64_t)(2 * q + t) * w + j] = p;
+    }
+}
+
 // Value c[i][j] of pair hf (0: Re, 1: Im of the couple's complex plane) as
 // the direct inverse DFT of column j of the row-pass output A (h x w,
 // row-major): sum_k A[k][j] exp(+2 pi i i k / h), summed in FP64 per thread
@@ -487,7 +531,9 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     PRNU_TRY(launch_transpose_c(A, B, h, w, couples, pstride, st));
     const dim3 cgrid(w, couples);
     const size_t csm = fft_row_smem(h);
-    if (h <= kFftSmallN)
+    if (h == 1024)
+        hipLaunchKernelGGL(pce_cols_wave_kernel, cgrid, dim3(64), 0, st, dh.tw, w, nb, B, pstride, part);
+    else if (h <= kFftSmallN)
         hipLaunchKernelGGL((pce_cols_kernel<kFftSmallN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, B, pstride,
                            part);
     else if (h <= kFftMidN)
