@@ -387,7 +387,9 @@ __device__ __forceinline__ void fft_row_c(float2 *buf, const float2 *tw, Get &ge
 constexpr int fft_wave_lds(int n) { return n + n / 16; }
 __device__ __forceinline__ int fft_wave_pad(int p) { return p + (p >> 4); }
 
-template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, class Get, class Put>
+// SYNC_FIRST: a block barrier between the first pass's reads (get) and its
+// writes, for callers whose get reads LDS that the waves' buffers overlay.
+template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, bool SYNC_FIRST = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get &get, Put &put) {
     constexpr int nb = N / R;
     static_assert(nb % 64 == 0, "fft_wave_pass: N / R must be a multiple of 64");
@@ -408,7 +410,7 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
             for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[fft_wave_pad(j + r * nb)], twiddle<DIR>(tw, r * k * twstep));
         }
     }
-    if constexpr (!FIRST) __syncthreads();  // every read of buf precedes the writes below
+    if constexpr (!FIRST || SYNC_FIRST) __syncthreads();  // every read precedes the writes below
 #pragma unroll
     for (int b = 0; b < per; ++b) {
         const int j = lane + 64 * b;
@@ -428,9 +430,9 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
 
 // 1024 points as radix 16, 16, 4 (16 values per lane); get(i) and put(i, X)
 // as fft_row's.  buf: fft_wave_lds(1024) values.
-template <int DIR, class Get, class Put>
+template <int DIR, bool SYNC_FIRST = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave_1024(float2 *buf, const float2 *tw, Get get, Put put) {
-    fft_wave_pass<16, DIR, true, false, 1024, 1>(buf, tw, get, put);
+    fft_wave_pass<16, DIR, true, false, 1024, 1, SYNC_FIRST>(buf, tw, get, put);
     fft_wave_pass<16, DIR, false, false, 1024, 16>(buf, tw, get, put);
     fft_wave_pass<4, DIR, false, true, 1024, 256>(buf, tw, get, put);
 }

@@ -281,6 +281,52 @@ __global__ __launch_bounds__(kFftThreads) void pce_cols_kernel(FftDesc d, int w,
     }
 }
 
+// pce_rows_mirror_kernel for w = 1024 with one wave per row: the two waves
+// form H_a, H_b of row u into LDS together, then wave 0 transforms row u and
+// wave 1 row -u at the same time (fft_wave_1024), their transform buffers
+// laid over H_a, H_b once the first pass has read them.  When u is its own
+// mirror, wave 1 transforms and stores nothing of its own (it still takes
+// part in every barrier).  grid (h/2 + 1, couples), 128 threads.
+template <int VARIANT>
+__global__ __launch_bounds__(128) void pce_rows_mirror_wave_kernel(const float2 *__restrict__ tw, int h, int nb,
+                                                                   PceBatch bt, float2 *__restrict__ Aall,
+                                                                   int64_t pstride) {
+    constexpr int w = 1024;
+    __shared__ float2 smem[2 * fft_wave_lds(w)];
+    float2 *ha = smem, *hb = smem + w;
+    const int u = blockIdx.x;
+    const int um = (h - u) % h;
+    const int q = blockIdx.y;
+    const float2 *Xa, *Ya, *Xb = nullptr, *Yb = nullptr;
+    pce_operands(bt, 2 * q, Xa, Ya);
+    const bool two = 2 * q + 1 < nb;
+    if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
+    for (int j = threadIdx.x; j < w; j += 128) {
+        ha[j] = herm_at<VARIANT>(Xa, Ya, h, w, u, j);
+        hb[j] = two ? herm_at<VARIANT>(Xb, Yb, h, w, u, j) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    float2 *buf = smem + wave * fft_wave_lds(w);
+    float2 *A = Aall + (int64_t)q * pstride;
+    const int row = wave == 0 ? u : um;
+    const bool store = wave == 0 || um != u;
+    fft_wave_1024<1, true>(
+        buf, tw,
+        [&](int j) {
+            if (wave == 0) {
+                const float2 a = ha[j], b = hb[j];
+                return make_float2(a.x - b.y, a.y + b.x);
+            }
+            const int mj = j == 0 ? 0 : w - j;
+            const float2 a = ha[mj], b = hb[mj];
+            return make_float2(a.x + b.y, b.x - a.y);
+        },
+        [&](int j, float2 f) {
+            if (store) A[(int64_t)row * w + j] = f;
+        });
+}
+
 // pce_cols_kernel for h = 1024 with one wave per column (fft_wave_1024: 16
 // values per lane, radix 16, 16, 4): every lane works in every pass and the
 // barriers between passes are one wave's.  Same sums; the FP64 energy is
@@ -510,7 +556,10 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     const size_t msm = 3 * sizeof(float2) * (size_t)w;
 #define PRNU_PCE_ROWS(V, M)                                                                                          \
     do {                                                                                                             \
-        if (mirror)                                                                                                  \
+        if (w == 1024)                                                                                               \
+            hipLaunchKernelGGL((pce_rows_mirror_wave_kernel<V>), mgrid, dim3(128), 0, st, dw.tw, h, nb, bt, A,       \
+                               pstride);                                                                             \
+        else if (mirror)                                                                                             \
             hipLaunchKernelGGL((pce_rows_mirror_kernel<V, M>), mgrid, dim3(kFftThreads), msm, st, dw, h, nb, bt, A, \
                                pstride);                                                                             \
         else                                                                                                         \
