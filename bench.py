@@ -270,6 +270,7 @@ def pce_pairs_leg(torch):
     res = {"pairs": int(len(pairs))}
     for variant, name in ((_lib.PCE_REFERENCE, "reference"), (_lib.PCE_TEXTBOOK, "textbook")):
         cmp = csi.PeakToCorrelationEnergy(PCE_H, PCE_W, variant, _ctx=f._ctx)
+        cmp.pairs(spec, pairs[:256])                               # warm-up: one batch, its workspace
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         cmp.pairs(spec, pairs)

@@ -634,9 +634,11 @@ int run_pce_pairs(Context *c, int variant, const void *const *d_tab, const int *
     PRNU_REQUIRE(npairs >= 0, "pce_pairs: negative pair count");
     PRNU_TRY(ensure_fft_plans(c));
     if (batch <= 0) {
-        // about 1 GiB of planes
+        // about 2 GiB of planes: 256 pairs of 1024x1024 (a sweep of 8 to 2048
+        // pairs there peaks at 256; smaller batches pay each launch's ramp and
+        // tail more often, larger ones lose the planes' L2 and cache reuse)
         const int64_t per = 8 * c->pixels();  // half of a couple's complex plane and of its transpose per pair
-        batch = (int)std::max<int64_t>(2, std::min<int64_t>(512, (int64_t(1) << 30) / per)) & ~1;
+        batch = (int)std::max<int64_t>(2, std::min<int64_t>(512, (int64_t(2) << 30) / per)) & ~1;
     }
     time_begin(c, st);
     for (int b0 = 0; b0 < npairs; b0 += batch) {
