@@ -799,6 +799,45 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
 }
 
 // grid (rows, B)
+// rows_fft_power_kernel, rows_ifft_scaled_kernel and
+// rows_fft_fwd_inplace_kernel for transposed columns of N = 1024 or 1536
+// points, one wave per column (fft_wave): every lane works in every pass and
+// the barriers are one wave's.  Same grids, 64 threads.
+template <int N>
+__global__ __launch_bounds__(64) void cols_fft_power_wave_kernel(const float2 *__restrict__ tw, float2 *B, float *P,
+                                                                 int64_t plane) {
+    __shared__ float2 buf[fft_wave_lds<N>()];
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * N;
+    float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * N;
+    fft_wave<-1, N>(
+        buf, tw, [&](int j) { return row[j]; },
+        [&](int j, float2 f) {
+            row[j] = f;
+            prow[j] = f.x * f.x + f.y * f.y;
+        });
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void cols_ifft_scaled_wave_kernel(const float2 *__restrict__ tw, float2 *B,
+                                                                   const float *S, int64_t plane, int R) {
+    __shared__ float2 buf[fft_wave_lds<N>()];
+    const int r = blockIdx.x;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * N;
+    const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * N;
+    const float *mrow = S + (int64_t)blockIdx.y * plane + (int64_t)(r == 0 ? 0 : R - r) * N;
+    fft_wave<1, N>(
+        buf, tw, [&](int j) { return cscale(row[j], 0.5f * (srow[j] + mrow[j == 0 ? 0 : N - j])); },
+        [&](int j, float2 f) { row[j] = f; });
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void cols_fft_fwd_wave_kernel(const float2 *__restrict__ tw, float2 *B,
+                                                               int64_t plane) {
+    __shared__ float2 buf[fft_wave_lds<N>()];
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * N;
+    fft_wave<-1, N>(buf, tw, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
+}
+
 template <int MAXN>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_fwd_inplace_kernel(FftDesc d, float2 *B, int64_t plane) {
     extern __shared__ float2 buf[];
@@ -1052,10 +1091,22 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
         // transposed layout: B = A^T (wh x h), column FFTs as rows of B; the
         // zero-padded box-min stencil is transpose invariant
         PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
-        PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(wh, nb), st, dh, e.B, e.P, n);
+        if (h == 1024)
+            hipLaunchKernelGGL(cols_fft_power_wave_kernel<1024>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.P, n);
+        else if (h == 1536)
+            hipLaunchKernelGGL(cols_fft_power_wave_kernel<1536>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.P, n);
+        else
+            PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(wh, nb), st, dh, e.B, e.P, n);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY(launch_varest(e.P, e.S, w, h, true, nb, e.var, st));
-        PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(wh, nb), st, dh, e.B, e.S, n, w);
+        if (h == 1024)
+            hipLaunchKernelGGL(cols_ifft_scaled_wave_kernel<1024>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.S, n,
+                               w);
+        else if (h == 1536)
+            hipLaunchKernelGGL(cols_ifft_scaled_wave_kernel<1536>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.S, n,
+                               w);
+        else
+            PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(wh, nb), st, dh, e.B, e.S, n, w);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY(launch_transpose_c(e.B, e.A, wh, h, nb, n, st));
     }
@@ -1083,7 +1134,12 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h);
     PRNU_CHECK_LAUNCH();
     PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
-    PRNU_LAUNCH_ROW(rows_fft_fwd_inplace_kernel, h, dim3(wh, nb), st, dh, e.B, n);
+    if (h == 1024)
+        hipLaunchKernelGGL(cols_fft_fwd_wave_kernel<1024>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, n);
+    else if (h == 1536)
+        hipLaunchKernelGGL(cols_fft_fwd_wave_kernel<1536>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, n);
+    else
+        PRNU_LAUNCH_ROW(rows_fft_fwd_inplace_kernel, h, dim3(wh, nb), st, dh, e.B, n);
     PRNU_CHECK_LAUNCH();
     PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st, out_stride));
     return PRNU_OK;

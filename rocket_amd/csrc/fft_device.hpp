@@ -382,14 +382,29 @@ __device__ __forceinline__ void fft_row_c(float2 *buf, const float2 *tw, Get &ge
 // A transform of N values by ONE wave (a 64-thread workgroup), N / 64 values
 // per lane: the same Stockham passes, with every lane busy in every pass and
 // a block barrier that is a single wave's (no waiting on other waves).  LDS
-// holds N values plus one pad slot per 16 (index p at p + p/16), so the
-// stride-16 writes of a radix-16 pass spread over all banks.
-constexpr int fft_wave_lds(int n) { return n + n / 16; }
-__device__ __forceinline__ int fft_wave_pad(int p) { return p + (p >> 4); }
+// holds N values plus one pad slot per 2^SH (index p at p + p/2^SH): per 16
+// for the radix-16 plans, per 8 for the radix-8 ones, so the strided writes
+// of the first passes spread over all banks.
+template <int N>
+struct FftWavePlan;
+template <>
+struct FftWavePlan<1024> {  // radix 16, 16, 4
+    static constexpr int kPadShift = 4;
+};
+template <>
+struct FftWavePlan<1536> {  // radix 8, 8, 8, 3
+    static constexpr int kPadShift = 3;
+};
+template <int N>
+constexpr int fft_wave_lds() { return N + (N >> FftWavePlan<N>::kPadShift); }
+constexpr int fft_wave_lds(int n) { return n + n / 16; }  // the 1024-point plan's
+template <int SH = 4>
+__device__ __forceinline__ int fft_wave_pad(int p) { return p + (p >> SH); }
 
 // SYNC_FIRST: a block barrier between the first pass's reads (get) and its
 // writes, for callers whose get reads LDS that the waves' buffers overlay.
-template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, bool SYNC_FIRST = false, class Get, class Put>
+template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, bool SYNC_FIRST = false, int SH = 4, class Get,
+          class Put>
 __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get &get, Put &put) {
     constexpr int nb = N / R;
     static_assert(nb % 64 == 0, "fft_wave_pass: N / R must be a multiple of 64");
@@ -405,9 +420,10 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
             for (int r = 0; r < R; ++r) v[b][r] = get(j + r * nb);
         } else {
             const int k = j & (NS - 1);
-            v[b][0] = buf[fft_wave_pad(j)];
+            v[b][0] = buf[fft_wave_pad<SH>(j)];
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[b][r] = cmul(buf[fft_wave_pad(j + r * nb)], twiddle<DIR>(tw, r * k * twstep));
+            for (int r = 1; r < R; ++r)
+                v[b][r] = cmul(buf[fft_wave_pad<SH>(j + r * nb)], twiddle<DIR>(tw, r * k * twstep));
         }
     }
     if constexpr (!FIRST || SYNC_FIRST) __syncthreads();  // every read precedes the writes below
@@ -422,7 +438,7 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
             if constexpr (LAST)
                 put(base + q * NS, v[b][q]);
             else
-                buf[fft_wave_pad(base + q * NS)] = v[b][q];
+                buf[fft_wave_pad<SH>(base + q * NS)] = v[b][q];
         }
     }
     __syncthreads();
@@ -435,6 +451,25 @@ __device__ __forceinline__ void fft_wave_1024(float2 *buf, const float2 *tw, Get
     fft_wave_pass<16, DIR, true, false, 1024, 1, SYNC_FIRST>(buf, tw, get, put);
     fft_wave_pass<16, DIR, false, false, 1024, 16>(buf, tw, get, put);
     fft_wave_pass<4, DIR, false, true, 1024, 256>(buf, tw, get, put);
+}
+
+// 1536 points as radix 8, 8, 8, 3 (24 values per lane).  buf:
+// fft_wave_lds<1536>() values.
+template <int DIR, class Get, class Put>
+__device__ __forceinline__ void fft_wave_1536(float2 *buf, const float2 *tw, Get get, Put put) {
+    fft_wave_pass<8, DIR, true, false, 1536, 1, false, 3>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, false, 1536, 8, false, 3>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, false, 1536, 64, false, 3>(buf, tw, get, put);
+    fft_wave_pass<3, DIR, false, true, 1536, 512, false, 3>(buf, tw, get, put);
+}
+
+// The one-wave transform of a compile-time length (1024 or 1536).
+template <int DIR, int N, class Get, class Put>
+__device__ __forceinline__ void fft_wave(float2 *buf, const float2 *tw, Get get, Put put) {
+    if constexpr (N == 1024)
+        fft_wave_1024<DIR>(buf, tw, get, put);
+    else
+        fft_wave_1536<DIR>(buf, tw, get, put);
 }
 
 // Transform of one row of length d.n <= MAXN from get(i) (i in [0, n), each
