@@ -705,35 +705,40 @@ int launch_varest(const float *P, float *S, int rows, int cols, bool mirror_rows
 // imaginary parts of bins 0 and w/2 dropped, as a real transform does);
 // out = z / (w*h), row 2r the real, row 2r+1 the imaginary part
 // (normalizeToReal).  grid (ceil(h/2), B)
-template <int MAXN>
-__global__ __launch_bounds__(kFftThreads) void rows_c2r_kernel(FftDesc d, const float2 *A, int64_t plane, int h,
-                                                               float *out_all, int64_t out_stride, float inv_n) {
+// BLOCKN = 2048: rows of 2048 by the two-wave transform (fft_block_2048, 128
+// threads, LDS fft_wave_lds<2048>() values); 0: fft_row on kFftThreads.
+template <int MAXN, int BLOCKN = 0>
+__global__ __launch_bounds__(BLOCKN ? 128 : kFftThreads) void rows_c2r_kernel(FftDesc d, const float2 *A,
+                                                                              int64_t plane, int h, float *out_all,
+                                                                              int64_t out_stride, float inv_n) {
     extern __shared__ float2 buf[];
     const int i0 = 2 * blockIdx.x, i1 = i0 + 1;
     const bool two = i1 < h;
     const int w = d.n, wh = w / 2 + 1;
     const float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * wh;
     float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i0 * w;
-    fft_row<1, MAXN>(
-        buf, d,
-        [&](int k) {
-            const bool up = k > w / 2;
-            const int kk = up ? w - k : k;
-            float2 x = a0[kk], y = two ? a0[wh + kk] : make_float2(0.f, 0.f);
-            if (kk == 0 || 2 * kk == w) {
-                x.y = 0.f;
-                y.y = 0.f;
-            }
-            if (up) {
-                x.y = -x.y;
-                y.y = -y.y;
-            }
-            return make_float2(x.x - y.y, x.y + y.x);
-        },
-        [&](int j, float2 z) {
-            out[j] = z.x * inv_n;
-            if (two) out[w + j] = z.y * inv_n;
-        });
+    auto get = [&](int k) {
+        const bool up = k > w / 2;
+        const int kk = up ? w - k : k;
+        float2 x = a0[kk], y = two ? a0[wh + kk] : make_float2(0.f, 0.f);
+        if (kk == 0 || 2 * kk == w) {
+            x.y = 0.f;
+            y.y = 0.f;
+        }
+        if (up) {
+            x.y = -x.y;
+            y.y = -y.y;
+        }
+        return make_float2(x.x - y.y, x.y + y.x);
+    };
+    auto put = [&](int j, float2 z) {
+        out[j] = z.x * inv_n;
+        if (two) out[w + j] = z.y * inv_n;
+    };
+    if constexpr (BLOCKN == 2048)
+        fft_block_2048<1>(buf, d.tw, get, put);
+    else
+        fft_row<1, MAXN>(buf, d, get, put);
 }
 
 // ---------------------------------------------------------------- spectrum
@@ -1078,6 +1083,9 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
     const int pairs = (h + 1) / 2;
+    // (the two-wave 2048-point form measured slower here, 331 -> 365 us per
+    // 32 frames of 2048x1536: four loads per value and FP64 sums at 4 waves
+    // per SIMD; it pays in rows_c2r_kernel)
     PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(pairs, nb), st, dw, img, img_stride, h, e.A, n, e.rowsq,
                     zm ? e.cz : nullptr, zm ? e.rz : nullptr);
     PRNU_CHECK_LAUNCH();
@@ -1110,8 +1118,13 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
         PRNU_CHECK_LAUNCH();
         PRNU_TRY(launch_transpose_c(e.B, e.A, wh, h, nb, n, st));
     }
-    PRNU_LAUNCH_ROW(rows_c2r_kernel, w, dim3(pairs, nb), st, dw, e.A, n, h, out, out_stride,
-                    1.0f / (float)((double)w * h));
+    if (w == 2048)
+        hipLaunchKernelGGL((rows_c2r_kernel<kFftSmallN, 2048>), dim3(pairs, nb), dim3(128),
+                           sizeof(float2) * fft_wave_lds<2048>(), st, dw, e.A, n, h, out, out_stride,
+                           1.0f / (float)((double)w * h));
+    else
+        PRNU_LAUNCH_ROW(rows_c2r_kernel, w, dim3(pairs, nb), st, dw, e.A, n, h, out, out_stride,
+                        1.0f / (float)((double)w * h));
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }

@@ -395,6 +395,10 @@ template <>
 struct FftWavePlan<1536> {  // radix 8, 8, 8, 3
     static constexpr int kPadShift = 3;
 };
+template <>
+struct FftWavePlan<2048> {  // radix 16, 16, 8 on two waves
+    static constexpr int kPadShift = 4;
+};
 template <int N>
 constexpr int fft_wave_lds() { return N + (N >> FftWavePlan<N>::kPadShift); }
 constexpr int fft_wave_lds(int n) { return n + n / 16; }  // the 1024-point plan's
@@ -403,18 +407,20 @@ __device__ __forceinline__ int fft_wave_pad(int p) { return p + (p >> SH); }
 
 // SYNC_FIRST: a block barrier between the first pass's reads (get) and its
 // writes, for callers whose get reads LDS that the waves' buffers overlay.
-template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, bool SYNC_FIRST = false, int SH = 4, class Get,
-          class Put>
+// T: the threads of the transform (64: one wave; 128: two waves, whose block
+// barrier then waits for the other wave).
+template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, bool SYNC_FIRST = false, int SH = 4, int T = 64,
+          class Get, class Put>
 __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get &get, Put &put) {
     constexpr int nb = N / R;
-    static_assert(nb % 64 == 0, "fft_wave_pass: N / R must be a multiple of 64");
-    constexpr int per = nb / 64;
+    static_assert(nb % T == 0, "fft_wave_pass: N / R must be a multiple of the thread count");
+    constexpr int per = nb / T;
     constexpr int twstep = N / (NS * R);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & (T - 1);
     float2 v[per][R];
 #pragma unroll
     for (int b = 0; b < per; ++b) {
-        const int j = lane + 64 * b;
+        const int j = lane + T * b;
         if constexpr (FIRST) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = get(j + r * nb);
@@ -429,7 +435,7 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
     if constexpr (!FIRST || SYNC_FIRST) __syncthreads();  // every read precedes the writes below
 #pragma unroll
     for (int b = 0; b < per; ++b) {
-        const int j = lane + 64 * b;
+        const int j = lane + T * b;
         dft<R, DIR>(v[b]);
         const int k = FIRST ? 0 : (j & (NS - 1));
         const int base = (j - k) * R + k;
@@ -461,6 +467,15 @@ __device__ __forceinline__ void fft_wave_1536(float2 *buf, const float2 *tw, Get
     fft_wave_pass<8, DIR, false, false, 1536, 8, false, 3>(buf, tw, get, put);
     fft_wave_pass<8, DIR, false, false, 1536, 64, false, 3>(buf, tw, get, put);
     fft_wave_pass<3, DIR, false, true, 1536, 512, false, 3>(buf, tw, get, put);
+}
+
+// 2048 points as radix 16, 16, 8 on two waves (128 threads, 16 values per
+// lane).  buf: fft_wave_lds<2048>() values.
+template <int DIR, class Get, class Put>
+__device__ __forceinline__ void fft_block_2048(float2 *buf, const float2 *tw, Get get, Put put) {
+    fft_wave_pass<16, DIR, true, false, 2048, 1, false, 4, 128>(buf, tw, get, put);
+    fft_wave_pass<16, DIR, false, false, 2048, 16, false, 4, 128>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, true, 2048, 256, false, 4, 128>(buf, tw, get, put);
 }
 
 // The one-wave transform of a compile-time length (1024 or 1536).
