@@ -628,7 +628,7 @@ template <bool MIRROR_ROWS>
 __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, float *S_all, int rows, int cols,
                                                            const double *var_all, int ntx, int nty, int ntiles) {
     constexpr int LW = kVX + 2 * kVB, LH = kVY + 2 * kVB;
-    __shared__ float T[LH][LW];
+    __shared__ __align__(16) float T[LH][LW];
     const int per = (ntiles + 7) / 8;
     const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (t >= ntiles) return;
@@ -637,16 +637,38 @@ __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, f
     const int64_t plane = (int64_t)rows * cols;
     const float *P = P_all + (int64_t)b * plane;
     float *S = S_all + (int64_t)b * plane;
-    // thread -> (LDS row, group of 4 columns): 18 groups per row
+    // thread -> (LDS row, group of 4 columns): 18 groups per row.  In the
+    // transposed layout (MIRROR_ROWS) a group inside the grid is 4
+    // consecutive values of one source row -- forward (one 16-byte load when
+    // cols is a multiple of 4: gx0 is) or backward in the mirrored half --
+    // so only groups at the grid's edges take the per-value path.
+    const bool vec4 = MIRROR_ROWS && (cols & 3) == 0;
     for (int q = threadIdx.x; q < LH * (LW / 4); q += 256) {
         const int ly = q / (LW / 4), lx = (q - ly * (LW / 4)) * 4;
         const int gy = r0 + ly - kVB;
+        const int gx0 = c0 + lx - kVB;
         const bool row_in = gy >= 0 && gy < rows;
+        float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (vec4 && row_in && gx0 >= 0 && gx0 + 3 < cols) {
+            if (gy <= rows / 2) {
+                val = *reinterpret_cast<const float4 *>(P + (int64_t)gy * cols + gx0);
+            } else {
+                const float *pr = P + (int64_t)(rows - gy) * cols;
+                val.x = pr[gx0 == 0 ? 0 : cols - gx0];
+                val.y = pr[cols - gx0 - 1];
+                val.z = pr[cols - gx0 - 2];
+                val.w = pr[cols - gx0 - 3];
+            }
+        } else if (row_in) {
+            float e[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int gx = c0 + lx + k - kVB;
-            T[ly][lx + k] = (row_in && gx >= 0 && gx < cols) ? power_full<MIRROR_ROWS>(P, gy, gx, rows, cols) : 0.f;
+            for (int k = 0; k < 4; ++k) {
+                const int gx = gx0 + k;
+                e[k] = (gx >= 0 && gx < cols) ? power_full<MIRROR_ROWS>(P, gy, gx, rows, cols) : 0.f;
+            }
+            val = make_float4(e[0], e[1], e[2], e[3]);
         }
+        *reinterpret_cast<float4 *>(&T[ly][lx]) = val;
     }
     __syncthreads();
     const int c = threadIdx.x & 63, oy0 = (threadIdx.x >> 6) * kVRows;
