@@ -685,15 +685,17 @@ __global__ __launch_bounds__(256) void varest_scale_kernel(const float *P_all, f
         h9[i] = h7[i] + row[-4] + row[4];
         const int j = i - 8;  // the last row output j needs
         if (j >= 0) {
-            float s3 = 0.f, s5 = 0.f, s7 = 0.f, s9 = 0.f;
+            // each sum starts from its first term: the same value as 0 + first
+            // term, since the terms are sums of powers (never -0), one add fewer
+            float s3 = h3[j + 3], s5 = h5[j + 2], s7 = h7[j + 1], s9 = h9[j];
 #pragma unroll
-            for (int d = -1; d <= 1; ++d) s3 += h3[j + 4 + d];
+            for (int d = 0; d <= 1; ++d) s3 += h3[j + 4 + d];
 #pragma unroll
-            for (int d = -2; d <= 2; ++d) s5 += h5[j + 4 + d];
+            for (int d = -1; d <= 2; ++d) s5 += h5[j + 4 + d];
 #pragma unroll
-            for (int d = -3; d <= 3; ++d) s7 += h7[j + 4 + d];
+            for (int d = -2; d <= 3; ++d) s7 += h7[j + 4 + d];
 #pragma unroll
-            for (int d = -4; d <= 4; ++d) s9 += h9[j + 4 + d];
+            for (int d = -3; d <= 4; ++d) s9 += h9[j + 4 + d];
             // means by the rounded reciprocals of f^2, the quotient by the
             // hardware reciprocal (each within an ulp or so)
             float best = s3 * (1.f / 9.f);
