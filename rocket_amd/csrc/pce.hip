@@ -375,21 +375,27 @@ __global__ __launch_bounds__(64) void pce_cols_wave_kernel(const float2 *__restr
 // the direct inverse DFT of column j of the row-pass output A (h x w,
 // row-major): sum_k A[k][j] exp(+2 pi i i k / h), summed in FP64 per thread
 // over k = t, t + 256, ... and then over the block (fixed order).  `tw` is
-// the length-h table exp(-2 pi i m / h); ncells cells (i[c], j[c]) at once.
+// the length-h table exp(-2 pi i m / h); ncells cells (i, j[c]) of one row i
+// at once, which share the twiddle of each k: its index (i k) mod h is
+// stepped by (256 i) mod h from one k to the next.
 constexpr int kMaxCells = 11;
 
 __device__ void dft_cells(const float2 *__restrict__ A, int h, int w, const float2 *__restrict__ tw, int hf,
-                          const int *ci, const int *cj, int ncells, double *out, double (*sh)[4]) {
+                          int crow, const int *cj, int ncells, double *out, double (*sh)[4]) {
     double acc[kMaxCells];
 #pragma unroll
     for (int c = 0; c < kMaxCells; ++c) acc[c] = 0.0;
+    const int step = (int)(((int64_t)crow * 256) % h);
+    int ti = (int)(((int64_t)crow * threadIdx.x) % h);
     for (int k = threadIdx.x; k < h; k += 256) {
         const float2 *row = A + (int64_t)k * w;
+        const float2 t = tw[ti];  // exp(-i theta); the inverse uses exp(+i theta)
+        ti += step;
+        if (ti >= h) ti -= h;
 #pragma unroll
         for (int c = 0; c < kMaxCells; ++c) {
             if (c < ncells) {
                 const float2 a = row[cj[c]];
-                const float2 t = tw[(int)(((int64_t)ci[c] * k) % h)];  // exp(-i theta); the inverse uses exp(+i theta)
                 // (a.x + i a.y)(t.x - i t.y): Re = a.x t.x + a.y t.y, Im = a.y t.x - a.x t.y
                 acc[c] += hf ? ((double)a.y * t.x - (double)a.x * t.y) : ((double)a.x * t.x + (double)a.y * t.y);
             }
@@ -489,7 +495,7 @@ __global__ __launch_bounds__(256) void pce_window_kernel(const PeakPart *__restr
             cj[t] = wrap ? ((c0 + t) % w + w) % w : c0 + t;
         }
         __syncthreads();
-        dft_cells(A, h, w, tw, hf, ci, cj, nc, cells, sh_cell);
+        dft_cells(A, h, w, tw, hf, ci[0], cj, nc, cells, sh_cell);  // one window row: every ci[c] is ci[0]
         if (t == 0) {
             double win = 0.0;
             for (int c = 0; c < nc; ++c) {
@@ -504,7 +510,7 @@ __global__ __launch_bounds__(256) void pce_window_kernel(const PeakPart *__restr
             cj[0] = w - 1;
         }
         __syncthreads();
-        dft_cells(A, h, w, tw, hf, ci, cj, 1, cells, sh_cell);
+        dft_cells(A, h, w, tw, hf, ci[0], cj, 1, cells, sh_cell);
         if (t == 0) slots[11] = cells[0];
     }
 }
