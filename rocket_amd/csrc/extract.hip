@@ -530,7 +530,8 @@ __global__ __launch_bounds__(256) void zm_apply_kernel(float *img_all, int64_t i
 // Zero-mean (cz non-null): x - cz[row parity][j] - rz[j & 1][row] in FP32,
 // the column terms cz = colmean - g and row means rz of zm_lattice_kernel /
 // zm_reduce_kernel (within an ulp of the FP64 closed form zm_value).
-template <int MAXN>
+// PAD2048: rows of 2048 by fft_block_2048_r8 (LDS kFft2048PadLds values).
+template <int MAXN, bool PAD2048 = false>
 __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img_all,
                                                                       int64_t img_stride, int h, float2 *A_all,
                                                                       int64_t plane, double *rowsq_all,
@@ -551,9 +552,7 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
     double sq0 = 0.0, sq1 = 0.0;
     const float *r0 = img + (int64_t)i0 * w;
     const float *r1 = img + (int64_t)(two ? i1 : i0) * w;
-    fft_row<-1, MAXN>(
-        buf, d,
-        [&](int j) {
+    auto get = [&](int j) {
             float va = r0[j], vb = two ? r1[j] : 0.f;
             if (zm) {
                 const bool odd = j & 1;
@@ -563,8 +562,12 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
             sq0 += (double)va * (double)va;
             sq1 += (double)vb * (double)vb;
             return make_float2(va, vb);
-        },
-        [&](int j, float2 f) { buf[j] = f; });  // the last pass has read buf before it writes
+        };
+    auto put = [&](int j, float2 f) { buf[j] = f; };  // the last pass has read buf before it writes
+    if constexpr (PAD2048)
+        fft_block_2048_r8<-1>(buf, d.tw, get, put);
+    else
+        fft_row<-1, MAXN>(buf, d, get, put);
     float2 *a0 = A_all + (int64_t)b * plane + (int64_t)i0 * wh;
     for (int k = threadIdx.x; k < wh; k += kFftThreads) {
         const float2 zk = buf[k], zm_ = buf[k == 0 ? 0 : w - k];
@@ -769,7 +772,8 @@ __global__ __launch_bounds__(BLOCKN ? 128 : kFftThreads) void rows_c2r_kernel(Ff
 // Forward FFTs (length w) of real rows 2r and 2r+1 as one complex transform
 // (z = a + i b, split as in wiener_rows_fwd_kernel); the w/2+1 bins of each
 // row go to A (h x wh, rows wh apart).  grid (ceil(h/2), B)
-template <int MAXN>
+// PAD2048 as wiener_rows_fwd_kernel's.
+template <int MAXN, bool PAD2048 = false>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img_all,
                                                                     int64_t img_stride, float2 *A, int64_t plane,
                                                                     int h) {
@@ -779,9 +783,12 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, c
     const int w = d.n, wh = w / 2 + 1;
     const float *r0 = img_all + (int64_t)blockIdx.y * img_stride + (int64_t)i0 * w;
     const float *r1 = two ? r0 + w : r0;
-    fft_row<-1, MAXN>(
-        buf, d, [&](int j) { return make_float2(r0[j], two ? r1[j] : 0.f); },
-        [&](int j, float2 f) { buf[j] = f; });  // the last pass has read buf before it writes
+    auto get = [&](int j) { return make_float2(r0[j], two ? r1[j] : 0.f); };
+    auto put = [&](int j, float2 f) { buf[j] = f; };  // the last pass has read buf before it writes
+    if constexpr (PAD2048)
+        fft_block_2048_r8<-1>(buf, d.tw, get, put);
+    else
+        fft_row<-1, MAXN>(buf, d, get, put);
     float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * wh;
     for (int k = threadIdx.x; k < wh; k += kFftThreads) {
         const float2 zk = buf[k], zm_ = buf[k == 0 ? 0 : w - k];
@@ -1110,8 +1117,13 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     // (the two-wave 2048-point form measured slower here, 331 -> 365 us per
     // 32 frames of 2048x1536: four loads per value and FP64 sums at 4 waves
     // per SIMD; it pays in rows_c2r_kernel)
-    PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(pairs, nb), st, dw, img, img_stride, h, e.A, n, e.rowsq,
-                    zm ? e.cz : nullptr, zm ? e.rz : nullptr);
+    if (w == 2048)
+        hipLaunchKernelGGL((wiener_rows_fwd_kernel<kFftSmallN, true>), dim3(pairs, nb), dim3(kFftThreads),
+                           sizeof(float2) * kFft2048PadLds, st, dw, img, img_stride, h, e.A, n, e.rowsq,
+                           zm ? e.cz : nullptr, zm ? e.rz : nullptr);
+    else
+        PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(pairs, nb), st, dw, img, img_stride, h, e.A, n, e.rowsq,
+                        zm ? e.cz : nullptr, zm ? e.rz : nullptr);
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
     PRNU_CHECK_LAUNCH();
@@ -1168,7 +1180,11 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
         return PRNU_OK;
     }
     const int64_t n = c->pixels();
-    PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h);
+    if (w == 2048)
+        hipLaunchKernelGGL((rows_fft_half_kernel<kFftSmallN, true>), dim3((h + 1) / 2, nb), dim3(kFftThreads),
+                           sizeof(float2) * kFft2048PadLds, st, dw, img, img_stride, e.A, n, h);
+    else
+        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h);
     PRNU_CHECK_LAUNCH();
     PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
     if (h == 1024)

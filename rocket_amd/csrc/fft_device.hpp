@@ -478,6 +478,18 @@ __device__ __forceinline__ void fft_block_2048(float2 *buf, const float2 *tw, Ge
     fft_wave_pass<8, DIR, false, true, 2048, 256, false, 4, 128>(buf, tw, get, put);
 }
 
+// 2048 points as radix 8, 8, 8, 4 on four waves (256 threads, 8 values per
+// thread), LDS padded one slot per 8: the plan of fft_row_c<2048> with a
+// tenth of its bank conflicts.  buf: 2048 + 256 values.
+constexpr int kFft2048PadLds = 2048 + 256;
+template <int DIR, class Get, class Put>
+__device__ __forceinline__ void fft_block_2048_r8(float2 *buf, const float2 *tw, Get get, Put put) {
+    fft_wave_pass<8, DIR, true, false, 2048, 1, false, 3, 256>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, false, 2048, 8, false, 3, 256>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, false, 2048, 64, false, 3, 256>(buf, tw, get, put);
+    fft_wave_pass<4, DIR, false, true, 2048, 512, false, 3, 256>(buf, tw, get, put);
+}
+
 // The one-wave transform of a compile-time length (1024 or 1536).
 template <int DIR, int N, class Get, class Put>
 __device__ __forceinline__ void fft_wave(float2 *buf, const float2 *tw, Get get, Put put) {

@@ -87,7 +87,7 @@ def test_wiener_constant_closed_form():
     f.cleanup()
 
 
-@pytest.mark.parametrize("h,w", [(48, 64), (45, 60), (80, 96), (1024, 1024)])
+@pytest.mark.parametrize("h,w", [(48, 64), (45, 60), (80, 96), (1024, 1024), (1536, 2048), (1024, 2048)])
 def test_spectral(h, w, rng):
     x = rng.normal(size=(h, w)).astype(np.float32)
     f = csi.PRNUFilter(h, w, apply_spectral_filter=True)
