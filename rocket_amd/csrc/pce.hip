@@ -134,6 +134,34 @@ __device__ __forceinline__ float2 herm_at(const float2 *X, const float2 *Y, int 
     return make_float2(0.5f * (c.x + m.x), 0.5f * (c.y - m.y));
 }
 
+// H(u, j) and, for 1 <= j < w/2, H(u, w - j) -- herm_at's values bit for bit
+// -- from the four spectrum values X, Y at (u, j) and (um, j), um = (h - u)
+// mod h: every cross-spectrum term either needs reads at (u, j) or (um, j).
+// herm_at reads each of them twice over the row (once for j, once for its
+// mirror column).
+template <int VARIANT>
+__device__ __forceinline__ void herm_pair(const float2 *X, const float2 *Y, int w, int u, int um, int j, float2 &hj,
+                                          float2 &hm) {
+    const int wh = w / 2 + 1;
+    const float2 x0 = X[(int64_t)u * wh + j], y0 = Y[(int64_t)u * wh + j];
+    const float2 x1 = X[(int64_t)um * wh + j], y1 = Y[(int64_t)um * wh + j];
+    if (VARIANT != PRNU_PCE_REFERENCE) {
+        hj = cmul(cconj(x0), y0);  // cross_at(u, j), j < wh
+        hm = cmul(x1, cconj(y1));  // cross_at(u, w - j): row um, column j
+        return;
+    }
+    // reference: c = cross_at(u, j); m = cross_at(um, (w - j) mod w), which is
+    // row um's column j (mirrored from w - j) unless j is 0 or w/2
+    const float2 c = cmul(cconj(x0), y0);
+    const float2 m = (j == 0 || 2 * j == w) ? cmul(cconj(x1), y1) : cmul(x1, cconj(y1));
+    hj = make_float2(0.5f * (c.x + m.x), 0.5f * (c.y - m.y));
+    // the mirror column w - j: c' = cross_at(u, w - j) = x0 conj(y0) (same row
+    // in the reference), m' = cross_at(um, j) = conj(x1) y1
+    const float2 c2 = cmul(x0, cconj(y0));
+    const float2 m2 = cmul(cconj(x1), y1);
+    hm = make_float2(0.5f * (c2.x + m2.x), 0.5f * (c2.y - m2.y));
+}
+
 // Row i of Z = H_a + i H_b for couple blockIdx.y (pairs 2q, 2q+1 of the
 // batch; the second may be absent), inverse row FFT: the first pass reads
 // the Hermitian parts straight from the half spectra, the last pass writes
@@ -301,9 +329,16 @@ __global__ __launch_bounds__(128) void pce_rows_mirror_wave_kernel(const float2 
     pce_operands(bt, 2 * q, Xa, Ya);
     const bool two = 2 * q + 1 < nb;
     if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
-    for (int j = threadIdx.x; j < w; j += 128) {
-        ha[j] = herm_at<VARIANT>(Xa, Ya, h, w, u, j);
-        hb[j] = two ? herm_at<VARIANT>(Xb, Yb, h, w, u, j) : make_float2(0.f, 0.f);
+    for (int j = threadIdx.x; j <= w / 2; j += 128) {  // each spectrum value read once
+        float2 a, am, b = make_float2(0.f, 0.f), bm = make_float2(0.f, 0.f);
+        herm_pair<VARIANT>(Xa, Ya, w, u, um, j, a, am);
+        if (two) herm_pair<VARIANT>(Xb, Yb, w, u, um, j, b, bm);
+        ha[j] = a;
+        hb[j] = b;
+        if (j > 0 && 2 * j < w) {
+            ha[w - j] = am;
+            hb[w - j] = bm;
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6;
