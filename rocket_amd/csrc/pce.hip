@@ -223,9 +223,16 @@ __global__ __launch_bounds__(kFftThreads) void pce_rows_mirror_kernel(FftDesc d,
     pce_operands(bt, 2 * q, Xa, Ya);
     const bool two = 2 * q + 1 < nb;
     if (two) pce_operands(bt, 2 * q + 1, Xb, Yb);
-    for (int j = threadIdx.x; j < w; j += kFftThreads) {
-        ha[j] = herm_at<VARIANT>(Xa, Ya, h, w, u, j);
-        hb[j] = two ? herm_at<VARIANT>(Xb, Yb, h, w, u, j) : make_float2(0.f, 0.f);
+    for (int j = threadIdx.x; j <= w / 2; j += kFftThreads) {  // each spectrum value read once
+        float2 a, am, b = make_float2(0.f, 0.f), bm = make_float2(0.f, 0.f);
+        herm_pair<VARIANT>(Xa, Ya, w, u, um, j, a, am);
+        if (two) herm_pair<VARIANT>(Xb, Yb, w, u, um, j, b, bm);
+        ha[j] = a;
+        hb[j] = b;
+        if (j > 0 && 2 * j < w) {
+            ha[w - j] = am;
+            hb[w - j] = bm;
+        }
     }
     __syncthreads();
     float2 *A = Aall + (int64_t)q * pstride;
