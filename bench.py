@@ -686,7 +686,7 @@ def extraction_bench(torch, batch=32, shape=None, only_mode=None):
             f = csi.PRNUFilter(h, w, apply_spectral_filter=(mode == _lib.MODE_PCE))
             out = torch.empty((batch, f.get_output_size() // 4), device="cuda")
             f.apply_gpu_batch(imgs, out, _lib.INPUT_RGB8)            # warm-up, sizes the workspace
-            reps = 3
+            reps = 10                                                 # steady state: ~18 ms at 2048x1536
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             for _ in range(reps):
