@@ -413,6 +413,67 @@ __global__ __launch_bounds__(64) void pce_cols_wave_kernel(const float2 *__restr
     }
 }
 
+// pce_cols_wave_kernel for h = 1024 reading the row-major row-pass output A
+// itself, so no transpose precedes it: a workgroup of 8 waves stages a strip
+// of 8 adjacent columns into LDS (every load a 64-byte piece of a row, each
+// column in its own padded buffer), then wave c transforms column c there.
+// Workgroup g takes strip (g mod 8) * per + g / 8, so that neighbouring strips
+// (the two halves of a 128-byte line) run on one XCD.  grid (8 per, couples).
+constexpr int kStripBuf = 1092;  // fft_wave_lds(1024) = 1088, +4 spreads the staging writes over banks
+__global__ __launch_bounds__(512) void pce_cols_strip_kernel(const float2 *__restrict__ tw, int w, int nb,
+                                                             const float2 *__restrict__ Aall, int64_t pstride,
+                                                             PeakPart *__restrict__ partall, int per) {
+    __shared__ __attribute__((aligned(16))) float2 lds[8 * kStripBuf];
+    const int g = blockIdx.x;
+    const int strip = (g & 7) * per + (g >> 3);
+    if (strip >= w / 8) return;
+    const int q = blockIdx.y;
+    const int j0 = strip * 8;
+    const float2 *A = Aall + (int64_t)q * pstride;
+    const int t = (int)threadIdx.x, c = t & 7, r = t >> 3;
+#pragma unroll 4
+    for (int k = 0; k < 1024 / 64; ++k) {
+        const int i = r + 64 * k;
+        lds[c * kStripBuf + fft_wave_pad<4>(i)] = A[(int64_t)i * w + j0 + c];
+    }
+    __syncthreads();
+    const int wave = t >> 6;
+    float2 *buf = lds + wave * kStripBuf;
+    const int j = j0 + wave;
+    double e0 = 0.0, e1 = 0.0;
+    float b0 = -1.0f, b1 = -1.0f;
+    int i0 = 0x7fffffff, i1 = 0x7fffffff;
+    fft_wave_1024<1, true>(
+        buf, tw, [&](int i) { return buf[fft_wave_pad<4>(i)]; },
+        [&](int i, float2 z) {
+            const int idx = i * w + j;
+            e0 += (double)z.x * (double)z.x;
+            e1 += (double)z.y * (double)z.y;
+            argmax_merge(b0, i0, fabsf(z.x), idx);
+            argmax_merge(b1, i1, fabsf(z.y), idx);
+        });
+    e0 = wave_sum_d(e0);
+    e1 = wave_sum_d(e1);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float v2 = __shfl_xor(b0, off, 64);
+        int k2 = __shfl_xor(i0, off, 64);
+        argmax_merge(b0, i0, v2, k2);
+        v2 = __shfl_xor(b1, off, 64);
+        k2 = __shfl_xor(i1, off, 64);
+        argmax_merge(b1, i1, v2, k2);
+    }
+    const int lane = t & 63;
+    const int halves = 2 * q + 1 < nb ? 2 : 1;
+    if (lane < halves) {
+        PeakPart p;
+        p.energy = lane == 0 ? e0 : e1;
+        p.absmax = lane == 0 ? b0 : b1;
+        p.index = lane == 0 ? i0 : i1;
+        partall[(int64_t)(2 * q + lane) * w + j] = p;
+    }
+}
+
 // Value c[i][j] of pair hf (0: Re, 1: Im of the couple's complex plane) as
 // the direct inverse DFT of column j of the row-pass output A (h x w,
 // row-major): sum_k A[k][j] exp(+2 pi i i k / h), summed in FP64 per thread
@@ -625,10 +686,15 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
     }
 #undef PRNU_PCE_ROWS
     PRNU_CHECK_LAUNCH();
-    PRNU_TRY(launch_transpose_c(A, B, h, w, couples, pstride, st));
     const dim3 cgrid(w, couples);
     const size_t csm = fft_row_smem(h);
-    if (h == 1024)
+    const bool strips = h == 1024 && w % 8 == 0;
+    if (!strips) PRNU_TRY(launch_transpose_c(A, B, h, w, couples, pstride, st));
+    if (strips) {
+        const int per = (w / 8 + 7) / 8;
+        hipLaunchKernelGGL(pce_cols_strip_kernel, dim3(8 * per, couples), dim3(512), 0, st, dh.tw, w, nb, A, pstride,
+                           part, per);
+    } else if (h == 1024)
         hipLaunchKernelGGL(pce_cols_wave_kernel, cgrid, dim3(64), 0, st, dh.tw, w, nb, B, pstride, part);
     else if (h <= kFftSmallN)
         hipLaunchKernelGGL((pce_cols_kernel<kFftSmallN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, B, pstride,
