@@ -853,6 +853,49 @@ __global__ __launch_bounds__(64) void cols_fft_power_wave_kernel(const float2 *_
         });
 }
 
+// cols_fft_power_wave_kernel reading the row-major half spectra A (h x wh
+// per frame) itself, so no transpose precedes it: a workgroup of 8 waves
+// stages a strip of 8 adjacent columns into LDS (64-byte row pieces, each
+// column in its own padded buffer), then wave c transforms column c there and
+// writes B, P in the transposed layout as before.  Workgroup g takes strip
+// (g mod 8) * per + g / 8 (neighbouring strips on one XCD); the columns of
+// the last strip past wh are neither loaded nor stored.  grid (8 per, frames).
+template <int N>
+__global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 *__restrict__ tw,
+                                                                  const float2 *__restrict__ A, int wh, float2 *B,
+                                                                  float *P, int64_t plane, int per) {
+    constexpr int SH = FftWavePlan<N>::kPadShift;
+    constexpr int SB = fft_wave_lds<N>() + 4;  // +4 spreads the staging writes of the 8 buffers over banks
+    __shared__ __attribute__((aligned(16))) float2 lds[8 * SB];
+    const int g = blockIdx.x;
+    const int strip = (g & 7) * per + (g >> 3);
+    if (strip >= (wh + 7) / 8) return;
+    const int j0 = strip * 8;
+    const float2 *Af = A + (int64_t)blockIdx.y * plane;
+    const int t = (int)threadIdx.x, c = t & 7, r = t >> 3;
+    if (j0 + c < wh) {
+#pragma unroll 4
+        for (int k = 0; k < N / 64; ++k) {
+            const int i = r + 64 * k;
+            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * wh + j0 + c];
+        }
+    }
+    __syncthreads();
+    const int wave = t >> 6, col = j0 + wave;
+    float2 *buf = lds + wave * SB;
+    const bool store = col < wh;
+    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)col * N;
+    float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)col * N;
+    fft_wave<-1, N, true>(
+        buf, tw, [&](int i) { return buf[fft_wave_pad<SH>(i)]; },
+        [&](int j, float2 f) {
+            if (store) {
+                row[j] = f;
+                prow[j] = f.x * f.x + f.y * f.y;
+            }
+        });
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void cols_ifft_scaled_wave_kernel(const float2 *__restrict__ tw, float2 *B,
                                                                    const float *S, int64_t plane, int R) {
@@ -1134,13 +1177,17 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     } else {
         // transposed layout: B = A^T (wh x h), column FFTs as rows of B; the
         // zero-padded box-min stencil is transpose invariant
-        PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
-        if (h == 1024)
-            hipLaunchKernelGGL(cols_fft_power_wave_kernel<1024>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.P, n);
-        else if (h == 1536)
-            hipLaunchKernelGGL(cols_fft_power_wave_kernel<1536>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.P, n);
-        else
+        const int per = ((wh + 7) / 8 + 7) / 8;
+        if (h == 1024) {
+            hipLaunchKernelGGL(cols_fft_power_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A,
+                               wh, e.B, e.P, n, per);
+        } else if (h == 1536) {
+            hipLaunchKernelGGL(cols_fft_power_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A,
+                               wh, e.B, e.P, n, per);
+        } else {
+            PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
             PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(wh, nb), st, dh, e.B, e.P, n);
+        }
         PRNU_CHECK_LAUNCH();
         PRNU_TRY(launch_varest(e.P, e.S, w, h, true, nb, e.var, st));
         if (h == 1024)

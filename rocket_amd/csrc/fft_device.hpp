@@ -461,9 +461,9 @@ __device__ __forceinline__ void fft_wave_1024(float2 *buf, const float2 *tw, Get
 
 // 1536 points as radix 8, 8, 8, 3 (24 values per lane).  buf:
 // fft_wave_lds<1536>() values.
-template <int DIR, class Get, class Put>
+template <int DIR, bool SYNC_FIRST = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave_1536(float2 *buf, const float2 *tw, Get get, Put put) {
-    fft_wave_pass<8, DIR, true, false, 1536, 1, false, 3>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, true, false, 1536, 1, SYNC_FIRST, 3>(buf, tw, get, put);
     fft_wave_pass<8, DIR, false, false, 1536, 8, false, 3>(buf, tw, get, put);
     fft_wave_pass<8, DIR, false, false, 1536, 64, false, 3>(buf, tw, get, put);
     fft_wave_pass<3, DIR, false, true, 1536, 512, false, 3>(buf, tw, get, put);
@@ -491,12 +491,12 @@ __device__ __forceinline__ void fft_block_2048_r8(float2 *buf, const float2 *tw,
 }
 
 // The one-wave transform of a compile-time length (1024 or 1536).
-template <int DIR, int N, class Get, class Put>
+template <int DIR, int N, bool SYNC_FIRST = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave(float2 *buf, const float2 *tw, Get get, Put put) {
     if constexpr (N == 1024)
-        fft_wave_1024<DIR>(buf, tw, get, put);
+        fft_wave_1024<DIR, SYNC_FIRST>(buf, tw, get, put);
     else
-        fft_wave_1536<DIR>(buf, tw, get, put);
+        fft_wave_1536<DIR, SYNC_FIRST>(buf, tw, get, put);
 }
 
 // Transform of one row of length d.n <= MAXN from get(i) (i in [0, n), each
