@@ -835,8 +835,8 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
 }
 
 // grid (rows, B)
-// rows_fft_power_kernel, rows_ifft_scaled_kernel and
-// rows_fft_fwd_inplace_kernel for transposed columns of N = 1024 or 1536
+// rows_fft_power_kernel and rows_ifft_scaled_kernel for transposed columns
+// of N = 1024 or 1536
 // points, one wave per column (fft_wave): every lane works in every pass and
 // the barriers are one wave's.  Same grids, 64 threads.
 template <int N>
@@ -896,6 +896,48 @@ __global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 
         });
 }
 
+// The spectrum's column FFTs in the row-major layout, without transposes:
+// strips of 8 columns of A (h x wh per frame) staged into LDS as in
+// cols_fft_power_strip_kernel, transformed there in place by one wave per
+// column, and written to out (h x wh per frame, frames out_stride apart) as
+// 64-byte row pieces.  grid (8 per, frames).
+template <int N>
+__global__ __launch_bounds__(512) void cols_fft_strip_kernel(const float2 *__restrict__ tw,
+                                                            const float2 *__restrict__ A, int64_t plane, int wh,
+                                                            float2 *__restrict__ out, int64_t out_stride, int per) {
+    constexpr int SH = FftWavePlan<N>::kPadShift;
+    constexpr int SB = fft_wave_lds<N>() + 4;
+    __shared__ __attribute__((aligned(16))) float2 lds[8 * SB];
+    const int g = blockIdx.x;
+    const int strip = (g & 7) * per + (g >> 3);
+    if (strip >= (wh + 7) / 8) return;
+    const int j0 = strip * 8;
+    const float2 *Af = A + (int64_t)blockIdx.y * plane;
+    const int t = (int)threadIdx.x, c = t & 7, r = t >> 3;
+    const bool cin = j0 + c < wh;
+    if (cin) {
+#pragma unroll 4
+        for (int k = 0; k < N / 64; ++k) {
+            const int i = r + 64 * k;
+            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * wh + j0 + c];
+        }
+    }
+    __syncthreads();
+    float2 *buf = lds + (t >> 6) * SB;
+    // the last pass has read buf before it writes (a barrier between)
+    fft_wave<-1, N, true>(
+        buf, tw, [&](int i) { return buf[fft_wave_pad<SH>(i)]; },
+        [&](int j, float2 f) { buf[fft_wave_pad<SH>(j)] = f; });
+    float2 *of = out + (int64_t)blockIdx.y * out_stride;
+    if (cin) {
+#pragma unroll 4
+        for (int k = 0; k < N / 64; ++k) {
+            const int i = r + 64 * k;
+            of[(int64_t)i * wh + j0 + c] = lds[c * SB + fft_wave_pad<SH>(i)];
+        }
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void cols_ifft_scaled_wave_kernel(const float2 *__restrict__ tw, float2 *B,
                                                                    const float *S, int64_t plane, int R) {
@@ -907,14 +949,6 @@ __global__ __launch_bounds__(64) void cols_ifft_scaled_wave_kernel(const float2 
     fft_wave<1, N>(
         buf, tw, [&](int j) { return cscale(row[j], 0.5f * (srow[j] + mrow[j == 0 ? 0 : N - j])); },
         [&](int j, float2 f) { row[j] = f; });
-}
-
-template <int N>
-__global__ __launch_bounds__(64) void cols_fft_fwd_wave_kernel(const float2 *__restrict__ tw, float2 *B,
-                                                               int64_t plane) {
-    __shared__ float2 buf[fft_wave_lds<N>()];
-    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * N;
-    fft_wave<-1, N>(buf, tw, [&](int j) { return row[j]; }, [&](int j, float2 f) { row[j] = f; });
 }
 
 template <int MAXN>
@@ -1233,13 +1267,19 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     else
         PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h);
     PRNU_CHECK_LAUNCH();
+    if (h == 1024 || h == 1536) {  // no transposes: strips of 8 columns through LDS
+        const int per = ((wh + 7) / 8 + 7) / 8;
+        if (h == 1024)
+            hipLaunchKernelGGL(cols_fft_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A, n, wh,
+                               out, out_stride, per);
+        else
+            hipLaunchKernelGGL(cols_fft_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A, n, wh,
+                               out, out_stride, per);
+        PRNU_CHECK_LAUNCH();
+        return PRNU_OK;
+    }
     PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
-    if (h == 1024)
-        hipLaunchKernelGGL(cols_fft_fwd_wave_kernel<1024>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, n);
-    else if (h == 1536)
-        hipLaunchKernelGGL(cols_fft_fwd_wave_kernel<1536>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, n);
-    else
-        PRNU_LAUNCH_ROW(rows_fft_fwd_inplace_kernel, h, dim3(wh, nb), st, dh, e.B, n);
+    PRNU_LAUNCH_ROW(rows_fft_fwd_inplace_kernel, h, dim3(wh, nb), st, dh, e.B, n);
     PRNU_CHECK_LAUNCH();
     PRNU_TRY(launch_transpose_c(e.B, out, wh, h, nb, n, st, out_stride));
     return PRNU_OK;
