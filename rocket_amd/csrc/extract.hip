@@ -938,6 +938,43 @@ __global__ __launch_bounds__(512) void cols_fft_strip_kernel(const float2 *__res
     }
 }
 
+// cols_ifft_scaled_wave_kernel writing the row-major plane A (h x wh per
+// frame) itself, so no transpose follows it: wave c of a workgroup
+// transforms transposed row col = j0 + c of B (scaled as there) into its LDS
+// buffer, then the workgroup writes the strip's 8 columns as 64-byte row
+// pieces.  Strip mapping as cols_fft_power_strip_kernel; R as there.
+template <int N>
+__global__ __launch_bounds__(512) void cols_ifft_scaled_strip_kernel(const float2 *__restrict__ tw,
+                                                                    const float2 *__restrict__ B,
+                                                                    const float *__restrict__ S, int64_t plane,
+                                                                    int R, int wh, float2 *__restrict__ A, int per) {
+    constexpr int SH = FftWavePlan<N>::kPadShift;
+    constexpr int SB = fft_wave_lds<N>() + 4;
+    __shared__ __attribute__((aligned(16))) float2 lds[8 * SB];
+    const int g = blockIdx.x;
+    const int strip = (g & 7) * per + (g >> 3);
+    if (strip >= (wh + 7) / 8) return;
+    const int j0 = strip * 8;
+    const int t = (int)threadIdx.x, wave = t >> 6;
+    const int col = min(j0 + wave, wh - 1);  // a column past wh transforms the last one again and stores nothing
+    float2 *buf = lds + wave * SB;
+    const float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)col * N;
+    const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)col * N;
+    const float *mrow = S + (int64_t)blockIdx.y * plane + (int64_t)(col == 0 ? 0 : R - col) * N;
+    fft_wave<1, N>(
+        buf, tw, [&](int j) { return cscale(row[j], 0.5f * (srow[j] + mrow[j == 0 ? 0 : N - j])); },
+        [&](int j, float2 f) { buf[fft_wave_pad<SH>(j)] = f; });  // the last pass has read buf before it writes
+    const int c = t & 7, r = t >> 3;
+    if (j0 + c < wh) {
+        float2 *Af = A + (int64_t)blockIdx.y * plane;
+#pragma unroll 4
+        for (int k = 0; k < N / 64; ++k) {
+            const int i = r + 64 * k;
+            Af[(int64_t)i * wh + j0 + c] = lds[c * SB + fft_wave_pad<SH>(i)];
+        }
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void cols_ifft_scaled_wave_kernel(const float2 *__restrict__ tw, float2 *B,
                                                                    const float *S, int64_t plane, int R) {
@@ -1224,16 +1261,18 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
         }
         PRNU_CHECK_LAUNCH();
         PRNU_TRY(launch_varest(e.P, e.S, w, h, true, nb, e.var, st));
-        if (h == 1024)
-            hipLaunchKernelGGL(cols_ifft_scaled_wave_kernel<1024>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.S, n,
-                               w);
-        else if (h == 1536)
-            hipLaunchKernelGGL(cols_ifft_scaled_wave_kernel<1536>, dim3(wh, nb), dim3(64), 0, st, dh.tw, e.B, e.S, n,
-                               w);
-        else
+        if (h == 1024) {
+            hipLaunchKernelGGL(cols_ifft_scaled_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.B,
+                               e.S, n, w, wh, e.A, per);
+        } else if (h == 1536) {
+            hipLaunchKernelGGL(cols_ifft_scaled_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.B,
+                               e.S, n, w, wh, e.A, per);
+        } else {
             PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(wh, nb), st, dh, e.B, e.S, n, w);
+            PRNU_CHECK_LAUNCH();
+            PRNU_TRY(launch_transpose_c(e.B, e.A, wh, h, nb, n, st));
+        }
         PRNU_CHECK_LAUNCH();
-        PRNU_TRY(launch_transpose_c(e.B, e.A, wh, h, nb, n, st));
     }
     if (w == 2048)
         hipLaunchKernelGGL((rows_c2r_kernel<kFftSmallN, 2048>), dim3(pairs, nb), dim3(128),
