@@ -835,26 +835,10 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
 }
 
 // grid (rows, B)
-// rows_fft_power_kernel and rows_ifft_scaled_kernel for transposed columns
-// of N = 1024 or 1536
-// points, one wave per column (fft_wave): every lane works in every pass and
-// the barriers are one wave's.  Same grids, 64 threads.
-template <int N>
-__global__ __launch_bounds__(64) void cols_fft_power_wave_kernel(const float2 *__restrict__ tw, float2 *B, float *P,
-                                                                 int64_t plane) {
-    __shared__ float2 buf[fft_wave_lds<N>()];
-    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * N;
-    float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)blockIdx.x * N;
-    fft_wave<-1, N>(
-        buf, tw, [&](int j) { return row[j]; },
-        [&](int j, float2 f) {
-            row[j] = f;
-            prow[j] = f.x * f.x + f.y * f.y;
-        });
-}
-
-// cols_fft_power_wave_kernel reading the row-major half spectra A (h x wh
-// per frame) itself, so no transpose precedes it: a workgroup of 8 waves
+// rows_fft_power_kernel for columns of N = 1024 or 1536 points, reading the
+// row-major half spectra A (h x wh per frame) itself, so no transpose
+// precedes it: each column by one wave (fft_wave: every lane works in every
+// pass).  A workgroup of 8 waves
 // stages a strip of 8 adjacent columns into LDS (64-byte row pieces, each
 // column in its own padded buffer), then wave c transforms column c there and
 // writes B, P in the transposed layout as before.  Workgroup g takes strip
@@ -938,9 +922,10 @@ __global__ __launch_bounds__(512) void cols_fft_strip_kernel(const float2 *__res
     }
 }
 
-// cols_ifft_scaled_wave_kernel writing the row-major plane A (h x wh per
-// frame) itself, so no transpose follows it: wave c of a workgroup
-// transforms transposed row col = j0 + c of B (scaled as there) into its LDS
+// rows_ifft_scaled_kernel for N = 1024 or 1536 writing the row-major plane A
+// (h x wh per frame) itself, so no transpose follows it: wave c of a
+// workgroup transforms transposed row col = j0 + c of B (scaled as there) by
+// one-wave transforms into its LDS
 // buffer, then the workgroup writes the strip's 8 columns as 64-byte row
 // pieces.  Strip mapping as cols_fft_power_strip_kernel; R as there.
 template <int N>
@@ -973,19 +958,6 @@ __global__ __launch_bounds__(512) void cols_ifft_scaled_strip_kernel(const float
             Af[(int64_t)i * wh + j0 + c] = lds[c * SB + fft_wave_pad<SH>(i)];
         }
     }
-}
-
-template <int N>
-__global__ __launch_bounds__(64) void cols_ifft_scaled_wave_kernel(const float2 *__restrict__ tw, float2 *B,
-                                                                   const float *S, int64_t plane, int R) {
-    __shared__ float2 buf[fft_wave_lds<N>()];
-    const int r = blockIdx.x;
-    float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)r * N;
-    const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)r * N;
-    const float *mrow = S + (int64_t)blockIdx.y * plane + (int64_t)(r == 0 ? 0 : R - r) * N;
-    fft_wave<1, N>(
-        buf, tw, [&](int j) { return cscale(row[j], 0.5f * (srow[j] + mrow[j == 0 ? 0 : N - j])); },
-        [&](int j, float2 f) { row[j] = f; });
 }
 
 template <int MAXN>
