@@ -870,7 +870,7 @@ __global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 
     const bool store = col < wh;
     float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)col * N;
     float *prow = P + (int64_t)blockIdx.y * plane + (int64_t)col * N;
-    fft_wave<-1, N, true>(
+    fft_wave<-1, N, true, true>(  // after the staging barrier each wave works in its own buffer
         buf, tw, [&](int i) { return buf[fft_wave_pad<SH>(i)]; },
         [&](int j, float2 f) {
             if (store) {
@@ -908,10 +908,12 @@ __global__ __launch_bounds__(512) void cols_fft_strip_kernel(const float2 *__res
     }
     __syncthreads();
     float2 *buf = lds + (t >> 6) * SB;
-    // the last pass has read buf before it writes (a barrier between)
-    fft_wave<-1, N, true>(
+    // the last pass has read buf before it writes (a barrier between); each
+    // wave works in its own buffer until the store of the whole strip
+    fft_wave<-1, N, true, true>(
         buf, tw, [&](int i) { return buf[fft_wave_pad<SH>(i)]; },
         [&](int j, float2 f) { buf[fft_wave_pad<SH>(j)] = f; });
+    __syncthreads();
     float2 *of = out + (int64_t)blockIdx.y * out_stride;
     if (cin) {
 #pragma unroll 4
@@ -946,9 +948,10 @@ __global__ __launch_bounds__(512) void cols_ifft_scaled_strip_kernel(const float
     const float2 *row = B + (int64_t)blockIdx.y * plane + (int64_t)col * N;
     const float *srow = S + (int64_t)blockIdx.y * plane + (int64_t)col * N;
     const float *mrow = S + (int64_t)blockIdx.y * plane + (int64_t)(col == 0 ? 0 : R - col) * N;
-    fft_wave<1, N>(
+    fft_wave<1, N, false, true>(  // each wave in its own buffer until the store of the whole strip
         buf, tw, [&](int j) { return cscale(row[j], 0.5f * (srow[j] + mrow[j == 0 ? 0 : N - j])); },
         [&](int j, float2 f) { buf[fft_wave_pad<SH>(j)] = f; });  // the last pass has read buf before it writes
+    __syncthreads();
     const int c = t & 7, r = t >> 3;
     if (j0 + c < wh) {
         float2 *Af = A + (int64_t)blockIdx.y * plane;

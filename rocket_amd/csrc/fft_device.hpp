@@ -407,11 +407,25 @@ __device__ __forceinline__ int fft_wave_pad(int p) { return p + (p >> SH); }
 
 // SYNC_FIRST: a block barrier between the first pass's reads (get) and its
 // writes, for callers whose get reads LDS that the waves' buffers overlay.
+// Ordering of one wave's LDS accesses among its own lanes: a wave's LDS
+// instructions execute in order, so the fences (which keep the compiler from
+// moving memory accesses across) and a wave barrier suffice; other waves of
+// the workgroup are not waited for.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // T: the threads of the transform (64: one wave; 128: two waves, whose block
-// barrier then waits for the other wave).
+// barrier then waits for the other wave).  WSYNC (T = 64 only): the
+// transform's buffer is its wave's alone, so its passes order their LDS
+// accesses with wave_lds_sync instead of block barriers (the caller
+// synchronises the workgroup where waves share LDS).
 template <int R, int DIR, bool FIRST, bool LAST, int N, int NS, bool SYNC_FIRST = false, int SH = 4, int T = 64,
-          class Get, class Put>
+          bool WSYNC = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get &get, Put &put) {
+    static_assert(!WSYNC || T == 64, "fft_wave_pass: wave-level ordering needs a one-wave transform");
     constexpr int nb = N / R;
     static_assert(nb % T == 0, "fft_wave_pass: N / R must be a multiple of the thread count");
     constexpr int per = nb / T;
@@ -432,7 +446,12 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
                 v[b][r] = cmul(buf[fft_wave_pad<SH>(j + r * nb)], twiddle<DIR>(tw, r * k * twstep));
         }
     }
-    if constexpr (!FIRST || SYNC_FIRST) __syncthreads();  // every read precedes the writes below
+    if constexpr (!FIRST || SYNC_FIRST) {  // every read precedes the writes below
+        if constexpr (WSYNC)
+            wave_lds_sync();
+        else
+            __syncthreads();
+    }
 #pragma unroll
     for (int b = 0; b < per; ++b) {
         const int j = lane + T * b;
@@ -447,26 +466,29 @@ __device__ __forceinline__ void fft_wave_pass(float2 *buf, const float2 *tw, Get
                 buf[fft_wave_pad<SH>(base + q * NS)] = v[b][q];
         }
     }
-    __syncthreads();
+    if constexpr (WSYNC)
+        wave_lds_sync();
+    else
+        __syncthreads();
 }
 
 // 1024 points as radix 16, 16, 4 (16 values per lane); get(i) and put(i, X)
 // as fft_row's.  buf: fft_wave_lds(1024) values.
-template <int DIR, bool SYNC_FIRST = false, class Get, class Put>
+template <int DIR, bool SYNC_FIRST = false, bool WSYNC = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave_1024(float2 *buf, const float2 *tw, Get get, Put put) {
-    fft_wave_pass<16, DIR, true, false, 1024, 1, SYNC_FIRST>(buf, tw, get, put);
-    fft_wave_pass<16, DIR, false, false, 1024, 16>(buf, tw, get, put);
-    fft_wave_pass<4, DIR, false, true, 1024, 256>(buf, tw, get, put);
+    fft_wave_pass<16, DIR, true, false, 1024, 1, SYNC_FIRST, 4, 64, WSYNC>(buf, tw, get, put);
+    fft_wave_pass<16, DIR, false, false, 1024, 16, false, 4, 64, WSYNC>(buf, tw, get, put);
+    fft_wave_pass<4, DIR, false, true, 1024, 256, false, 4, 64, WSYNC>(buf, tw, get, put);
 }
 
 // 1536 points as radix 8, 8, 8, 3 (24 values per lane).  buf:
 // fft_wave_lds<1536>() values.
-template <int DIR, bool SYNC_FIRST = false, class Get, class Put>
+template <int DIR, bool SYNC_FIRST = false, bool WSYNC = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave_1536(float2 *buf, const float2 *tw, Get get, Put put) {
-    fft_wave_pass<8, DIR, true, false, 1536, 1, SYNC_FIRST, 3>(buf, tw, get, put);
-    fft_wave_pass<8, DIR, false, false, 1536, 8, false, 3>(buf, tw, get, put);
-    fft_wave_pass<8, DIR, false, false, 1536, 64, false, 3>(buf, tw, get, put);
-    fft_wave_pass<3, DIR, false, true, 1536, 512, false, 3>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, true, false, 1536, 1, SYNC_FIRST, 3, 64, WSYNC>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, false, 1536, 8, false, 3, 64, WSYNC>(buf, tw, get, put);
+    fft_wave_pass<8, DIR, false, false, 1536, 64, false, 3, 64, WSYNC>(buf, tw, get, put);
+    fft_wave_pass<3, DIR, false, true, 1536, 512, false, 3, 64, WSYNC>(buf, tw, get, put);
 }
 
 // 2048 points as radix 16, 16, 8 on two waves (128 threads, 16 values per
@@ -491,12 +513,12 @@ __device__ __forceinline__ void fft_block_2048_r8(float2 *buf, const float2 *tw,
 }
 
 // The one-wave transform of a compile-time length (1024 or 1536).
-template <int DIR, int N, bool SYNC_FIRST = false, class Get, class Put>
+template <int DIR, int N, bool SYNC_FIRST = false, bool WSYNC = false, class Get, class Put>
 __device__ __forceinline__ void fft_wave(float2 *buf, const float2 *tw, Get get, Put put) {
     if constexpr (N == 1024)
-        fft_wave_1024<DIR, SYNC_FIRST>(buf, tw, get, put);
+        fft_wave_1024<DIR, SYNC_FIRST, WSYNC>(buf, tw, get, put);
     else
-        fft_wave_1536<DIR, SYNC_FIRST>(buf, tw, get, put);
+        fft_wave_1536<DIR, SYNC_FIRST, WSYNC>(buf, tw, get, put);
 }
 
 // Transform of one row of length d.n <= MAXN from get(i) (i in [0, n), each

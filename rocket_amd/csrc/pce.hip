@@ -443,7 +443,7 @@ __global__ __launch_bounds__(512) void pce_cols_strip_kernel(const float2 *__res
     double e0 = 0.0, e1 = 0.0;
     float b0 = -1.0f, b1 = -1.0f;
     int i0 = 0x7fffffff, i1 = 0x7fffffff;
-    fft_wave_1024<1, true>(
+    fft_wave_1024<1, true, true>(  // after the staging barrier each wave works in its own buffer
         buf, tw, [&](int i) { return buf[fft_wave_pad<4>(i)]; },
         [&](int i, float2 z) {
             const int idx = i * w + j;
