@@ -353,20 +353,24 @@ __global__ __launch_bounds__(128) void pce_rows_mirror_wave_kernel(const float2 
     float2 *A = Aall + (int64_t)q * pstride;
     const int row = wave == 0 ? u : um;
     const bool store = wave == 0 || um != u;
-    fft_wave_1024<1, true>(
-        buf, tw,
-        [&](int j) {
-            if (wave == 0) {
-                const float2 a = ha[j], b = hb[j];
-                return make_float2(a.x - b.y, a.y + b.x);
-            }
-            const int mj = j == 0 ? 0 : w - j;
-            const float2 a = ha[mj], b = hb[mj];
-            return make_float2(a.x + b.y, b.x - a.y);
-        },
-        [&](int j, float2 f) {
-            if (store) A[(int64_t)row * w + j] = f;
-        });
+    auto get = [&](int j) {
+        if (wave == 0) {
+            const float2 a = ha[j], b = hb[j];
+            return make_float2(a.x - b.y, a.y + b.x);
+        }
+        const int mj = j == 0 ? 0 : w - j;
+        const float2 a = ha[mj], b = hb[mj];
+        return make_float2(a.x + b.y, b.x - a.y);
+    };
+    auto put = [&](int j, float2 f) {
+        if (store) A[(int64_t)row * w + j] = f;
+    };
+    // fft_wave_1024's passes; the first with workgroup barriers (both waves
+    // read H_a, H_b, which their buffers overlay), the others each in its
+    // own buffer with wave-level ordering
+    fft_wave_pass<16, 1, true, false, 1024, 1, true>(buf, tw, get, put);
+    fft_wave_pass<16, 1, false, false, 1024, 16, false, 4, 64, true>(buf, tw, get, put);
+    fft_wave_pass<4, 1, false, true, 1024, 256, false, 4, 64, true>(buf, tw, get, put);
 }
 
 // pce_cols_kernel for h = 1024 with one wave per column (fft_wave_1024: 16
