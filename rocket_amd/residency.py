@@ -174,10 +174,12 @@ class CachedBlockGram:
 
     `nslots` None sizes the cache as the device cache does: `requested`
     bytes, or the free device memory less `-requested` bytes when
-    `requested` <= 0."""
+    `requested` <= 0; the default leaves 500 MiB free, the reference's
+    default (``R/RocketLauncherArgs.java:14-15``)."""
 
     def __init__(self, gram: "csi.GramNCC", n: int, block: int, format: str = "split_bf16",
-                 nslots: int | None = None, requested: int = 0, window: int = 3, slab_values: int = 0):
+                 nslots: int | None = None, requested: int = -500 * 1024 * 1024, window: int = 3,
+                 slab_values: int = 0):
         import torch
         if block < 1 or window < 1:
             raise ValueError("block and window must be positive")
