@@ -237,7 +237,11 @@ class CachedBlockGram:
         """NCC of every pair of patterns in the blocks `keys` (in that order)
         as a symmetric float64 matrix, and the call's statistics (cache hits,
         misses and evictions, block loads, bytes copied to the device,
-        leaves, seconds)."""
+        leaves, seconds).  If the call fails (a block source raising, say),
+        it first gives back every entry it holds, as the reference's worker
+        releases its entries whatever the outcome (DeviceWorker.java:
+        285-293): a writer released unfilled leaves its entry INVALID, to be
+        filled by the next call that asks for it."""
         import torch
         keys = [int(k) for k in keys]
         if len(set(keys)) != len(keys):
@@ -249,11 +253,42 @@ class CachedBlockGram:
         total = int(offs[-1])
         result = np.zeros((total, total), dtype=np.float64)
         before = self.cache.stats()
-        loads = moved = 0
         leaves = indexspace.all_pairs(len(keys), order, param, include_diagonal=True)
         inflight: deque = deque()
-        B = self.block
+        held: list = []                      # keys of the leaf being set up, until it is in flight
+        walk = {"loads": 0, "moved": 0}
         t0 = time.perf_counter()
+        try:
+            self._walk(source, keys, counts, offs, leaves, inflight, held, result, walk)
+            while inflight:
+                self._retire(inflight, result)
+        except BaseException:
+            torch.cuda.synchronize()
+            while inflight:
+                _, ks, _ = inflight.popleft()
+                for key in ks:
+                    self.cache.release(key)
+            for key in held:
+                self.cache.release(key)
+            raise
+        seconds = time.perf_counter() - t0
+        il = np.tril_indices(total, -1)
+        result[il] = result.T[il]
+        after = self.cache.stats()
+        stats = {k: after[k] - before[k] for k in ("hits", "misses", "evictions")}
+        stats.update({"block_loads": walk["loads"], "bytes_to_device": walk["moved"], "leaves": len(leaves),
+                      "seconds": seconds, "slots": self.nslots, "block": self.block})
+        self.totals["leaves"] += len(leaves)
+        self.totals["block_loads"] += walk["loads"]
+        self.totals["bytes_to_device"] += walk["moved"]
+        return result, stats
+
+    def _walk(self, source, keys, counts, offs, leaves, inflight: deque, held: list, result: np.ndarray,
+              walk: dict) -> None:
+        """The leaves in order: acquire, fill misses, launch, keep `window`
+        leaves in flight.  `held` names the keys of the leaf being set up."""
+        import torch
+        B = self.block
         for step, (a, b) in enumerate(leaves):
             a, b = int(a), int(b)
             ka, kb = keys[a], keys[b]
@@ -266,6 +301,7 @@ class CachedBlockGram:
                     raise RuntimeError("slot cache transaction cannot complete with no leaf in flight")
                 self._retire(inflight, result)
                 got = self.cache.poll(txn)
+            held[:] = [kk for kk, _, _ in got]
             slot = {}
             for key, s, writer in got:
                 slot[key] = s
@@ -275,8 +311,8 @@ class CachedBlockGram:
                         source.load(key, self.store, s * B)
                     self.loaded[s].record(self.copy)
                     self.cache.ready(key, source.count(key))
-                    loads += 1
-                    moved += source.nbytes(key)
+                    walk["loads"] += 1
+                    walk["moved"] += source.nbytes(key)
             sa, sb = slot[ka], slot[kb]
             na, nb = counts[a], counts[b]
             k = step % self.window
@@ -294,20 +330,8 @@ class CachedBlockGram:
             self.used[sa].record(self.compute)
             self.used[sb].record(self.compute)
             self.done[k].record(self.compute)
-            inflight.append((k, [kk for kk, _, _ in got], (int(offs[a]), int(offs[b]), na, nb)))
-        while inflight:
-            self._retire(inflight, result)
-        seconds = time.perf_counter() - t0
-        il = np.tril_indices(total, -1)
-        result[il] = result.T[il]
-        after = self.cache.stats()
-        stats = {k: after[k] - before[k] for k in ("hits", "misses", "evictions")}
-        stats.update({"block_loads": loads, "bytes_to_device": moved, "leaves": len(leaves), "seconds": seconds,
-                      "slots": self.nslots, "block": self.block})
-        self.totals["leaves"] += len(leaves)
-        self.totals["block_loads"] += loads
-        self.totals["bytes_to_device"] += moved
-        return result, stats
+            inflight.append((k, list(held), (int(offs[a]), int(offs[b]), na, nb)))
+            held.clear()
 
     def close(self) -> None:
         import torch

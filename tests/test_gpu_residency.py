@@ -124,3 +124,26 @@ def test_cache_sized_from_a_byte_budget(setup):
     got, _ = cg.ncc(residency.TierSource(tier), range(tier.nblocks))
     assert np.array_equal(got, resident)
     cg.close()
+
+
+def test_failed_call_gives_back_its_entries(setup):
+    """A block source that fails mid-call: the call raises, every entry it
+    held is released (the unfilled one INVALID, refilled later), and the next
+    call on the same cache gives the resident Gram's bits."""
+    p, gram, slab, resident, tier, want = setup
+    counts = {b: tier.block_range(b)[1] - tier.block_range(b)[0] for b in range(tier.nblocks)}
+
+    def block(b):
+        if b == 4:
+            raise RuntimeError("block 4 unavailable")
+        r0, r1 = tier.block_range(b)
+        return p[r0:r1]
+
+    cg = residency.CachedBlockGram(gram, N, BLOCK, "split_bf16", nslots=3, window=2, slab_values=slab)
+    with pytest.raises(RuntimeError, match="block 4 unavailable"):
+        cg.ncc(residency.PatternSource(block, counts), range(tier.nblocks))
+    st = cg.cache.stats()
+    assert st["waiting"] == 0 and st["unreferenced"] == st["entries"]   # nothing still referenced
+    got, _ = cg.ncc(residency.TierSource(tier), range(tier.nblocks))
+    assert np.array_equal(got, resident)
+    cg.close()
