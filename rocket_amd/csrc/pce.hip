@@ -553,24 +553,36 @@ __device__ void pair_peak(const PeakPart *part, int w, int h, double *shd, float
 // the last element c[h-1][w-1] in the reference variant; the textbook
 // variant's peak value comes with the window row that holds it).  grid
 // (batch, 12).
+// Per pair, once: the total energy and the argmax of the column partials
+// (pair_peak) into slots 12 and 14 (the index as an exact double), read by
+// every window row's block.  grid (pairs), 256 threads.
+__global__ __launch_bounds__(256) void pce_peak_kernel(const PeakPart *__restrict__ partall, int h, int w,
+                                                       double *__restrict__ wbuf) {
+    __shared__ double shd[4];
+    __shared__ float shv[4];
+    __shared__ int shi[4];
+    __shared__ int s_peak;
+    const int p = blockIdx.x;
+    double total;
+    pair_peak(partall + (int64_t)p * w, w, h, shd, shv, shi, &s_peak, &total);
+    if (threadIdx.x == 0) {
+        wbuf[(int64_t)p * kWinSlots + 12] = total;
+        wbuf[(int64_t)p * kWinSlots + 14] = (double)s_peak;
+    }
+}
+
 template <int VARIANT>
 __global__ __launch_bounds__(256) void pce_window_kernel(const PeakPart *__restrict__ partall,
                                                          const float2 *__restrict__ Aall, int64_t pstride,
                                                          const float2 *__restrict__ tw, int h, int w,
                                                          double *__restrict__ wbuf) {
-    __shared__ double shd[4];
-    __shared__ float shv[4];
-    __shared__ int shi[4];
-    __shared__ int s_peak;
     __shared__ double sh_cell[kMaxCells][4];
     __shared__ double cells[kMaxCells];
     __shared__ int ci[kMaxCells], cj[kMaxCells];
     const int p = blockIdx.x, y = blockIdx.y, hf = p & 1;
     const float2 *A = Aall + (int64_t)(p >> 1) * pstride;
     double *slots = wbuf + (int64_t)p * kWinSlots;
-    double total;
-    pair_peak(partall + (int64_t)p * w, w, h, shd, shv, shi, &s_peak, &total);
-    const int peak = s_peak;
+    const int peak = (int)slots[14];  // pce_peak_kernel's
     const int py = peak / w, px = peak - py * w;
     int r0, nr, c0, nc;
     bool wrap;
@@ -587,10 +599,8 @@ __global__ __launch_bounds__(256) void pce_window_kernel(const PeakPart *__restr
         c0 = w < kSquare ? 0 : px - kRadius;
         wrap = true;
     }
-    if (y == 0 && threadIdx.x == 0) {
-        slots[12] = total;
-        slots[13] = VARIANT == PRNU_PCE_REFERENCE ? (double)(kSquare * kSquare) : (double)(nr * nc);  // the reference divides by N - 121
-    }
+    if (y == 0 && threadIdx.x == 0)  // the reference divides by N - 121
+        slots[13] = VARIANT == PRNU_PCE_REFERENCE ? (double)(kSquare * kSquare) : (double)(nr * nc);
     const int t = (int)threadIdx.x;  // signed: c0 may be negative before the wrap
     if (y < kSquare) {
         if (y >= nr) {
@@ -710,16 +720,19 @@ int pce_batch(Context *c, int variant, const PceBatch &bt, int nb, double *out, 
         hipLaunchKernelGGL((pce_cols_kernel<kFftMaxN>), cgrid, dim3(kFftThreads), csm, st, dh, w, nb, B, pstride,
                            part);
     PRNU_CHECK_LAUNCH();
-    const dim3 wgrid(nb, kSquare + 1), sgrid((nb + 63) / 64);
+    // window rows (and, for the reference variant, the last element's value)
+    const dim3 wgrid_ref(nb, kSquare + 1), wgrid_txt(nb, kSquare), sgrid((nb + 63) / 64);
+    hipLaunchKernelGGL(pce_peak_kernel, dim3(nb), dim3(256), 0, st, part, h, w, wbuf);
+    PRNU_CHECK_LAUNCH();
     if (variant == PRNU_PCE_REFERENCE) {
-        hipLaunchKernelGGL(pce_window_kernel<PRNU_PCE_REFERENCE>, wgrid, dim3(256), 0, st, part, A, pstride, dh.tw, h,
-                           w, wbuf);
+        hipLaunchKernelGGL(pce_window_kernel<PRNU_PCE_REFERENCE>, wgrid_ref, dim3(256), 0, st, part, A, pstride, dh.tw,
+                           h, w, wbuf);
         PRNU_CHECK_LAUNCH();
         hipLaunchKernelGGL(pce_score_kernel<PRNU_PCE_REFERENCE>, sgrid, dim3(64), 0, st, wbuf, nb, h, w, out,
                            bt.base);
     } else {
-        hipLaunchKernelGGL(pce_window_kernel<PRNU_PCE_TEXTBOOK>, wgrid, dim3(256), 0, st, part, A, pstride, dh.tw, h,
-                           w, wbuf);
+        hipLaunchKernelGGL(pce_window_kernel<PRNU_PCE_TEXTBOOK>, wgrid_txt, dim3(256), 0, st, part, A, pstride, dh.tw,
+                           h, w, wbuf);
         PRNU_CHECK_LAUNCH();
         hipLaunchKernelGGL(pce_score_kernel<PRNU_PCE_TEXTBOOK>, sgrid, dim3(64), 0, st, wbuf, nb, h, w, out, bt.base);
     }
