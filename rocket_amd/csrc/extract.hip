@@ -535,7 +535,8 @@ template <int MAXN, bool PAD2048 = false>
 __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d, const float *img_all,
                                                                       int64_t img_stride, int h, float2 *A_all,
                                                                       int64_t plane, double *rowsq_all,
-                                                                      const float *cz_all, const float *rz_all) {
+                                                                      const float *cz_all, const float *rz_all,
+                                                                      int lda) {
     extern __shared__ float2 buf[];
     __shared__ double sh[8];
     const int i0 = 2 * blockIdx.x, i1 = i0 + 1, b = blockIdx.y;
@@ -568,11 +569,11 @@ __global__ __launch_bounds__(kFftThreads) void wiener_rows_fwd_kernel(FftDesc d,
         fft_block_2048_r8<-1>(buf, d.tw, get, put);
     else
         fft_row<-1, MAXN>(buf, d, get, put);
-    float2 *a0 = A_all + (int64_t)b * plane + (int64_t)i0 * wh;
+    float2 *a0 = A_all + (int64_t)b * plane + (int64_t)i0 * lda;  // rows lda values apart
     for (int k = threadIdx.x; k < wh; k += kFftThreads) {
         const float2 zk = buf[k], zm_ = buf[k == 0 ? 0 : w - k];
         a0[k] = make_float2(0.5f * (zk.x + zm_.x), 0.5f * (zk.y - zm_.y));
-        if (two) a0[wh + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
+        if (two) a0[lda + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
     }
     // both row sums in one fixed-order block reduction
     sq0 = wave_sum_d(sq0);
@@ -737,17 +738,18 @@ int launch_varest(const float *P, float *S, int rows, int cols, bool mirror_rows
 template <int MAXN, int BLOCKN = 0>
 __global__ __launch_bounds__(BLOCKN ? 128 : kFftThreads) void rows_c2r_kernel(FftDesc d, const float2 *A,
                                                                               int64_t plane, int h, float *out_all,
-                                                                              int64_t out_stride, float inv_n) {
+                                                                              int64_t out_stride, float inv_n,
+                                                                              int lda) {
     extern __shared__ float2 buf[];
     const int i0 = 2 * blockIdx.x, i1 = i0 + 1;
     const bool two = i1 < h;
     const int w = d.n, wh = w / 2 + 1;
-    const float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * wh;
+    const float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * lda;  // rows lda values apart
     float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i0 * w;
     auto get = [&](int k) {
         const bool up = k > w / 2;
         const int kk = up ? w - k : k;
-        float2 x = a0[kk], y = two ? a0[wh + kk] : make_float2(0.f, 0.f);
+        float2 x = a0[kk], y = two ? a0[lda + kk] : make_float2(0.f, 0.f);
         if (kk == 0 || 2 * kk == w) {
             x.y = 0.f;
             y.y = 0.f;
@@ -776,7 +778,7 @@ __global__ __launch_bounds__(BLOCKN ? 128 : kFftThreads) void rows_c2r_kernel(Ff
 template <int MAXN, bool PAD2048 = false>
 __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, const float *img_all,
                                                                     int64_t img_stride, float2 *A, int64_t plane,
-                                                                    int h) {
+                                                                    int h, int lda) {
     extern __shared__ float2 buf[];
     const int i0 = 2 * blockIdx.x, i1 = i0 + 1;
     const bool two = i1 < h;
@@ -789,11 +791,11 @@ __global__ __launch_bounds__(kFftThreads) void rows_fft_half_kernel(FftDesc d, c
         fft_block_2048_r8<-1>(buf, d.tw, get, put);
     else
         fft_row<-1, MAXN>(buf, d, get, put);
-    float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * wh;
+    float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * lda;  // rows lda values apart
     for (int k = threadIdx.x; k < wh; k += kFftThreads) {
         const float2 zk = buf[k], zm_ = buf[k == 0 ? 0 : w - k];
         a0[k] = make_float2(0.5f * (zk.x + zm_.x), 0.5f * (zk.y - zm_.y));
-        if (two) a0[wh + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
+        if (two) a0[lda + k] = make_float2(0.5f * (zk.y + zm_.y), -0.5f * (zk.x - zm_.x));
     }
 }
 
@@ -847,7 +849,7 @@ __global__ __launch_bounds__(kFftThreads) void rows_ifft_scaled_kernel(FftDesc d
 template <int N>
 __global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 *__restrict__ tw,
                                                                   const float2 *__restrict__ A, int wh, float2 *B,
-                                                                  float *P, int64_t plane, int per) {
+                                                                  float *P, int64_t plane, int per, int lda) {
     constexpr int SH = FftWavePlan<N>::kPadShift;
     constexpr int SB = fft_wave_lds<N>() + 4;  // +4 spreads the staging writes of the 8 buffers over banks
     __shared__ __attribute__((aligned(16))) float2 lds[8 * SB];
@@ -861,7 +863,7 @@ __global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 
 #pragma unroll 4
         for (int k = 0; k < N / 64; ++k) {
             const int i = r + 64 * k;
-            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * wh + j0 + c];
+            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * lda + j0 + c];
         }
     }
     __syncthreads();
@@ -886,9 +888,10 @@ __global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 
 // column, and written to out (h x wh per frame, frames out_stride apart) as
 // 64-byte row pieces.  grid (8 per, frames).
 template <int N>
-__global__ __launch_bounds__(512) void cols_fft_strip_kernel(const float2 *__restrict__ tw,
+__global__ __launch_bounds__(512) void spectrum_cols_strip_kernel(const float2 *__restrict__ tw,
                                                             const float2 *__restrict__ A, int64_t plane, int wh,
-                                                            float2 *__restrict__ out, int64_t out_stride, int per) {
+                                                            float2 *__restrict__ out, int64_t out_stride, int per,
+                                                            int lda) {
     constexpr int SH = FftWavePlan<N>::kPadShift;
     constexpr int SB = fft_wave_lds<N>() + 4;
     __shared__ __attribute__((aligned(16))) float2 lds[8 * SB];
@@ -903,7 +906,7 @@ __global__ __launch_bounds__(512) void cols_fft_strip_kernel(const float2 *__res
 #pragma unroll 4
         for (int k = 0; k < N / 64; ++k) {
             const int i = r + 64 * k;
-            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * wh + j0 + c];
+            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * lda + j0 + c];
         }
     }
     __syncthreads();
@@ -934,7 +937,8 @@ template <int N>
 __global__ __launch_bounds__(512) void cols_ifft_scaled_strip_kernel(const float2 *__restrict__ tw,
                                                                     const float2 *__restrict__ B,
                                                                     const float *__restrict__ S, int64_t plane,
-                                                                    int R, int wh, float2 *__restrict__ A, int per) {
+                                                                    int R, int wh, float2 *__restrict__ A, int per,
+                                                                    int lda) {
     constexpr int SH = FftWavePlan<N>::kPadShift;
     constexpr int SB = fft_wave_lds<N>() + 4;
     __shared__ __attribute__((aligned(16))) float2 lds[8 * SB];
@@ -958,7 +962,7 @@ __global__ __launch_bounds__(512) void cols_ifft_scaled_strip_kernel(const float
 #pragma unroll 4
         for (int k = 0; k < N / 64; ++k) {
             const int i = r + 64 * k;
-            Af[(int64_t)i * wh + j0 + c] = lds[c * SB + fft_wave_pad<SH>(i)];
+            Af[(int64_t)i * lda + j0 + c] = lds[c * SB + fft_wave_pad<SH>(i)];
         }
     }
 }
@@ -1190,6 +1194,17 @@ int launch_cols_fft(const FftDesc &dh, float2 *A, int64_t plane, int ld, int nco
     return PRNU_OK;
 }
 
+// Row pitch of the chain's half-spectrum plane A: 128-byte rows (a multiple
+// of 16 complex values) where the strip kernels read or write A in 64-byte
+// row pieces (1024 and 1536 rows), so that no piece straddles two lines;
+// w/2 + 1 elsewhere (transposes and short-column strips take A packed).  A
+// plane holds h x w complex values, so the padded pitch is used only when it
+// fits in w.
+int half_plane_pitch(int h, int w) {
+    const int wh = w / 2 + 1, padded = (wh + 15) & ~15;
+    return (h == 1024 || h == 1536) && padded <= w ? padded : wh;
+}
+
 // Wiener filter of nb images (img, img_stride floats apart) into out, on
 // half spectra: row FFTs two rows at a time, column FFTs (+ |F|^2) over the
 // w/2+1 columns, the variance stencil on the full grid (mirrored reads of
@@ -1203,16 +1218,17 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
     const int pairs = (h + 1) / 2;
+    const int lda = half_plane_pitch(h, w);
     // (the two-wave 2048-point form measured slower here, 331 -> 365 us per
     // 32 frames of 2048x1536: four loads per value and FP64 sums at 4 waves
     // per SIMD; it pays in rows_c2r_kernel)
     if (w == 2048)
         hipLaunchKernelGGL((wiener_rows_fwd_kernel<kFftSmallN, true>), dim3(pairs, nb), dim3(kFftThreads),
                            sizeof(float2) * kFft2048PadLds, st, dw, img, img_stride, h, e.A, n, e.rowsq,
-                           zm ? e.cz : nullptr, zm ? e.rz : nullptr);
+                           zm ? e.cz : nullptr, zm ? e.rz : nullptr, lda);
     else
         PRNU_LAUNCH_ROW(wiener_rows_fwd_kernel, w, dim3(pairs, nb), st, dw, img, img_stride, h, e.A, n, e.rowsq,
-                        zm ? e.cz : nullptr, zm ? e.rz : nullptr);
+                        zm ? e.cz : nullptr, zm ? e.rz : nullptr, lda);
     PRNU_CHECK_LAUNCH();
     hipLaunchKernelGGL(wiener_var_kernel, dim3(nb), dim3(256), 0, st, e.rowsq, h, n, e.var);
     PRNU_CHECK_LAUNCH();
@@ -1226,10 +1242,10 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
         const int per = ((wh + 7) / 8 + 7) / 8;
         if (h == 1024) {
             hipLaunchKernelGGL(cols_fft_power_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A,
-                               wh, e.B, e.P, n, per);
+                               wh, e.B, e.P, n, per, lda);
         } else if (h == 1536) {
             hipLaunchKernelGGL(cols_fft_power_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A,
-                               wh, e.B, e.P, n, per);
+                               wh, e.B, e.P, n, per, lda);
         } else {
             PRNU_TRY(launch_transpose_c(e.A, e.B, h, wh, nb, n, st));
             PRNU_LAUNCH_ROW(rows_fft_power_kernel, h, dim3(wh, nb), st, dh, e.B, e.P, n);
@@ -1238,10 +1254,10 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
         PRNU_TRY(launch_varest(e.P, e.S, w, h, true, nb, e.var, st));
         if (h == 1024) {
             hipLaunchKernelGGL(cols_ifft_scaled_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.B,
-                               e.S, n, w, wh, e.A, per);
+                               e.S, n, w, wh, e.A, per, lda);
         } else if (h == 1536) {
             hipLaunchKernelGGL(cols_ifft_scaled_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.B,
-                               e.S, n, w, wh, e.A, per);
+                               e.S, n, w, wh, e.A, per, lda);
         } else {
             PRNU_LAUNCH_ROW(rows_ifft_scaled_kernel, h, dim3(wh, nb), st, dh, e.B, e.S, n, w);
             PRNU_CHECK_LAUNCH();
@@ -1252,10 +1268,10 @@ int launch_wiener(Context *c, ExtractWs &e, int nb, const float *img, int64_t im
     if (w == 2048)
         hipLaunchKernelGGL((rows_c2r_kernel<kFftSmallN, 2048>), dim3(pairs, nb), dim3(128),
                            sizeof(float2) * fft_wave_lds<2048>(), st, dw, e.A, n, h, out, out_stride,
-                           1.0f / (float)((double)w * h));
+                           1.0f / (float)((double)w * h), lda);
     else
         PRNU_LAUNCH_ROW(rows_c2r_kernel, w, dim3(pairs, nb), st, dw, e.A, n, h, out, out_stride,
-                        1.0f / (float)((double)w * h));
+                        1.0f / (float)((double)w * h), lda);
     PRNU_CHECK_LAUNCH();
     return PRNU_OK;
 }
@@ -1269,26 +1285,28 @@ int launch_spectrum(Context *c, ExtractWs &e, int nb, const float *img, int64_t 
     const FftDesc &dw = fft_desc(c->fft_w);
     const FftDesc &dh = fft_desc(c->fft_h);
     if (strips_for(h)) {
-        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, out, out_stride, h);
+        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, out, out_stride, h,
+                        wh);
         PRNU_CHECK_LAUNCH();
         PRNU_TRY((launch_cols_fft<-1, 0>(dh, out, out_stride, wh, wh, nb, nullptr, nullptr, st)));
         return PRNU_OK;
     }
     const int64_t n = c->pixels();
+    const int lda = half_plane_pitch(h, w);
     if (w == 2048)
         hipLaunchKernelGGL((rows_fft_half_kernel<kFftSmallN, true>), dim3((h + 1) / 2, nb), dim3(kFftThreads),
-                           sizeof(float2) * kFft2048PadLds, st, dw, img, img_stride, e.A, n, h);
+                           sizeof(float2) * kFft2048PadLds, st, dw, img, img_stride, e.A, n, h, lda);
     else
-        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h);
+        PRNU_LAUNCH_ROW(rows_fft_half_kernel, w, dim3((h + 1) / 2, nb), st, dw, img, img_stride, e.A, n, h, lda);
     PRNU_CHECK_LAUNCH();
     if (h == 1024 || h == 1536) {  // no transposes: strips of 8 columns through LDS
         const int per = ((wh + 7) / 8 + 7) / 8;
         if (h == 1024)
-            hipLaunchKernelGGL(cols_fft_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A, n, wh,
-                               out, out_stride, per);
+            hipLaunchKernelGGL(spectrum_cols_strip_kernel<1024>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A, n,
+                               wh, out, out_stride, per, lda);
         else
-            hipLaunchKernelGGL(cols_fft_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A, n, wh,
-                               out, out_stride, per);
+            hipLaunchKernelGGL(spectrum_cols_strip_kernel<1536>, dim3(8 * per, nb), dim3(512), 0, st, dh.tw, e.A, n,
+                               wh, out, out_stride, per, lda);
         PRNU_CHECK_LAUNCH();
         return PRNU_OK;
     }
