@@ -743,7 +743,7 @@ __global__ __launch_bounds__(BLOCKN ? 128 : kFftThreads) void rows_c2r_kernel(Ff
     extern __shared__ float2 buf[];
     const int i0 = 2 * blockIdx.x, i1 = i0 + 1;
     const bool two = i1 < h;
-    const int w = d.n, wh = w / 2 + 1;
+    const int w = d.n;
     const float2 *a0 = A + (int64_t)blockIdx.y * plane + (int64_t)i0 * lda;  // rows lda values apart
     float *out = out_all + (int64_t)blockIdx.y * out_stride + (int64_t)i0 * w;
     auto get = [&](int k) {
@@ -858,12 +858,28 @@ __global__ __launch_bounds__(512) void cols_fft_power_strip_kernel(const float2 
     if (strip >= (wh + 7) / 8) return;
     const int j0 = strip * 8;
     const float2 *Af = A + (int64_t)blockIdx.y * plane;
-    const int t = (int)threadIdx.x, c = t & 7, r = t >> 3;
-    if (j0 + c < wh) {
+    const int t = (int)threadIdx.x;
+    {
+        // two adjacent columns per thread, one 16-byte load when lda is even
+        // (j0 is, and the plane is 16-byte aligned); the second column of a
+        // strip's last pair may lie past wh, in the row's pitch or the next
+        // row, inside the plane, and is never stored
+        const int c2 = (t & 3) * 2, r = t >> 2;
+        if (j0 + c2 < wh) {
 #pragma unroll 4
-        for (int k = 0; k < N / 64; ++k) {
-            const int i = r + 64 * k;
-            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * lda + j0 + c];
+            for (int k = 0; k < N / 128; ++k) {
+                const int i = r + 128 * k;
+                const float2 *src = Af + (int64_t)i * lda + j0 + c2;
+                float4 v;
+                if ((lda & 1) == 0) {  // 16-byte aligned pairs
+                    v = *reinterpret_cast<const float4 *>(src);
+                } else {  // an odd (unpadded) pitch: two 8-byte loads
+                    const float2 a = src[0], b = src[1];
+                    v = make_float4(a.x, a.y, b.x, b.y);
+                }
+                lds[c2 * SB + fft_wave_pad<SH>(i)] = make_float2(v.x, v.y);
+                lds[(c2 + 1) * SB + fft_wave_pad<SH>(i)] = make_float2(v.z, v.w);
+            }
         }
     }
     __syncthreads();
@@ -902,11 +918,25 @@ __global__ __launch_bounds__(512) void spectrum_cols_strip_kernel(const float2 *
     const float2 *Af = A + (int64_t)blockIdx.y * plane;
     const int t = (int)threadIdx.x, c = t & 7, r = t >> 3;
     const bool cin = j0 + c < wh;
-    if (cin) {
+    {
+        // two adjacent columns per thread, one 16-byte load (as in
+        // cols_fft_power_strip_kernel)
+        const int c2 = (t & 3) * 2, r2 = t >> 2;
+        if (j0 + c2 < wh) {
 #pragma unroll 4
-        for (int k = 0; k < N / 64; ++k) {
-            const int i = r + 64 * k;
-            lds[c * SB + fft_wave_pad<SH>(i)] = Af[(int64_t)i * lda + j0 + c];
+            for (int k = 0; k < N / 128; ++k) {
+                const int i = r2 + 128 * k;
+                const float2 *src = Af + (int64_t)i * lda + j0 + c2;
+                float4 v;
+                if ((lda & 1) == 0) {  // 16-byte aligned pairs
+                    v = *reinterpret_cast<const float4 *>(src);
+                } else {  // an odd (unpadded) pitch: two 8-byte loads
+                    const float2 a = src[0], b = src[1];
+                    v = make_float4(a.x, a.y, b.x, b.y);
+                }
+                lds[c2 * SB + fft_wave_pad<SH>(i)] = make_float2(v.x, v.y);
+                lds[(c2 + 1) * SB + fft_wave_pad<SH>(i)] = make_float2(v.z, v.w);
+            }
         }
     }
     __syncthreads();

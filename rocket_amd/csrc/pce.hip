@@ -434,11 +434,17 @@ __global__ __launch_bounds__(512) void pce_cols_strip_kernel(const float2 *__res
     const int q = blockIdx.y;
     const int j0 = strip * 8;
     const float2 *A = Aall + (int64_t)q * pstride;
-    const int t = (int)threadIdx.x, c = t & 7, r = t >> 3;
+    const int t = (int)threadIdx.x;
+    {
+        // two adjacent columns per thread, one 16-byte load (w and j0 even)
+        const int c2 = (t & 3) * 2, r = t >> 2;
 #pragma unroll 4
-    for (int k = 0; k < 1024 / 64; ++k) {
-        const int i = r + 64 * k;
-        lds[c * kStripBuf + fft_wave_pad<4>(i)] = A[(int64_t)i * w + j0 + c];
+        for (int k = 0; k < 1024 / 128; ++k) {
+            const int i = r + 128 * k;
+            const float4 v = *reinterpret_cast<const float4 *>(A + (int64_t)i * w + j0 + c2);
+            lds[c2 * kStripBuf + fft_wave_pad<4>(i)] = make_float2(v.x, v.y);
+            lds[(c2 + 1) * kStripBuf + fft_wave_pad<4>(i)] = make_float2(v.z, v.w);
+        }
     }
     __syncthreads();
     const int wave = t >> 6;
